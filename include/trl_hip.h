@@ -1,0 +1,165 @@
+/*
+ * trl_hip.h — C-ABI of libtrl_hip.so, the MI355X (gfx950) kernel library
+ * behind the trl::hip block-Lanczos backend.
+ *
+ * Every entry point replaces one operation of trl's backend concepts
+ * (include/trl/concepts.hh in the reference) or of its OpenMP implementation
+ * (the include/trl/impl/openmp headers); the reference location is cited beside each
+ * declaration.  Plain pointers and sizes only: no torch or C++ types cross.
+ *
+ * Conventions
+ *  - A "panel" is an n x b block of a multivector, row-major: element (r, c)
+ *    at offset r*b + c (impl/openmp/multivector.hh:88).  Block j of a
+ *    multivector starts j*n*b elements after block 0.
+ *  - Small b x b blocks ("coefficient blocks", the MatrixBlockView of the
+ *    concepts) are row-major FP64 in device memory for both panel types:
+ *    the FP32 form keeps every coefficient in FP64.
+ *  - dtype selects the panel scalar (FP32 or FP64).  CSR values, diagonal
+ *    scalings and mass diagonals have the panel dtype; indices are int32.
+ *  - b must be one of 1, 2, 4, 6, 8, 16.
+ *  - All calls are ordered on the context's single stream and return as soon
+ *    as the work is queued, except those documented as synchronous.
+ *  - Return value: 0 = success, >0 = hipError_t, <0 = TRL_HIP_E* below.
+ *    A Cholesky breakdown inside a queued orthonormalization cannot be
+ *    returned by the call that queued it; it is recorded in a sticky device
+ *    flag and reported as TRL_HIP_ECHOL by the next trl_hip_ctx_check().
+ *  - One context per host thread; a context is not thread-safe.
+ */
+#ifndef TRL_HIP_H
+#define TRL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRL_HIP_OK 0
+#define TRL_HIP_EINVAL (-1) /* bad argument (std::invalid_argument in the reference) */
+#define TRL_HIP_ECHOL (-2)  /* Cholesky breakdown (evp_base.hh:55 runtime_error)     */
+#define TRL_HIP_ENOMEM (-3) /* device allocation failed                               */
+#define TRL_HIP_EEIG (-4)   /* dense eigensolver did not converge (evp_base.hh:96)   */
+#define TRL_HIP_ERANGE (-5) /* block index out of range (multivector.hh:87)           */
+
+typedef enum { TRL_HIP_F32 = 0, TRL_HIP_F64 = 1 } trl_hip_dtype;
+
+typedef struct trl_hip_ctx trl_hip_ctx;
+
+/* ---- context, memory ---------------------------------------------------- */
+
+/* Creates a context on `device` with one in-order stream (the in-order SYCL
+ * queue of impl/sycl/blockview.hh:19-22 is the reference's device analogue). */
+int trl_hip_ctx_create(int device, trl_hip_ctx** out);
+int trl_hip_ctx_destroy(trl_hip_ctx* ctx);
+/* Waits for all queued work. */
+int trl_hip_ctx_sync(trl_hip_ctx* ctx);
+/* Waits, then reads and clears the sticky Cholesky flag.  Returns
+ * TRL_HIP_ECHOL (and *pivot = failing pivot index, 1-based) if any
+ * orthonormalization since the last check broke down. */
+int trl_hip_ctx_check(trl_hip_ctx* ctx, int* pivot);
+/* The context's hipStream_t, for callers that interleave their own work. */
+void* trl_hip_ctx_stream(trl_hip_ctx* ctx);
+/* Number of compute units of the context's device. */
+int trl_hip_ctx_num_cus(trl_hip_ctx* ctx);
+const char* trl_hip_strerror(int code);
+
+/* Zero-initialised device allocation (multivector.hh:44-48 zero-fills). */
+int trl_hip_malloc(trl_hip_ctx* ctx, size_t bytes, void** out);
+int trl_hip_free(trl_hip_ctx* ctx, void* p);
+/* Synchronous with respect to the host: return after the copy. */
+int trl_hip_memcpy_h2d(trl_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int trl_hip_memcpy_d2h(trl_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Queued device-to-device copy: BlockView::copy_from (blockview.hh:59-63). */
+int trl_hip_memcpy_d2d(trl_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Queued zero fill by a kernel (no memset node, so it is safe inside a
+ * captured graph): BlockView::set_zero (blockview.hh:31-36). */
+int trl_hip_fill_zero(trl_hip_ctx* ctx, void* dst, size_t bytes);
+/* Element conversion between panel dtypes (count elements). */
+int trl_hip_convert(trl_hip_ctx* ctx, trl_hip_dtype src_t, const void* src, trl_hip_dtype dst_t, void* dst, size_t count);
+
+/* ---- panel operations (BlockVectorView, concepts.hh:85-106) ------------- */
+
+/* X -= Y over n*b elements: BlockView::operator-= (blockview.hh:69-74). */
+int trl_hip_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const void* Y);
+
+#define TRL_HIP_MULT 0           /* Y  = X * W   : BlockView::mult (blockview.hh:104-119)            */
+#define TRL_HIP_MULT_ADD 1       /* Y += X * W   : BlockView::mult_add (blockview.hh:81-97)          */
+#define TRL_HIP_MULT_TRANSPOSE 2 /* Y  = X * W^T : BlockView::mult_transpose (blockview.hh:126-141)  */
+#define TRL_HIP_SUBTRACT_PRODUCT 3 /* X -= Y * W : BlockView::subtract_product (blockview.hh:189-201) */
+/* W: b x b FP64 device block. */
+int trl_hip_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const double* W, void* Y, int mode);
+
+/* ---- block products (the Gram / update contractions) -------------------- */
+
+/* C = V[0:k]^T diag(m) W, C is (k*b) x b FP64 on the device, block j of C
+ * (rows j*b..j*b+b-1) = V_j^T diag(m) W.  mdiag may be NULL (m = 1).
+ * k = 1 is BlockView::dot / EVPBase::dot (blockview.hh:152-182,
+ * evp_base.hh:37); mdiag != NULL is the B-inner product of
+ * CSRGeneralizedEVP::dot (examples/openmp/csrevp.hh:89-97) with a diagonal B.
+ * k > 1 is the fused form of the ModifiedGS dot loop
+ * (reorthogonalization.hh:14-21). */
+int trl_hip_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* mdiag, double* C);
+
+/* W -= sum_{j<k} V_j * C_j with C as returned by trl_hip_gram: the fused
+ * form of the subtract_product loop of ModifiedGS (reorthogonalization.hh:19). */
+int trl_hip_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W);
+
+/* Cholesky-QR of one panel: W <- W R^{-1}, R upper triangular with positive
+ * diagonal and R^T R = W^T diag(m) W, so that W_old = W_new R
+ * (concepts.hh:153-155; EVPBase::orthonormalize, evp_base.hh:39-71).
+ * The b x b Cholesky runs in FP64 on the device.  passes = 1 is the
+ * reference's single CholQR; passes = 2 repeats it on the result and returns
+ * R = R2 R1 (CholQR2).  R: b x b FP64 device block.  On breakdown W is left
+ * unchanged and the sticky flag is set (see trl_hip_ctx_check). */
+int trl_hip_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* mdiag, double* R, int passes);
+
+/* Thick-restart Ritz rotation (lanczos.hh:105-114 as one contraction):
+ * Out_j = sum_{i<m} V_i * Y(i, j) for j < kc, where Y is a FP64 device block
+ * matrix in BlockMatrix layout (block (i, j) at Y + (i*y_block_cols + j)*b*b,
+ * impl/openmp/blockmatrix.hh:52-58).  Out holds kc panels and must not
+ * overlap V. */
+int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int y_block_cols, int kc, void* Out);
+
+/* ---- operator ------------------------------------------------------------ */
+
+/* Y = diag(dscale) A X for a CSR matrix A (n x n, int32 rowptr[n+1] and
+ * col[nnz], values of the panel dtype): EVP::apply of CSREVP
+ * (examples/openmp/csrevp.hh:36-44; SYCL kernel examples/sycl/csrevp.hh:159-187).
+ * dscale may be NULL; dscale = 1/diag(M) applies M^{-1}K for a generalized
+ * problem with a diagonal M.  lanes_per_row: 0 = tuned default, else one of
+ * 1, 2, 4, 8, 16 lanes of a wavefront cooperating on one row. */
+int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
+                     const void* dscale, int lanes_per_row);
+
+/* The same product for a matrix in sliced-ELL form: slot s of row r at
+ * index s*n + r of ell_col / ell_val (padding slots: col = r, val = 0). */
+int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* ell_col, const void* ell_val, const void* X, void* Y,
+                     const void* dscale);
+
+/* ---- small b x b blocks (MatrixBlockViewConcept, concepts.hh:26-39) ----- */
+
+/* dst = src (transpose = 0) or src^T (transpose = 1):
+ * BlockMatrixBlockView::copy_from / copy_from_transpose (matrixblockview.hh:30-43). */
+int trl_hip_small_copy(trl_hip_ctx* ctx, int b, const double* src, double* dst, int transpose);
+/* dst = 0: BlockMatrixBlockView::set_zero (matrixblockview.hh:49). */
+int trl_hip_small_zero(trl_hip_ctx* ctx, int b, double* dst);
+/* diag(dst) = host_values[0..b): set_diagonal (matrixblockview.hh:71-74). */
+int trl_hip_small_set_diagonal(trl_hip_ctx* ctx, int b, double* dst, const double* host_values);
+/* C = A B: BlockMatrixBlockView::mult (matrixblockview.hh:55-64). */
+int trl_hip_small_mult(trl_hip_ctx* ctx, int b, const double* A, const double* B, double* C);
+
+/* ---- checks -------------------------------------------------------------- */
+
+/* Synchronous.  For the b columns c of panel X and of Y:
+ * out[c] = || Y[:,c] - lambda[c] * diag(m) X[:,c] ||_2^2 and
+ * out[b + c] = || Y[:,c] ||_2^2, accumulated in FP64.  lambda: host array of b
+ * values (may be NULL: lambda = 0).  mdiag may be NULL.  Used for the true
+ * residual of Ritz pairs (Y = A X). */
+int trl_hip_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* mdiag, const double* lambda, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRL_HIP_H */

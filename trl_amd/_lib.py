@@ -1,0 +1,185 @@
+"""ctypes binding of the C-ABI in include/trl_hip.h and include/trl_hip_solver.h.
+
+Loading fails loudly: there is no CPU fallback for any operation of this
+package.  The libraries are the in-tree builds under trl_amd/lib/.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+
+F32 = 0
+F64 = 1
+
+c_size = C.c_size_t
+c_int = C.c_int
+c_void_p = C.c_void_p
+c_double_p = C.POINTER(C.c_double)
+c_int_p = C.POINTER(C.c_int)
+
+# (name, restype, argtypes) of every symbol declared in include/trl_hip.h
+HIP_API = [
+    ("trl_hip_ctx_create", c_int, [c_int, C.POINTER(c_void_p)]),
+    ("trl_hip_ctx_destroy", c_int, [c_void_p]),
+    ("trl_hip_ctx_sync", c_int, [c_void_p]),
+    ("trl_hip_ctx_check", c_int, [c_void_p, c_int_p]),
+    ("trl_hip_ctx_stream", c_void_p, [c_void_p]),
+    ("trl_hip_ctx_num_cus", c_int, [c_void_p]),
+    ("trl_hip_strerror", C.c_char_p, [c_int]),
+    ("trl_hip_malloc", c_int, [c_void_p, c_size, C.POINTER(c_void_p)]),
+    ("trl_hip_free", c_int, [c_void_p, c_void_p]),
+    ("trl_hip_memcpy_h2d", c_int, [c_void_p, c_void_p, c_void_p, c_size]),
+    ("trl_hip_memcpy_d2h", c_int, [c_void_p, c_void_p, c_void_p, c_size]),
+    ("trl_hip_memcpy_d2d", c_int, [c_void_p, c_void_p, c_void_p, c_size]),
+    ("trl_hip_fill_zero", c_int, [c_void_p, c_void_p, c_size]),
+    ("trl_hip_convert", c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_size]),
+    ("trl_hip_panel_sub", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p]),
+    ("trl_hip_panel_mult", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
+    ("trl_hip_gram", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_update", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_cholqr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
+    ("trl_hip_rotate", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    ("trl_hip_spmm_csr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    ("trl_hip_spmm_ell", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_small_copy", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
+    ("trl_hip_small_zero", c_int, [c_void_p, c_int, c_void_p]),
+    ("trl_hip_small_set_diagonal", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    ("trl_hip_small_mult", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_residual_sq", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_sddmm_adjoint", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+]
+
+_hip = None
+
+
+class TrlHipError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        self.code = code
+        super().__init__(f"{what}: {strerror(code)} (code {code})")
+
+
+def _load(name: str) -> C.CDLL:
+    path = LIB_DIR / name
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: build the HIP extension first (python -m trl_amd.build); there is no CPU fallback")
+    return C.CDLL(str(path), mode=os.RTLD_NOW | C.RTLD_GLOBAL)
+
+
+def hip() -> C.CDLL:
+    global _hip
+    if _hip is None:
+        lib = _load("libtrl_hip.so")
+        for name, res, args in HIP_API:
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _hip = lib
+    return _hip
+
+
+def strerror(code: int) -> str:
+    try:
+        return hip().trl_hip_strerror(code).decode()
+    except Exception:  # library missing
+        return f"error {code}"
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise TrlHipError(rc, what)
+
+
+def dtype_code(dt) -> int:
+    dt = np.dtype(dt)
+    if dt == np.float32:
+        return F32
+    if dt == np.float64:
+        return F64
+    raise ValueError(f"unsupported dtype {dt}")
+
+
+class DeviceArray:
+    """A device allocation with a numpy-like shape/dtype (row-major)."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = c_void_p()
+        check(hip().trl_hip_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)), "trl_hip_malloc")
+        self.ptr = p.value
+
+    def offset(self, elements: int) -> int:
+        return self.ptr + int(elements) * self.dtype.itemsize
+
+    def upload(self, host: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(host, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        check(hip().trl_hip_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "h2d")
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(hip().trl_hip_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes), "d2h")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            hip().trl_hip_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device + one in-order stream (trl_hip_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = c_void_p()
+        check(hip().trl_hip_ctx_create(device, C.byref(h)), "trl_hip_ctx_create")
+        self.h = h.value
+
+    def close(self) -> None:
+        if self.h:
+            hip().trl_hip_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self) -> None:
+        check(hip().trl_hip_ctx_sync(self.h), "sync")
+
+    def check(self) -> int:
+        piv = c_int(0)
+        rc = hip().trl_hip_ctx_check(self.h, C.byref(piv))
+        if rc not in (0, -2):
+            check(rc, "check")
+        return piv.value
+
+    @property
+    def num_cus(self) -> int:
+        return hip().trl_hip_ctx_num_cus(self.h)
+
+    def empty(self, shape, dtype=np.float64) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, host: np.ndarray, dtype=None) -> DeviceArray:
+        host = np.asarray(host)
+        d = DeviceArray(self, host.shape, dtype or host.dtype)
+        return d.upload(host)

@@ -1,0 +1,282 @@
+// C-ABI entry points of libtrl_hip.so: argument checks, context and memory
+// management, dispatch to the kernel families.
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "common.hh"
+
+using namespace trl_hip_impl;
+
+namespace trl_hip_impl {
+
+int ensure_partials(trl_hip_ctx* ctx, size_t doubles)
+{
+  if (doubles <= ctx->partials_cap) return 0;
+  size_t cap = ctx->partials_cap ? ctx->partials_cap : (size_t(1) << 16);
+  while (cap < doubles) cap *= 2;
+  // the stream may still use the old buffer
+  TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ctx->partials) TRL_HIP_TRY(hipFree(ctx->partials));
+  ctx->partials = nullptr;
+  ctx->partials_cap = 0;
+  if (hipMalloc(&ctx->partials, cap * sizeof(double)) != hipSuccess) return TRL_HIP_ENOMEM;
+  ctx->partials_cap = cap;
+  return 0;
+}
+
+} // namespace trl_hip_impl
+
+extern "C" {
+
+const char* trl_hip_strerror(int code)
+{
+  switch (code) {
+  case TRL_HIP_OK:
+    return "success";
+  case TRL_HIP_EINVAL:
+    return "invalid argument";
+  case TRL_HIP_ECHOL:
+    return "Cholesky factorization failed in orthonormalize";
+  case TRL_HIP_ENOMEM:
+    return "device allocation failed";
+  case TRL_HIP_EEIG:
+    return "Eigendecomposition failed";
+  case TRL_HIP_ERANGE:
+    return "block index out of range";
+  default:
+    if (code > 0) return hipGetErrorString(static_cast<hipError_t>(code));
+    return "unknown trl_hip error";
+  }
+}
+
+int trl_hip_ctx_create(int device, trl_hip_ctx** out)
+{
+  if (!out) return TRL_HIP_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  TRL_HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return TRL_HIP_EINVAL;
+  TRL_HIP_TRY(hipSetDevice(device));
+  auto* ctx = new (std::nothrow) trl_hip_ctx();
+  if (!ctx) return TRL_HIP_ENOMEM;
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  // default-priority, non-blocking in-order stream
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&ctx->flags, 64 * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(ctx->flags, 0, 64 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&ctx->small, kSmallScratch * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(ctx->small, 0, kSmallScratch * sizeof(double));
+  if (e != hipSuccess) {
+    trl_hip_ctx_destroy(ctx);
+    return static_cast<int>(e);
+  }
+  int rc = ensure_partials(ctx, size_t(1) << 18);
+  if (rc) {
+    trl_hip_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return 0;
+}
+
+int trl_hip_ctx_destroy(trl_hip_ctx* ctx)
+{
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->partials) (void)hipFree(ctx->partials);
+  if (ctx->flags) (void)hipFree(ctx->flags);
+  if (ctx->small) (void)hipFree(ctx->small);
+  if (ctx->coef) (void)hipFree(ctx->coef);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+int trl_hip_ctx_sync(trl_hip_ctx* ctx)
+{
+  if (!ctx) return TRL_HIP_EINVAL;
+  TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int trl_hip_ctx_check(trl_hip_ctx* ctx, int* pivot)
+{
+  if (!ctx) return TRL_HIP_EINVAL;
+  int flag = 0;
+  TRL_HIP_TRY(hipMemcpyAsync(&flag, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (pivot) *pivot = flag;
+  if (flag != 0) {
+    int zero = 0;
+    TRL_HIP_TRY(hipMemcpyAsync(ctx->flags, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return TRL_HIP_ECHOL;
+  }
+  return 0;
+}
+
+void* trl_hip_ctx_stream(trl_hip_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int trl_hip_ctx_num_cus(trl_hip_ctx* ctx) { return ctx ? ctx->num_cus : 0; }
+
+int trl_hip_malloc(trl_hip_ctx* ctx, size_t bytes, void** out)
+{
+  if (!ctx || !out) return TRL_HIP_EINVAL;
+  *out = nullptr;
+  if (bytes == 0) bytes = 1;
+  const size_t aligned = (bytes + 255) & ~size_t(255);
+  void* p = nullptr;
+  if (hipMalloc(&p, aligned) != hipSuccess) return TRL_HIP_ENOMEM;
+  int rc = launch_fill_zero(ctx, p, aligned);
+  if (rc) {
+    (void)hipFree(p);
+    return rc;
+  }
+  *out = p;
+  return 0;
+}
+
+int trl_hip_free(trl_hip_ctx* ctx, void* p)
+{
+  if (!ctx) return TRL_HIP_EINVAL;
+  if (!p) return 0;
+  TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  TRL_HIP_TRY(hipFree(p));
+  return 0;
+}
+
+int trl_hip_memcpy_h2d(trl_hip_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+  if (!ctx || (bytes && (!dst || !src))) return TRL_HIP_EINVAL;
+  if (!bytes) return 0;
+  TRL_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int trl_hip_memcpy_d2h(trl_hip_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+  if (!ctx || (bytes && (!dst || !src))) return TRL_HIP_EINVAL;
+  if (!bytes) return 0;
+  TRL_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int trl_hip_memcpy_d2d(trl_hip_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+  if (!ctx || (bytes && (!dst || !src))) return TRL_HIP_EINVAL;
+  if (!bytes || dst == src) return 0;
+  TRL_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return 0;
+}
+
+int trl_hip_fill_zero(trl_hip_ctx* ctx, void* dst, size_t bytes)
+{
+  if (!ctx || (bytes && !dst)) return TRL_HIP_EINVAL;
+  if (!bytes) return 0;
+  return launch_fill_zero(ctx, dst, bytes);
+}
+
+int trl_hip_convert(trl_hip_ctx* ctx, trl_hip_dtype st, const void* src, trl_hip_dtype dt, void* dst, size_t count)
+{
+  if (!ctx || (count && (!src || !dst))) return TRL_HIP_EINVAL;
+  if (!count) return 0;
+  return launch_convert(ctx, st, src, dt, dst, count);
+}
+
+int trl_hip_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const void* Y)
+{
+  if (!ctx || !valid_b(b) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  return launch_panel_sub(ctx, t, n, b, X, Y);
+}
+
+int trl_hip_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const double* W, void* Y, int mode)
+{
+  if (!ctx || !valid_b(b) || mode < 0 || mode > 3 || (n && (!X || !W || !Y))) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  return launch_panel_mult(ctx, t, n, b, X, W, Y, mode);
+}
+
+int trl_hip_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* mdiag, double* C)
+{
+  if (!ctx || !valid_b(b) || k < 1 || !C || (n && (!V || !W))) return TRL_HIP_EINVAL;
+  return launch_gram(ctx, t, n, b, k, V, W, mdiag, C);
+}
+
+int trl_hip_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W)
+{
+  if (!ctx || !valid_b(b) || k < 1 || !C || (n && (!V || !W))) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  return launch_update(ctx, t, n, b, k, V, C, W);
+}
+
+int trl_hip_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* mdiag, double* R, int passes)
+{
+  if (!ctx || !valid_b(b) || !R || passes < 1 || passes > 2 || (n && !W)) return TRL_HIP_EINVAL;
+  return launch_cholqr(ctx, t, n, b, W, mdiag, R, passes);
+}
+
+int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int y_block_cols, int kc, void* Out)
+{
+  if (!ctx || !valid_b(b) || m < 1 || kc < 1 || y_block_cols < kc || !Y || (n && (!V || !Out))) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  return launch_rotate(ctx, t, n, b, m, V, Y, y_block_cols, kc, Out);
+}
+
+int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
+                     const void* dscale, int lanes_per_row)
+{
+  if (!ctx || !valid_b(b) || (n && (!rowptr || !X || !Y))) return TRL_HIP_EINVAL;
+  if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16) return TRL_HIP_EINVAL;
+  if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  return launch_spmm_csr(ctx, t, n, b, rowptr, col, val, X, Y, dscale, lanes_per_row);
+}
+
+int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* ell_col, const void* ell_val, const void* X, void* Y,
+                     const void* dscale)
+{
+  if (!ctx || !valid_b(b) || width < 0 || (n && (!X || !Y)) || (n && width && (!ell_col || !ell_val))) return TRL_HIP_EINVAL;
+  if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  return launch_spmm_ell(ctx, t, n, b, width, ell_col, ell_val, X, Y, dscale);
+}
+
+int trl_hip_small_copy(trl_hip_ctx* ctx, int b, const double* src, double* dst, int transpose)
+{
+  if (!ctx || b < 1 || b > 16 || !src || !dst) return TRL_HIP_EINVAL;
+  if (!transpose) return trl_hip_memcpy_d2d(ctx, dst, src, size_t(b) * b * sizeof(double));
+  return launch_small(ctx, 1, b, src, nullptr, dst, nullptr);
+}
+
+int trl_hip_small_zero(trl_hip_ctx* ctx, int b, double* dst)
+{
+  if (!ctx || b < 1 || b > 16 || !dst) return TRL_HIP_EINVAL;
+  return launch_small(ctx, 0, b, nullptr, nullptr, dst, nullptr);
+}
+
+int trl_hip_small_set_diagonal(trl_hip_ctx* ctx, int b, double* dst, const double* host_values)
+{
+  if (!ctx || b < 1 || b > 16 || !dst || !host_values) return TRL_HIP_EINVAL;
+  return launch_small(ctx, 2, b, nullptr, nullptr, dst, host_values);
+}
+
+int trl_hip_small_mult(trl_hip_ctx* ctx, int b, const double* A, const double* B, double* C)
+{
+  if (!ctx || b < 1 || b > 16 || !A || !B || !C) return TRL_HIP_EINVAL;
+  return launch_small(ctx, 3, b, A, B, C, nullptr);
+}
+
+int trl_hip_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* mdiag, const double* lambda, double* out)
+{
+  if (!ctx || !valid_b(b) || !out || (n && (!X || !Y))) return TRL_HIP_EINVAL;
+  return launch_residual_sq(ctx, t, n, b, X, Y, mdiag, lambda, out);
+}
+
+} // extern "C"

@@ -1,0 +1,190 @@
+// Internal declarations shared by the libtrl_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "trl_hip.h"
+
+// Context: one device, one in-order stream, and the scratch the kernels need
+// for their two-level reductions.
+struct trl_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+  // per-workgroup partial sums of the block products (FP64)
+  double* partials = nullptr;
+  size_t partials_cap = 0; // in doubles
+  // flags[0]: sticky Cholesky breakdown pivot (1-based, 0 = none)
+  int* flags = nullptr;
+  // small FP64 scratch: R^{-1}, R of the first CholQR pass, coefficient staging
+  double* small = nullptr;
+  // coefficient scratch for multi-block products (grown on demand)
+  double* coef = nullptr;
+  size_t coef_cap = 0;
+};
+
+namespace trl_hip_impl {
+
+constexpr int kSmallScratch = 4 * 256; // doubles in ctx->small
+
+// Grows ctx->partials to hold at least `doubles` entries.
+int ensure_partials(trl_hip_ctx* ctx, size_t doubles);
+
+inline bool valid_b(int b) { return b == 1 || b == 2 || b == 4 || b == 6 || b == 8 || b == 16; }
+
+// launch wrappers (one translation unit each family)
+int launch_fill_zero(trl_hip_ctx* ctx, void* dst, size_t bytes);
+int launch_convert(trl_hip_ctx* ctx, trl_hip_dtype st, const void* src, trl_hip_dtype dt, void* dst, size_t count);
+int launch_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const void* Y);
+int launch_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const double* W, void* Y, int mode);
+int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* md, double* C);
+int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W);
+int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes);
+int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int ybc, int kc, void* Out);
+int launch_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
+                    const void* dscale, int lanes);
+int launch_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* col, const void* val, const void* X, void* Y, const void* dscale);
+int launch_small(trl_hip_ctx* ctx, int op, int b, const double* A, const double* B, double* C, const double* hostvals);
+int launch_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* md, const double* lambda, double* out);
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? 0 : static_cast<int>(e); }
+
+} // namespace trl_hip_impl
+
+#define TRL_HIP_TRY(expr)                                                                                                                            \
+  do {                                                                                                                                               \
+    hipError_t e_ = (expr);                                                                                                                          \
+    if (e_ != hipSuccess) return static_cast<int>(e_);                                                                                               \
+  } while (0)
+
+#define TRL_HIP_LAUNCH_CHECK() TRL_HIP_TRY(hipGetLastError())
+
+// Runtime block size -> compile-time constant B.
+#define TRL_DISPATCH_B(bval, ...)                                                                                                                    \
+  switch (bval) {                                                                                                                                    \
+  case 1: {                                                                                                                                          \
+    constexpr int B = 1;                                                                                                                             \
+    __VA_ARGS__;                                                                                                                                     \
+  } break;                                                                                                                                           \
+  case 2: {                                                                                                                                          \
+    constexpr int B = 2;                                                                                                                             \
+    __VA_ARGS__;                                                                                                                                     \
+  } break;                                                                                                                                           \
+  case 4: {                                                                                                                                          \
+    constexpr int B = 4;                                                                                                                             \
+    __VA_ARGS__;                                                                                                                                     \
+  } break;                                                                                                                                           \
+  case 6: {                                                                                                                                          \
+    constexpr int B = 6;                                                                                                                             \
+    __VA_ARGS__;                                                                                                                                     \
+  } break;                                                                                                                                           \
+  case 8: {                                                                                                                                          \
+    constexpr int B = 8;                                                                                                                             \
+    __VA_ARGS__;                                                                                                                                     \
+  } break;                                                                                                                                           \
+  case 16: {                                                                                                                                         \
+    constexpr int B = 16;                                                                                                                            \
+    __VA_ARGS__;                                                                                                                                     \
+  } break;                                                                                                                                           \
+  default:                                                                                                                                           \
+    return TRL_HIP_EINVAL;                                                                                                                           \
+  }
+
+#define TRL_DISPATCH_T(tval, ...)                                                                                                                    \
+  if ((tval) == TRL_HIP_F32) {                                                                                                                       \
+    using S = float;                                                                                                                                 \
+    __VA_ARGS__;                                                                                                                                     \
+  }                                                                                                                                                  \
+  else if ((tval) == TRL_HIP_F64) {                                                                                                                  \
+    using S = double;                                                                                                                                \
+    __VA_ARGS__;                                                                                                                                     \
+  }                                                                                                                                                  \
+  else                                                                                                                                               \
+    return TRL_HIP_EINVAL;
+
+// ---------------------------------------------------------------------------
+// device helpers
+namespace trl_hip_dev {
+
+// Loads/stores of one panel row (B contiguous scalars) with the widest
+// aligned vector the row allows (rows start at r*B*sizeof(S)).
+template <typename S, int B>
+struct RowIO {
+  static constexpr int bytes = B * int(sizeof(S));
+  static constexpr int vec = (bytes % 16 == 0) ? 16 : (bytes % 8 == 0) ? 8 : int(sizeof(S));
+  static constexpr int per = vec / int(sizeof(S)); // scalars per vector
+  static constexpr int nv = B / per;
+  using V16 = __attribute__((ext_vector_type(16 / sizeof(S)))) S;
+
+  __device__ static inline void load(const S* __restrict__ p, S (&v)[B])
+  {
+    if constexpr (vec == 16) {
+      using VT = __attribute__((ext_vector_type(16 / sizeof(S)))) S;
+      const VT* q = reinterpret_cast<const VT*>(p);
+#pragma unroll
+      for (int i = 0; i < nv; ++i) {
+        VT t = q[i];
+#pragma unroll
+        for (int e = 0; e < per; ++e) v[i * per + e] = t[e];
+      }
+    }
+    else if constexpr (vec == 8 && per == 2) {
+      using VT = __attribute__((ext_vector_type(2))) S;
+      const VT* q = reinterpret_cast<const VT*>(p);
+#pragma unroll
+      for (int i = 0; i < nv; ++i) {
+        VT t = q[i];
+        v[2 * i] = t[0];
+        v[2 * i + 1] = t[1];
+      }
+    }
+    else {
+#pragma unroll
+      for (int i = 0; i < B; ++i) v[i] = p[i];
+    }
+  }
+
+  __device__ static inline void store(S* __restrict__ p, const S (&v)[B])
+  {
+    if constexpr (vec == 16) {
+      using VT = __attribute__((ext_vector_type(16 / sizeof(S)))) S;
+      VT* q = reinterpret_cast<VT*>(p);
+#pragma unroll
+      for (int i = 0; i < nv; ++i) {
+        VT t;
+#pragma unroll
+        for (int e = 0; e < per; ++e) t[e] = v[i * per + e];
+        q[i] = t;
+      }
+    }
+    else if constexpr (vec == 8 && per == 2) {
+      using VT = __attribute__((ext_vector_type(2))) S;
+      VT* q = reinterpret_cast<VT*>(p);
+#pragma unroll
+      for (int i = 0; i < nv; ++i) {
+        VT t;
+        t[0] = v[2 * i];
+        t[1] = v[2 * i + 1];
+        q[i] = t;
+      }
+    }
+    else {
+#pragma unroll
+      for (int i = 0; i < B; ++i) p[i] = v[i];
+    }
+  }
+};
+
+// Error-free transformation a + b = s + e (Knuth's TwoSum).
+template <typename T>
+__device__ inline void two_sum(T a, T b, T& s, T& e)
+{
+  s = a + b;
+  T bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+} // namespace trl_hip_dev

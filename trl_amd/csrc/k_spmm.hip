@@ -1,0 +1,147 @@
+// Sparse operator kernels: Y = diag(d) A X with X, Y n x b row-major panels.
+//
+// CSR: L lanes of a wavefront share one row (L in {1,2,4,8,16}); each lane
+// walks nonzeros beg+sub, beg+sub+L, ... with coalesced column-index and value
+// loads, gathers the b-wide row of X for each nonzero into registers, and the
+// L partial rows are combined by a butterfly (shfl_xor) reduction inside the
+// lane group.  ELL: one row per lane, slot-major arrays (fully coalesced).
+//
+// Workgroups are renumbered so that each XCD walks a contiguous range of rows:
+// the stencil neighbours of a row (r +- 1, r +- nx, ...) then share that XCD's L2.
+#include "common.hh"
+
+using namespace trl_hip_dev;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr unsigned kXcds = 8;
+
+__device__ inline unsigned xcd_block(unsigned bid, unsigned nb)
+{
+  if (nb % kXcds != 0) return bid;
+  return (bid % kXcds) * (nb / kXcds) + bid / kXcds;
+}
+
+template <typename S, int B, int L>
+__global__ __launch_bounds__(kThreads) void k_spmm_csr(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
+                                                       const S* __restrict__ X, S* __restrict__ Y, const S* __restrict__ dscale, int n)
+{
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int gid = int(blk * blockDim.x + threadIdx.x);
+  const int row = gid / L;
+  const int sub = int(threadIdx.x) % L;
+  if (row >= n) return; // whole lane groups leave together
+  const int beg = rowptr[row], end = rowptr[row + 1];
+  S acc[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) acc[c] = S(0);
+  for (int p = beg + sub; p < end; p += L) {
+    const int c = col[p];
+    const S v = val[p];
+    S x[B];
+    RowIO<S, B>::load(X + size_t(c) * B, x);
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
+  }
+  if constexpr (L > 1) {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
+  }
+  if (sub != 0) return;
+  if (dscale) {
+    const S d = dscale[row];
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] *= d;
+  }
+  RowIO<S, B>::store(Y + size_t(row) * B, acc);
+}
+
+template <typename S, int B>
+__global__ __launch_bounds__(kThreads) void k_spmm_ell(int width, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X,
+                                                       S* __restrict__ Y, const S* __restrict__ dscale, int n)
+{
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int row = int(blk * blockDim.x + threadIdx.x);
+  if (row >= n) return;
+  S acc[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) acc[c] = S(0);
+  for (int s = 0; s < width; ++s) {
+    const size_t p = size_t(s) * n + row;
+    const int c = col[p];
+    const S v = val[p];
+    S x[B];
+    RowIO<S, B>::load(X + size_t(c) * B, x);
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
+  }
+  if (dscale) {
+    const S d = dscale[row];
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] *= d;
+  }
+  RowIO<S, B>::store(Y + size_t(row) * B, acc);
+}
+
+template <typename S, int B, int L>
+int launch_csr_l(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* X, S* Y, const S* ds)
+{
+  const size_t threads = n * L;
+  size_t g = (threads + kThreads - 1) / kThreads;
+  // round the grid up to a multiple of the XCD count so the renumbering applies
+  g = (g + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL((k_spmm_csr<S, B, L>), dim3(unsigned(g)), dim3(kThreads), 0, ctx->stream, rowptr, col, val, X, Y, ds, int(n));
+  TRL_HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+} // namespace
+
+namespace trl_hip_impl {
+
+int launch_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
+                    const void* dscale, int lanes)
+{
+  TRL_DISPATCH_T(t, {
+    TRL_DISPATCH_B(b, {
+      const S* v = static_cast<const S*>(val);
+      const S* x = static_cast<const S*>(X);
+      S* y = static_cast<S*>(Y);
+      const S* d = static_cast<const S*>(dscale);
+      int L = lanes;
+      if (L == 0) L = (B >= 8) ? 1 : 4;
+      switch (L) {
+      case 1:
+        return launch_csr_l<S, B, 1>(ctx, n, rowptr, col, v, x, y, d);
+      case 2:
+        return launch_csr_l<S, B, 2>(ctx, n, rowptr, col, v, x, y, d);
+      case 4:
+        return launch_csr_l<S, B, 4>(ctx, n, rowptr, col, v, x, y, d);
+      case 8:
+        return launch_csr_l<S, B, 8>(ctx, n, rowptr, col, v, x, y, d);
+      default:
+        return launch_csr_l<S, B, 16>(ctx, n, rowptr, col, v, x, y, d);
+      }
+    });
+  });
+  return 0;
+}
+
+int launch_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* col, const void* val, const void* X, void* Y, const void* dscale)
+{
+  TRL_DISPATCH_T(t, {
+    TRL_DISPATCH_B(b, {
+      size_t g = (n + kThreads - 1) / kThreads;
+      g = (g + kXcds - 1) / kXcds * kXcds;
+      hipLaunchKernelGGL((k_spmm_ell<S, B>), dim3(unsigned(g)), dim3(kThreads), 0, ctx->stream, width, col, static_cast<const S*>(val),
+                         static_cast<const S*>(X), static_cast<S*>(Y), static_cast<const S*>(dscale), int(n));
+      TRL_HIP_LAUNCH_CHECK();
+    });
+  });
+  return 0;
+}
+
+} // namespace trl_hip_impl
