@@ -1,0 +1,112 @@
+#pragma once
+// Sparse eigenproblems on the HIP backend.
+//
+// CSREVP follows examples/openmp/csrevp.hh:19-48 (Y = A X for a symmetric
+// CSR matrix); with a diagonal mass M it is the generalized problem
+// K x = lambda M x in the form A = M^{-1} K, self-adjoint in the M-inner
+// product (the B-inner product of examples/openmp/csrevp.hh:89-97 with a
+// diagonal B; no factorisation is needed).  Arrays are device arrays of the
+// panel type, either borrowed from the caller or uploaded and owned.
+
+#include <cstdint>
+#include <vector>
+
+#include "evp_base.hh"
+
+namespace trl::hip {
+
+template <class PanelScalar, unsigned int block_size>
+class CSREVP : public EVPBase<PanelScalar, block_size> {
+public:
+  using Base = EVPBase<PanelScalar, block_size>;
+  using BlockView = typename Base::BlockView;
+
+  // Borrows device arrays (rowptr[n+1], col[nnz], val[nnz]; mass and its
+  // reciprocal optional, both or neither).
+  CSREVP(const Context& ctx, std::size_t n, const int32_t* rowptr, const int32_t* col, const PanelScalar* val, const PanelScalar* mass = nullptr,
+         const PanelScalar* mass_inv = nullptr)
+      : Base(ctx, n), rowptr_(rowptr), col_(col), val_(val), minv_(mass_inv)
+  {
+    if ((mass == nullptr) != (mass_inv == nullptr)) throw std::invalid_argument("CSREVP: give both the mass diagonal and its reciprocal, or neither");
+    this->set_mass(mass);
+  }
+
+  // Uploads host CSR arrays (values and mass converted to the panel type).
+  static std::unique_ptr<CSREVP> from_host(const Context& ctx, std::size_t n, const std::vector<int32_t>& rowptr, const std::vector<int32_t>& col,
+                                           const std::vector<double>& val, const std::vector<double>* mass = nullptr)
+  {
+    if (rowptr.size() != n + 1 || col.size() != val.size() || std::size_t(rowptr.back()) != val.size())
+      throw std::invalid_argument("CSREVP::from_host: inconsistent CSR arrays");
+    auto owned = std::make_shared<Owned>(ctx, n, rowptr, col, val, mass);
+    std::unique_ptr<CSREVP> e(new CSREVP(ctx, n, owned->rowptr.data(), owned->col.data(), owned->val.data(), mass ? owned->mass.data() : nullptr,
+                                         mass ? owned->minv.data() : nullptr));
+    e->owned_ = owned;
+    return e;
+  }
+
+  void apply(BlockView X, BlockView Y) override
+  {
+    throw_on(trl_hip_spmm_csr(this->ctx_.get(), Base::dtype, this->N, block_size, rowptr_, col_, val_, X.data_, Y.data_, minv_, lanes_per_row_),
+             "apply");
+  }
+
+  // 0 = tuned default, else 1, 2, 4, 8 or 16 lanes per row
+  void set_lanes_per_row(int l) { lanes_per_row_ = l; }
+
+  const int32_t* rowptr() const { return rowptr_; }
+  const int32_t* col() const { return col_; }
+  const PanelScalar* val() const { return val_; }
+  const PanelScalar* mass_inv() const { return minv_; }
+
+private:
+  struct Owned {
+    DeviceBuffer<int32_t> rowptr, col;
+    DeviceBuffer<PanelScalar> val, mass, minv;
+    Owned(const Context& ctx, std::size_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& c, const std::vector<double>& v,
+          const std::vector<double>* m)
+        : rowptr(ctx, rp.size()), col(ctx, std::max<std::size_t>(c.size(), 1)), val(ctx, std::max<std::size_t>(v.size(), 1))
+    {
+      throw_on(trl_hip_memcpy_h2d(ctx.get(), rowptr.data(), rp.data(), rp.size() * sizeof(int32_t)), "upload rowptr");
+      throw_on(trl_hip_memcpy_h2d(ctx.get(), col.data(), c.data(), c.size() * sizeof(int32_t)), "upload col");
+      std::vector<PanelScalar> vv(v.begin(), v.end());
+      throw_on(trl_hip_memcpy_h2d(ctx.get(), val.data(), vv.data(), vv.size() * sizeof(PanelScalar)), "upload val");
+      if (m) {
+        if (m->size() != n) throw std::invalid_argument("mass size");
+        mass = DeviceBuffer<PanelScalar>(ctx, n);
+        minv = DeviceBuffer<PanelScalar>(ctx, n);
+        std::vector<PanelScalar> mm(n), mi(n);
+        for (std::size_t i = 0; i < n; ++i) {
+          mm[i] = PanelScalar((*m)[i]);
+          mi[i] = PanelScalar(1.0 / (*m)[i]);
+        }
+        throw_on(trl_hip_memcpy_h2d(ctx.get(), mass.data(), mm.data(), n * sizeof(PanelScalar)), "upload mass");
+        throw_on(trl_hip_memcpy_h2d(ctx.get(), minv.data(), mi.data(), n * sizeof(PanelScalar)), "upload mass");
+      }
+    }
+  };
+
+  const int32_t* rowptr_;
+  const int32_t* col_;
+  const PanelScalar* val_;
+  const PanelScalar* minv_;
+  int lanes_per_row_ = 0;
+  std::shared_ptr<Owned> owned_;
+};
+
+// Block classical Gram-Schmidt, two passes, through the reference's
+// Reorth hook (lanczos.hh:311, reorthogonalization.hh:29-32): each pass is
+// one multi-block Gram C = V[0:count]^T M V_next and one multi-block update
+// V_next -= V[0:count] C, in place of ModifiedGS's 2*count single-block
+// calls (reorthogonalization.hh:14-21).
+struct BlockCGS2 {
+  template <typename EVP, typename BMV>
+  void operator()(EVP& evp, BMV& V, unsigned int count, typename BMV::BlockView V_next, typename BMV::BlockMatrix::BlockView /*tmp*/) const
+  {
+    for (int pass = 0; pass < 2; ++pass) {
+      const double* C = evp.gram_blocks(V, count, V_next);
+      evp.update_blocks(V, count, C, V_next);
+    }
+  }
+};
+
+} // namespace trl::hip

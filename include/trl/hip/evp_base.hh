@@ -1,0 +1,211 @@
+#pragma once
+// Eigenproblem base of the HIP backend: the inner product, Cholesky-QR, the
+// Rayleigh-Ritz step and the factories the Eigenproblem concept asks for
+// (include/trl/concepts.hh:157-188), following impl/openmp/evp_base.hh:15-182.
+//
+// Differences from the OpenMP base, all deliberate:
+//  - Scalar (eigenvalues, coefficients, tolerance) is double for both panel
+//    precisions; BlockMultivector::Scalar is the panel type.
+//  - orthonormalize runs Gram, a FP64 Cholesky and the scale on the device;
+//    a breakdown is raised (same exception and message) at the next host
+//    synchronisation instead of inside the call.
+//  - the eigenvalue order is selectable: |lambda| descending (the OpenMP
+//    default, evp_base.hh:103), ascending or descending algebraic (the SYCL
+//    example's EigenvalueOrder, examples/sycl/evp_base.hh:24,110-117).
+//  - an optional diagonal mass M turns dot into the M-inner product
+//    (examples/openmp/csrevp.hh:89-97 with a diagonal B).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <span>
+#include <stdexcept>
+#include <vector>
+
+#include "trl_amd/dense.hh"
+
+#include "blockmatrix.hh"
+#include "context.hh"
+#include "multivector.hh"
+
+namespace trl::hip {
+
+enum class EigenvalueOrder { LargestMagnitude = 0, Ascending = 1, Descending = 2 };
+
+template <class PanelScalar, unsigned int block_size>
+class EVPBase {
+public:
+  using Scalar = double;
+  using PanelType = PanelScalar;
+  constexpr static auto blocksize = block_size;
+
+  using BlockMultivector = ::trl::hip::BlockMultivector<PanelScalar, blocksize>;
+  using BlockView = typename BlockMultivector::BlockView;
+  using BlockMatrix = typename BlockMultivector::BlockMatrix;
+  using BlockMatrixBlockView = typename BlockMatrix::BlockView;
+  static constexpr trl_hip_dtype dtype = dtype_of<PanelScalar>();
+
+  EVPBase(const Context& ctx, std::size_t n) : ctx_(ctx), N(n) {}
+  virtual ~EVPBase() = default;
+
+  virtual void apply(BlockView X, BlockView Y) = 0;
+
+  // R = V^T M W (evp_base.hh:37)
+  virtual void dot(BlockView V, BlockView W, BlockMatrixBlockView R)
+  {
+    throw_on(trl_hip_gram(ctx_.get(), dtype, N, blocksize, 1, V.data_, W.data_, mass_, R.data_), "dot");
+  }
+
+  // V <- V R^{-1}, V_old = V_new R, R upper triangular (evp_base.hh:39-71)
+  virtual void orthonormalize(BlockView V, BlockMatrixBlockView R)
+  {
+    throw_on(trl_hip_cholqr(ctx_.get(), dtype, N, blocksize, V.data_, mass_, R.data_, cholqr_passes_), "orthonormalize");
+  }
+
+  std::size_t size() const { return N; }
+
+  BlockMultivector create_multivector(std::size_t rows, std::size_t cols) const { return BlockMultivector(ctx_, rows, cols); }
+
+  BlockMatrix create_blockmatrix(std::size_t block_rows, std::size_t block_cols) const { return BlockMatrix(ctx_, block_rows, block_cols); }
+
+  // Rayleigh-Ritz on the projected matrix and the converged count
+  // (evp_base.hh:79-154): eigen-decompose T in FP64 on the host, order the
+  // pairs, keep the eigenvectors on the device, and count the first nev
+  // pairs with ||beta y_last|| / max(|theta|, eps) < tol.
+  virtual std::size_t solve_small_dense(const BlockMatrix& T, BlockMatrixBlockView beta, std::size_t nev)
+  {
+    ctx_.sync(); // raises a pending Cholesky breakdown
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::size_t nb = T.block_rows();
+    const std::size_t nt = nb * blocksize;
+    const std::vector<double> tb = T.to_host();
+    const std::vector<double> bh = beta.to_host();
+    dense_.resize(nt * nt);
+    for (std::size_t i = 0; i < nb; ++i)
+      for (std::size_t j = 0; j < T.block_cols(); ++j)
+        for (unsigned bi = 0; bi < blocksize; ++bi)
+          for (unsigned bj = 0; bj < blocksize; ++bj)
+            dense_[(i * blocksize + bi) * nt + (j * blocksize + bj)] = tb[((i * T.block_cols() + j) * blocksize + bi) * blocksize + bj];
+    std::vector<double> w(nt), Z(nt * nt);
+    if (trl_amd::dense::syev(int(nt), dense_.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
+
+    std::vector<unsigned> idx(nt);
+    std::iota(idx.begin(), idx.end(), 0u);
+    if (order_ == EigenvalueOrder::LargestMagnitude)
+      std::stable_sort(idx.begin(), idx.end(), [&](unsigned a, unsigned b) { return std::abs(w[a]) > std::abs(w[b]); });
+    else if (order_ == EigenvalueOrder::Descending)
+      std::reverse(idx.begin(), idx.end());
+
+    eigenvalues.resize(nt);
+    for (std::size_t i = 0; i < nt; ++i) eigenvalues[i] = w[idx[i]];
+    if (!eigenvectors || eigenvectors->block_rows() != nb) eigenvectors = std::make_unique<BlockMatrix>(ctx_, nb, T.block_cols());
+    evec_host_.assign(nb * T.block_cols() * blocksize * blocksize, 0.0);
+    for (std::size_t i = 0; i < nb; ++i)
+      for (std::size_t j = 0; j < T.block_cols(); ++j)
+        for (unsigned bi = 0; bi < blocksize; ++bi)
+          for (unsigned bj = 0; bj < blocksize; ++bj)
+            evec_host_[((i * T.block_cols() + j) * blocksize + bi) * blocksize + bj] = Z[(i * blocksize + bi) * nt + idx[j * blocksize + bj]];
+    eigenvectors->from_host(evec_host_);
+
+    std::size_t n_converged = 0;
+    const std::size_t last = nb - 1;
+    const double eps = std::numeric_limits<double>::epsilon();
+    const std::size_t n_check = std::min<std::size_t>(nev, nt);
+    last_residuals_.assign(n_check, 0.0);
+    for (std::size_t col = 0; col < n_check; ++col) {
+      const std::size_t bc = col / blocksize, cb = col % blocksize;
+      const double* vl = evec_host_.data() + (last * T.block_cols() + bc) * blocksize * blocksize;
+      double nsq = 0.0;
+      for (unsigned i = 0; i < blocksize; ++i) {
+        double s = 0.0;
+        for (unsigned k = 0; k < blocksize; ++k) s += bh[i * blocksize + k] * vl[k * blocksize + cb];
+        nsq += s * s;
+      }
+      const double rel = std::sqrt(nsq) / std::max(std::abs(eigenvalues[col]), eps);
+      last_residuals_[col] = rel;
+      if (rel < tolerance_) ++n_converged;
+    }
+    host_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return n_converged;
+  }
+
+  const std::vector<double>& get_current_eigenvalues() const { return eigenvalues; }
+
+  const BlockMatrix& get_current_eigenvectors() const
+  {
+    if (!eigenvectors) throw std::logic_error("no Rayleigh-Ritz solve yet");
+    return *eigenvectors;
+  }
+
+  // The eigenvectors of the last Rayleigh-Ritz step, host copy.
+  const std::vector<double>& get_current_eigenvectors_host() const { return evec_host_; }
+
+  std::span<double, blocksize> get_eigenvalues_block(std::size_t block) { return std::span<double, blocksize>(eigenvalues.data() + block * blocksize, blocksize); }
+
+  void set_tolerance(double tol) { tolerance_ = tol; }
+  double get_tolerance() const { return tolerance_; }
+
+  void set_order(EigenvalueOrder o) { order_ = o; }
+  EigenvalueOrder get_order() const { return order_; }
+
+  // 1: Cholesky-QR as in the reference; 2: CholQR2 (R = R2 R1)
+  void set_cholqr_passes(int p) { cholqr_passes_ = p; }
+  int get_cholqr_passes() const { return cholqr_passes_; }
+
+  // Optional diagonal inner-product weight M (device, panel type, n entries).
+  void set_mass(const PanelScalar* m) { mass_ = m; }
+  const PanelScalar* mass() const { return mass_; }
+
+  const Context& context() const { return ctx_; }
+  const std::vector<double>& last_residuals() const { return last_residuals_; }
+  double host_seconds() const { return host_seconds_; }
+
+  // ---- fused block products (used by BlockCGS2 and trl_amd::BlockLanczos) --
+
+  // C = V[0:count]^T M W, C a device (count*b x b) FP64 array owned by this EVP.
+  double* gram_blocks(const BlockMultivector& V, unsigned count, const BlockView& W)
+  {
+    double* C = coef(std::size_t(count) * blocksize * blocksize);
+    throw_on(trl_hip_gram(ctx_.get(), dtype, N, blocksize, int(count), V.data(), W.data_, mass_, C), "gram_blocks");
+    return C;
+  }
+
+  // W -= V[0:count] C
+  void update_blocks(const BlockMultivector& V, unsigned count, const double* C, const BlockView& W)
+  {
+    throw_on(trl_hip_update(ctx_.get(), dtype, N, blocksize, int(count), V.data(), C, W.data_), "update_blocks");
+  }
+
+  // Out[0:kc] = V[0:m] Y[:, 0:kc] (lanczos.hh:105-114 as one contraction)
+  void rotate(const BlockMultivector& V, unsigned m, const BlockMatrix& Y, unsigned kc, BlockMultivector& Out)
+  {
+    throw_on(trl_hip_rotate(ctx_.get(), dtype, N, blocksize, int(m), V.data(), Y.data(), int(Y.block_cols()), int(kc), Out.data()), "rotate");
+  }
+
+protected:
+  double* coef(std::size_t count)
+  {
+    if (coef_.size() < count) coef_ = DeviceBuffer<double>(ctx_, std::max<std::size_t>(count, 4096));
+    return coef_.data();
+  }
+
+  Context ctx_;
+  std::size_t N;
+  const PanelScalar* mass_ = nullptr;
+  int cholqr_passes_ = std::is_same_v<PanelScalar, float> ? 2 : 1;
+
+  std::vector<double> eigenvalues;
+  std::unique_ptr<BlockMatrix> eigenvectors;
+  std::vector<double> evec_host_;
+  std::vector<double> dense_;
+  std::vector<double> last_residuals_;
+  double tolerance_ = 1e-8;
+  EigenvalueOrder order_ = EigenvalueOrder::LargestMagnitude;
+  double host_seconds_ = 0.0;
+  DeviceBuffer<double> coef_;
+};
+
+} // namespace trl::hip

@@ -1,0 +1,245 @@
+#pragma once
+// trl_amd::BlockLanczos — block Lanczos with thick restart over any backend
+// satisfying trl's Eigenproblem concept (include/trl/concepts.hh:157-188).
+//
+// This is the build's own restatement of the reference driver
+// include/trl/eigensolvers/lanczos.hh:22-312, kept step for step so that it
+// produces the same Krylov basis, projected matrix and converged count:
+//   extend(k, m)  lanczos.hh:227-287  (three-term recurrence, full
+//                                      reorthogonalisation, CholQR)
+//   solve()       lanczos.hh:65-199   (Rayleigh-Ritz, thick restart keeping
+//                                      nev/b + 1 Ritz blocks, arrow rebuild,
+//                                      one quasi-Lanczos step)
+// The reference's driver itself also runs unchanged on trl::hip (the
+// conformance build compiles it against these backend types); this copy
+// exists because the GPU runtime does not carry the reference headers, and
+// it adds two backend hooks when the EVP provides them:
+//   - evp.rotate(V, m, Y, k_r, W): the Ritz rotation as one contraction
+//     instead of k_r*m mult_add calls (lanczos.hh:105-114);
+//   - ritz_vectors(): X = V Y[:, 0:nev], which the reference never forms.
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+namespace trl_amd {
+
+struct EigensolverParams {
+  unsigned int nev;
+  unsigned int ncv;
+  unsigned int max_restarts = 1000;
+  double tolerance = 1e-8;
+};
+
+struct EigensolverResult {
+  bool converged;
+  unsigned int iterations;
+  unsigned int n_op_apply;
+};
+
+// ModifiedGS of reorthogonalization.hh:12-22, for backends without a fused
+// multi-block product.
+struct ModifiedGS {
+  template <typename EVP, typename BMV>
+  void operator()(EVP& evp, BMV& V, unsigned int count, typename BMV::BlockView V_next, typename BMV::BlockMatrix::BlockView tmp) const
+  {
+    for (unsigned int j = 0; j < count; ++j) {
+      auto Vj = V.block_view(j);
+      evp.dot(Vj, V_next, tmp);
+      V_next.subtract_product(Vj, tmp);
+    }
+  }
+};
+
+template <class EVP, class Reorth = ModifiedGS>
+class BlockLanczos {
+public:
+  using BMV = typename EVP::BlockMultivector;
+  using Scalar = typename BMV::Scalar;
+  static constexpr unsigned int blocksize = BMV::blocksize;
+
+  BlockLanczos(std::shared_ptr<EVP> evp_, const EigensolverParams& params)
+      : evp(std::move(evp_)), nev(params.nev), ncv(params.ncv), max_restarts(params.max_restarts),
+        V(evp->create_multivector(evp->size(), ncv + blocksize)), W(evp->create_multivector(evp->size(), ncv + blocksize)),
+        T(evp->create_blockmatrix(ncv / blocksize, ncv / blocksize)), B(evp->create_blockmatrix(1, 2))
+  {
+    // the same three checks, in the same order, as lanczos.hh:45-58
+    if (ncv + blocksize > evp->size())
+      throw std::invalid_argument("ncv (" + std::to_string(ncv) + ") + blocksize (" + std::to_string(blocksize) + ") exceeds problem dimension (" +
+                                  std::to_string(evp->size()) + "). Krylov subspace would be exhausted. Reduce ncv to at most " +
+                                  std::to_string(evp->size() - blocksize) + ".");
+    if (nev >= ncv) throw std::invalid_argument("nev must be strictly smaller than ncv");
+    const unsigned int min_blocks = nev / blocksize + 2;
+    if (ncv / blocksize < min_blocks)
+      throw std::invalid_argument("ncv (" + std::to_string(ncv) + ") is too small for thick restart with nev=" + std::to_string(nev) +
+                                  " and blocksize=" + std::to_string(blocksize) + ". Minimum required: ncv >= " + std::to_string(min_blocks * blocksize) +
+                                  ".");
+    if constexpr (requires { evp->set_tolerance(params.tolerance); }) evp->set_tolerance(params.tolerance);
+  }
+
+  EigensolverResult solve()
+  {
+    EigensolverResult res{false, 0, 0};
+    const unsigned int m = ncv / blocksize;
+    const unsigned int kr = nev / blocksize + 1;
+    unsigned int k = 0;
+    auto beta = B.block_view(0, 0);
+    auto Z0 = B.block_view(0, 1);
+    auto W0 = W.block_view(0);
+
+    while (res.iterations < max_restarts) {
+      ++res.iterations;
+      res.n_op_apply += extend(k, m);
+      restarted_ = false;
+
+      last_nconv = static_cast<unsigned int>(evp->solve_small_dense(T, beta, nev));
+      if (last_nconv >= nev) {
+        res.converged = true;
+        return res;
+      }
+      if (kr >= m) break; // no room to restart
+
+      // Ritz rotation: W_j = sum_i V_i Y(i, j), j < kr; W_kr = V_m; swap
+      const auto& Y = evp->get_current_eigenvectors();
+      if constexpr (requires { evp->rotate(V, m, Y, kr, W); }) {
+        evp->rotate(V, m, Y, kr, W);
+      }
+      else {
+        for (unsigned int j = 0; j < kr; ++j) {
+          auto Wj = W.block_view(j);
+          Wj.set_zero();
+          for (unsigned int i = 0; i < m; ++i) V.block_view(i).mult_add(Y.block_view(i, j), Wj);
+        }
+      }
+      W.block_view(kr).copy_from(V.block_view(m));
+      std::swap(V, W);
+      W0 = W.block_view(0);
+      restarted_ = true;
+
+      // T <- [diag(theta) arrow; arrow^T ...]
+      for (unsigned int i = 0; i < kr; ++i)
+        for (unsigned int j = 0; j < kr; ++j) {
+          auto Tij = T.block_view(i, j);
+          Tij.set_zero();
+          if (i == j) Tij.set_diagonal(evp->get_eigenvalues_block(i));
+        }
+      for (unsigned int i = 0; i < kr; ++i) {
+        auto Tki = T.block_view(kr, i);
+        beta.mult(Y.block_view(m - 1, i), Tki);
+        T.block_view(i, kr).copy_from_transpose(Tki);
+      }
+
+      // one quasi-Lanczos step from block kr restores the block recurrence
+      k = kr;
+      auto Vk = V.block_view(k);
+      auto Vk1 = V.block_view(k + 1);
+      evp->apply(Vk, Vk1);
+      ++res.n_op_apply;
+      auto Tkk = T.block_view(k, k);
+      evp->dot(Vk, Vk1, Tkk);
+      Vk.mult(Tkk, W0);
+      Vk1 -= W0;
+      if (k > 0) {
+        V.block_view(k - 1).mult_transpose(T.block_view(k, k - 1), W0);
+        Vk1 -= W0;
+      }
+      reorth_(*evp, V, k + 1, Vk1, Z0);
+      evp->orthonormalize(Vk1, beta);
+      if (k + 1 < T.block_rows()) {
+        T.block_view(k + 1, k).copy_from(beta);
+        T.block_view(k, k + 1).copy_from_transpose(beta);
+        k += 1;
+      }
+      else
+        break;
+    }
+    return res;
+  }
+
+  // Grow a block-k Lanczos factorisation to block m; returns the number of
+  // operator applications (lanczos.hh:227-287).
+  unsigned int extend(unsigned int k, unsigned int m)
+  {
+    if (k >= m || m > ncv / blocksize) throw std::invalid_argument("extend: need k < m <= ncv/blocksize");
+    unsigned int napp = 0;
+    auto W0 = W.block_view(0);
+    auto beta = B.block_view(0, 0);
+    auto Z0 = B.block_view(0, 1);
+    if (k == 0) evp->orthonormalize(V.block_view(0), Z0);
+    for (unsigned int i = k; i < m; ++i) {
+      auto Vc = V.block_view(i);
+      auto Vn = V.block_view(i + 1);
+      evp->apply(Vc, Vn); // V_{i+1} = A V_i
+      ++napp;
+      if (i > 0) { // V_{i+1} -= V_{i-1} beta^T
+        V.block_view(i - 1).mult_transpose(beta, W0);
+        Vn -= W0;
+      }
+      auto Tii = T.block_view(i, i); // T_ii = <V_i, V_{i+1}>
+      evp->dot(Vc, Vn, Tii);
+      Vc.mult(Tii, W0); // V_{i+1} -= V_i T_ii
+      Vn -= W0;
+      reorth_(*evp, V, i + 1, Vn, Z0);
+      evp->orthonormalize(Vn, beta); // V_{i+1} beta = old V_{i+1}
+      if (i + 1 < T.block_rows()) {
+        T.block_view(i + 1, i).copy_from(beta);
+        T.block_view(i, i + 1).copy_from_transpose(beta);
+      }
+    }
+    return napp;
+  }
+
+  // True if solve() stopped after a thick restart: the leading nev/b + 1
+  // blocks of the basis then already are the Ritz vectors of the last
+  // Rayleigh-Ritz step; otherwise they are V Y.
+  bool basis_holds_ritz_blocks() const { return restarted_; }
+
+  // Ritz vectors of the last Rayleigh-Ritz step: X[0:nblk] = V[0:m] Y[:, 0:nblk]
+  // with nblk = ceil(nev / b) blocks of X.
+  void ritz_vectors(BMV& X)
+  {
+    const unsigned int m = ncv / blocksize;
+    const unsigned int nblk = (nev + blocksize - 1) / blocksize;
+    if (X.blocks() < nblk) throw std::invalid_argument("ritz_vectors: X too small");
+    if (restarted_) {
+      for (unsigned int j = 0; j < nblk; ++j) X.block_view(j).copy_from(V.block_view(j));
+      return;
+    }
+    const auto& Y = evp->get_current_eigenvectors();
+    if constexpr (requires { evp->rotate(V, m, Y, nblk, X); }) {
+      evp->rotate(V, m, Y, nblk, X);
+    }
+    else {
+      for (unsigned int j = 0; j < nblk; ++j) {
+        auto Xj = X.block_view(j);
+        Xj.set_zero();
+        for (unsigned int i = 0; i < m; ++i) V.block_view(i).mult_add(Y.block_view(i, j), Xj);
+      }
+    }
+  }
+
+  auto initial_block() { return V.block_view(0); }
+  auto& get_basis() { return V; }
+  auto& get_T() { return T; }
+  auto& get_B() { return B; }
+  EVP& get_evp() { return *evp; }
+  // Converged count of the last Rayleigh-Ritz step.  (The reference's
+  // get_nconv() always returns 0: lanczos.hh:121,217.)
+  unsigned int get_nconv() const { return last_nconv; }
+
+private:
+  std::shared_ptr<EVP> evp;
+  unsigned int nev;
+  unsigned int ncv;
+  unsigned int max_restarts;
+  unsigned int last_nconv = 0;
+  bool restarted_ = false;
+  BMV V;
+  BMV W;
+  typename BMV::BlockMatrix T;
+  typename BMV::BlockMatrix B;
+  Reorth reorth_{};
+};
+
+} // namespace trl_amd

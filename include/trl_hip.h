@@ -149,6 +149,19 @@ int trl_hip_small_set_diagonal(trl_hip_ctx* ctx, int b, double* dst, const doubl
 /* C = A B: BlockMatrixBlockView::mult (matrixblockview.hh:55-64). */
 int trl_hip_small_mult(trl_hip_ctx* ctx, int b, const double* A, const double* B, double* C);
 
+/* ---- eigenvalue adjoint (no reference counterpart) ------------------------ */
+
+/* SDDMM on the pattern of A: for every stored entry p = (i, j) of the CSR
+ * pattern, G[p] = sum_q w_q X[i][q] X[j][q], the gradient of the loss
+ * sum_q w_q lambda_q with respect to A_ij for unit (or M-unit) eigenvectors
+ * (d lambda_q / d A_ij = x_iq x_jq).  X holds nvec = nblocks*b eigenvectors as
+ * nblocks row-major n x b panels (block j at X + j*n*b), w is a device FP64
+ * array of nvec weights, G a device FP64 array of nnz entries.  If gm is not
+ * NULL it receives, per row i, the mass-diagonal gradient of a generalized
+ * problem, gm[i] = -sum_q w_q lambda_q X[i][q]^2 (lambda: device FP64, nvec). */
+int trl_hip_sddmm_adjoint(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, int b, int nblocks, const void* X,
+                          const double* w, const double* lambda, double* G, double* gm);
+
 /* ---- checks -------------------------------------------------------------- */
 
 /* Synchronous.  For the b columns c of panel X and of Y:

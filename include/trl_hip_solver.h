@@ -1,0 +1,84 @@
+/*
+ * trl_hip_solver.h — C-ABI of libtrl_hip_solver.so: whole thick-restart
+ * block-Lanczos solves on the MI355X backend, for callers that are not C++
+ * (ctypes, the benchmark, the autograd wrapper).
+ *
+ * Each entry point constructs the trl::hip eigenproblem over caller-owned
+ * device arrays, runs trl_amd::BlockLanczos (the restatement of the
+ * reference driver include/trl/eigensolvers/lanczos.hh:22-312; parameters as
+ * in include/trl/eigensolvers/params.hh:5-16) and returns the Ritz pairs.
+ * The reference has no C-ABI; these entry points stand where its C++
+ * callers construct BlockLanczos and call solve() (examples/openmp/csrtest.cc:
+ * 100-103, tests/test_lanczos_convergence.hh:113-122).
+ *
+ * Status codes are those of trl_hip.h (0 ok, <0 TRL_HIP_E*, >0 hipError_t).
+ */
+#ifndef TRL_HIP_SOLVER_H
+#define TRL_HIP_SOLVER_H
+
+#include "trl_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int b;                 /* block size: 1, 2, 4, 6, 8, 16                        */
+  unsigned nev;          /* wanted eigenpairs             (params.hh:6)          */
+  unsigned ncv;          /* basis size, multiple of b     (params.hh:7)          */
+  unsigned max_restarts; /* restart cycles                (params.hh:8)          */
+  double tol;            /* relative residual tolerance   (params.hh:9)          */
+  int order;             /* 0 |lambda| descending (evp_base.hh:103), 1 ascending,
+                            2 descending                                         */
+  int reorth;            /* 0 ModifiedGS (reorthogonalization.hh:12-22),
+                            1 block CGS2 (two multi-block Gram/update passes)    */
+  int cholqr_passes;     /* 1 CholQR (evp_base.hh:39-71), 2 CholQR2, 0 default  */
+  int lanes_per_row;     /* SpMM lanes per CSR row, 0 = default                  */
+} trl_hip_solver_params;
+
+typedef struct {
+  int converged;           /* EigensolverResult::converged  (params.hh:13)   */
+  unsigned iterations;     /* EigensolverResult::iterations (params.hh:14)   */
+  unsigned n_op_apply;     /* EigensolverResult::n_op_apply (params.hh:15)   */
+  double seconds;          /* wall time of solve(), device work included     */
+  double host_seconds;     /* of which the host Rayleigh-Ritz steps          */
+} trl_hip_solver_result;
+
+/* Standard (mass = mass_inv = NULL) or generalized problem with a diagonal
+ * mass M (both device arrays of the panel dtype, mass_inv = 1/M).
+ * val: CSR values in the panel dtype.  V0: device n x b initial block.
+ * evals: host, nev Ritz values in the chosen order.  X (optional): device,
+ * ceil(nev/b) panels (n x b each) receiving the Ritz vectors.
+ * resid (optional): host, nev true relative residuals
+ * ||A x - lambda M x|| / ||A x|| evaluated on the device. */
+int trl_hip_solve_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, const void* mass,
+                      const void* mass_inv, const trl_hip_solver_params* p, const void* V0, double* evals, void* X, double* resid,
+                      trl_hip_solver_result* res);
+
+typedef struct {
+  double tol_fp32;           /* stopping tolerance of the FP32 phase          */
+  unsigned max_restarts_fp32;
+  unsigned long long seed;   /* mixing of the FP32 Ritz vectors (see below)   */
+} trl_hip_mixed_params;
+
+/* Mixed precision: an FP32 solve (FP32 basis, SpMM and Gram products, FP64
+ * coefficients, Cholesky and Rayleigh-Ritz) to tol_fp32, then an FP64 solve
+ * to p->tol started from V0 = X32 G, where X32 are the nev/b + 1 FP32 Ritz
+ * blocks converted to FP64 and G is a seeded Gaussian (nev/b+1)*b x b
+ * mixing matrix, so the FP64 Krylov space opens on the FP32 Ritz subspace.
+ * val32/mass32/minv32 are FP32 copies of the FP64 arrays.  evals, X, resid
+ * and res refer to the final FP64 phase; res32 (optional) to the FP32 one. */
+int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, const double* val64, const float* val32,
+                            const double* mass64, const double* minv64, const float* mass32, const float* minv32, const trl_hip_solver_params* p,
+                            const trl_hip_mixed_params* mp, const double* V0, double* evals, double* X, double* resid, trl_hip_solver_result* res,
+                            trl_hip_solver_result* res32);
+
+/* Host FP64 symmetric eigensolver used by the Rayleigh-Ritz step:
+ * A (n x n row-major) -> ascending w, eigenvectors in the columns of Z. */
+int trl_hip_dense_syev(int n, const double* A, double* w, double* Z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRL_HIP_SOLVER_H */

@@ -1,0 +1,326 @@
+"""Kernel-level parity of libtrl_hip.so against the CPU oracle.
+
+Every entry point of include/trl_hip.h is called through the C-ABI on
+seeded inputs and compared with the oracle's restatement of the reference
+operation (oracle/trl_oracle.cc).  FP64 within 1e-12 relative, FP32 within
+the bound stated per test.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from trl_amd import _lib as L
+from trl_amd import problems as P
+from trl_amd.formats import csr_to_ell
+
+pytestmark = pytest.mark.gpu
+
+DT = {"f64": np.float64, "f32": np.float32}
+TOL = {"f64": 1e-12, "f32": 2e-5}
+BS = [1, 2, 4, 6, 8, 16]
+
+
+def H():
+    return L.hip()
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def dev(ctx, a, dt):
+    return ctx.to_device(np.ascontiguousarray(a, dtype=dt))
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", BS)
+@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 8, 16])
+def test_spmm_csr(ctx, rng, p, b, lanes):
+    print(f"spmm_csr {p} b={b} lanes={lanes}", flush=True)
+    A = P.laplacian_3d(7, 5, 6, rng.uniform(0, 1, 210))
+    X = rng.standard_normal((A.n, b))
+    d = rng.uniform(0.5, 2.0, A.n)
+    dt = DT[p]
+    rp, ci = dev(ctx, A.rowptr, np.int32), dev(ctx, A.col, np.int32)
+    va, Xd, dd = dev(ctx, A.val, dt), dev(ctx, X, dt), dev(ctx, d, dt)
+    Y = ctx.empty((A.n, b), dt)
+    for scale in (None, dd):
+        L.check(H().trl_hip_spmm_csr(ctx.h, L.dtype_code(dt), A.n, b, rp.ptr, ci.ptr, va.ptr, Xd.ptr, Y.ptr, scale.ptr if scale else None, lanes), "spmm")
+        ref = oracle.spmm(A, X.astype(dt).astype(np.float64), d.astype(dt).astype(np.float64) if scale else None)
+        assert rel(Y.download(), ref) < TOL[p]
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", [1, 4, 8])
+def test_spmm_ell(ctx, rng, p, b):
+    A = P.stiffness_mass(3, 3, 2, dof=2)[0]
+    X = rng.standard_normal((A.n, b))
+    width, ecol, evals = csr_to_ell(A)
+    dt = DT[p]
+    Y = ctx.empty((A.n, b), dt)
+    ec, ev, Xd = dev(ctx, ecol, np.int32), dev(ctx, evals, dt), dev(ctx, X, dt)
+    L.check(H().trl_hip_spmm_ell(ctx.h, L.dtype_code(dt), A.n, b, width, ec.ptr, ev.ptr, Xd.ptr, Y.ptr, None), "ell")
+    assert rel(Y.download(), oracle.spmm(A, X.astype(dt).astype(np.float64))) < TOL[p]
+
+
+def test_spmm_empty_rows(ctx, rng):
+    # rows with no entries give zero rows (ragged CSR)
+    n, b = 50, 8
+    rowptr = np.zeros(n + 1, np.int32)
+    cols, vals = [], []
+    for r in range(n):
+        k = r % 4  # 0..3 entries, some rows empty
+        cs = sorted(rng.choice(n, k, replace=False)) if k else []
+        cols += cs
+        vals += list(rng.standard_normal(k))
+        rowptr[r + 1] = rowptr[r] + k
+    A = P.CSR(n, rowptr, np.array(cols, np.int32), np.array(vals))
+    X = rng.standard_normal((n, b))
+    for lanes in (1, 4, 16):
+        Y = ctx.empty((n, b))
+        L.check(H().trl_hip_spmm_csr(ctx.h, L.F64, n, b, dev(ctx, A.rowptr, np.int32).ptr, dev(ctx, A.col, np.int32).ptr, dev(ctx, A.val, np.float64).ptr,
+                                     dev(ctx, X, np.float64).ptr, Y.ptr, None, lanes), "spmm")
+        assert rel(Y.download(), oracle.spmm(A, X)) < 1e-14
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", BS)
+def test_panel_ops(ctx, rng, p, b):
+    n = 1037
+    dt = DT[p]
+    X = rng.standard_normal((n, b)).astype(dt).astype(np.float64)
+    Y0 = rng.standard_normal((n, b)).astype(dt).astype(np.float64)
+    W = rng.standard_normal((b, b))
+    Wd = dev(ctx, W, np.float64)
+    for mode in range(4):
+        Xd, Yd = dev(ctx, X, dt), dev(ctx, Y0, dt)
+        L.check(H().trl_hip_panel_mult(ctx.h, L.dtype_code(dt), n, b, Xd.ptr, Wd.ptr, Yd.ptr, mode), "panel_mult")
+        Xr, Yr = X.copy(), Y0.copy()
+        oracle.panel_mult(Xr, W, Yr, mode)
+        if mode == 3:
+            assert rel(Xd.download(), Xr) < TOL[p]
+        else:
+            assert rel(Yd.download(), Yr) < TOL[p]
+    Xd, Yd = dev(ctx, X, dt), dev(ctx, Y0, dt)
+    L.check(H().trl_hip_panel_sub(ctx.h, L.dtype_code(dt), n, b, Xd.ptr, Yd.ptr), "sub")
+    assert rel(Xd.download(), X - Y0) < TOL[p]
+
+
+def test_panel_sub_unaligned(ctx, rng):
+    # a panel starting at an offset that is not 16-byte aligned
+    n, b = 3, 1
+    X = rng.standard_normal(4 * n)
+    Y = rng.standard_normal(4 * n)
+    Xd, Yd = dev(ctx, X, np.float64), dev(ctx, Y, np.float64)
+    L.check(H().trl_hip_panel_sub(ctx.h, L.F64, n, b, Xd.offset(1), Yd.offset(1)), "sub")
+    out = Xd.download()
+    ref = X.copy()
+    ref[1 : 1 + n] -= Y[1 : 1 + n]
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", BS)
+@pytest.mark.parametrize("k", [1, 3, 17, 33])
+@pytest.mark.parametrize("mass", [False, True])
+def test_gram(ctx, rng, p, b, k, mass):
+    print(f"gram {p} b={b} k={k} mass={mass}", flush=True)
+    n = 4099
+    dt = DT[p]
+    V = rng.standard_normal((k, n, b)).astype(dt).astype(np.float64)
+    W = rng.standard_normal((n, b)).astype(dt).astype(np.float64)
+    m = rng.uniform(0.5, 2, n).astype(dt).astype(np.float64) if mass else None
+    Cd = ctx.empty((k * b, b))
+    md = dev(ctx, m, dt) if mass else None
+    L.check(H().trl_hip_gram(ctx.h, L.dtype_code(dt), n, b, k, dev(ctx, V, dt).ptr, dev(ctx, W, dt).ptr, md.ptr if md else None, Cd.ptr), "gram")
+    ref = oracle.gram(V, W, m)
+    scale = np.sqrt(n)  # entries ~ sqrt(n); compare against the typical magnitude
+    err = np.abs(Cd.download() - ref).max() / scale
+    assert err < (1e-13 if p == "f64" else 2e-6)
+
+
+def test_gram_many_blocks_and_tiny_n(ctx, rng):
+    # k beyond one launch (chunked on the host) and n < one row quad
+    for n, b, k in [(3, 8, 1), (1, 4, 5), (257, 1, 1100), (301, 8, 140)]:
+        V = rng.standard_normal((k, n, b))
+        W = rng.standard_normal((n, b))
+        Cd = ctx.empty((k * b, b))
+        L.check(H().trl_hip_gram(ctx.h, L.F64, n, b, k, dev(ctx, V, np.float64).ptr, dev(ctx, W, np.float64).ptr, None, Cd.ptr), "gram")
+        assert rel(Cd.download(), oracle.gram(V, W)) < 1e-12
+
+
+def test_gram_deterministic(ctx, rng):
+    n, b, k = 100003, 8, 9
+    V = dev(ctx, rng.standard_normal((k, n, b)), np.float32)
+    W = dev(ctx, rng.standard_normal((n, b)), np.float32)
+    outs = []
+    for _ in range(3):
+        Cd = ctx.empty((k * b, b))
+        L.check(H().trl_hip_gram(ctx.h, L.F32, n, b, k, V.ptr, W.ptr, None, Cd.ptr), "gram")
+        outs.append(Cd.download())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_gram_fp32_compensated(ctx):
+    # a long FP32 dot whose naive float sum loses digits: sum of 2^22 ones plus
+    # a small residue per row must come out exact to FP64 rounding
+    n, b = 1 << 22, 1
+    x = np.ones((n, b), np.float32)
+    y = np.full((n, b), 1.0 + 2.0**-20, np.float32)
+    Cd = ctx.empty((1, 1))
+    L.check(H().trl_hip_gram(ctx.h, L.F32, n, b, 1, dev(ctx, x, np.float32).ptr, dev(ctx, y, np.float32).ptr, None, Cd.ptr), "gram")
+    exact = n * (1.0 + 2.0**-20)
+    assert abs(Cd.download()[0, 0] - exact) / exact < 1e-12
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", BS)
+@pytest.mark.parametrize("k", [1, 5, 33])
+def test_update(ctx, rng, p, b, k):
+    n = 2051
+    dt = DT[p]
+    V = rng.standard_normal((k, n, b)).astype(dt).astype(np.float64)
+    W = rng.standard_normal((n, b)).astype(dt).astype(np.float64)
+    Cm = rng.standard_normal((k * b, b)) * 0.1
+    Wd = dev(ctx, W, dt)
+    L.check(H().trl_hip_update(ctx.h, L.dtype_code(dt), n, b, k, dev(ctx, V, dt).ptr, dev(ctx, Cm, np.float64).ptr, Wd.ptr), "update")
+    ref = W.copy()
+    oracle.update(V, Cm.astype(dt).astype(np.float64), ref)
+    assert rel(Wd.download(), ref) < (1e-12 if p == "f64" else 1e-5)
+
+
+@pytest.mark.parametrize("b", BS)
+@pytest.mark.parametrize("mass", [False, True])
+def test_cholqr_fp64_matches_oracle(ctx, rng, b, mass):
+    n = 3001
+    X = rng.standard_normal((n, b))
+    m = rng.uniform(0.5, 2, n) if mass else None
+    Xd = dev(ctx, X, np.float64)
+    Rd = ctx.empty((b, b))
+    md = dev(ctx, m, np.float64) if mass else None
+    L.check(H().trl_hip_cholqr(ctx.h, L.F64, n, b, Xd.ptr, md.ptr if md else None, Rd.ptr, 1), "cholqr")
+    assert ctx.check() == 0
+    Xr = X.copy()
+    rc, Rr = oracle.cholqr(Xr, m)
+    assert rc == 0
+    assert rel(Rd.download(), Rr) < 1e-12
+    assert rel(Xd.download(), Xr) < 1e-11
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", [2, 8])
+def test_cholqr2_contract(ctx, rng, p, b):
+    # ill-conditioned block: CholQR2 restores orthogonality, X_old = X_new R
+    n = 20000
+    dt = DT[p]
+    Q, _ = np.linalg.qr(rng.standard_normal((n, b)))
+    X = (Q * np.logspace(0, 2, b)) @ np.linalg.qr(rng.standard_normal((b, b)))[0]
+    X = X.astype(dt).astype(np.float64)
+    Xd = dev(ctx, X, dt)
+    Rd = ctx.empty((b, b))
+    L.check(H().trl_hip_cholqr(ctx.h, L.dtype_code(dt), n, b, Xd.ptr, None, Rd.ptr, 2), "cholqr2")
+    assert ctx.check() == 0
+    Xn = Xd.download().astype(np.float64)
+    R = Rd.download()
+    assert np.allclose(np.tril(R, -1), 0) and (np.diag(R) > 0).all()
+    assert np.abs(Xn.T @ Xn - np.eye(b)).max() < (1e-14 * n**0.5 if p == "f64" else 1e-6)
+    assert rel(Xn @ R, X) < (1e-13 if p == "f64" else 1e-6)
+
+
+def test_cholqr_breakdown_flag(ctx, rng):
+    n, b = 500, 4
+    X = rng.standard_normal((n, b))
+    X[:, 2] = 0.0
+    Xd = dev(ctx, X, np.float64)
+    Rd = ctx.empty((b, b))
+    L.check(H().trl_hip_cholqr(ctx.h, L.F64, n, b, Xd.ptr, None, Rd.ptr, 1), "cholqr")
+    assert ctx.check() == 3  # 1-based failing pivot, as the oracle reports it
+    assert oracle.cholqr(X.copy())[0] == 3
+    assert np.array_equal(Xd.download(), X)  # left unchanged
+    assert ctx.check() == 0  # flag cleared
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", BS)
+@pytest.mark.parametrize("m,kc", [(3, 1), (8, 3), (32, 5)])
+def test_rotate(ctx, rng, p, b, m, kc):
+    n = 1999
+    dt = DT[p]
+    V = rng.standard_normal((m, n, b)).astype(dt).astype(np.float64)
+    ybc = kc + 1
+    Y = rng.standard_normal((m, ybc, b, b))
+    Out = ctx.empty((kc, n, b), dt)
+    L.check(H().trl_hip_rotate(ctx.h, L.dtype_code(dt), n, b, m, dev(ctx, V, dt).ptr, dev(ctx, Y, np.float64).ptr, ybc, kc, Out.ptr), "rotate")
+    ref = oracle.rotate(V, Y.astype(dt).astype(np.float64), kc)
+    assert rel(Out.download(), ref) < (1e-12 if p == "f64" else 1e-5)
+
+
+def test_small_ops(ctx, rng):
+    for b in (1, 3, 8, 16):
+        A = rng.standard_normal((b, b))
+        Bm = rng.standard_normal((b, b))
+        Ad, Bd, Cd = dev(ctx, A, np.float64), dev(ctx, Bm, np.float64), ctx.empty((b, b))
+        L.check(H().trl_hip_small_mult(ctx.h, b, Ad.ptr, Bd.ptr, Cd.ptr), "mult")
+        assert rel(Cd.download(), A @ Bm) < 1e-14
+        L.check(H().trl_hip_small_copy(ctx.h, b, Ad.ptr, Cd.ptr, 1), "copy_t")
+        assert np.array_equal(Cd.download(), A.T)
+        vals = np.arange(1.0, b + 1)
+        L.check(H().trl_hip_small_set_diagonal(ctx.h, b, Cd.ptr, vals.ctypes.data), "diag")
+        ref = A.T.copy()
+        np.fill_diagonal(ref, vals)
+        assert np.array_equal(Cd.download(), ref)
+        L.check(H().trl_hip_small_zero(ctx.h, b, Cd.ptr), "zero")
+        assert not Cd.download().any()
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+def test_residual_sq(ctx, rng, p):
+    n, b = 777, 4
+    dt = DT[p]
+    X = rng.standard_normal((n, b)).astype(dt).astype(np.float64)
+    Y = rng.standard_normal((n, b)).astype(dt).astype(np.float64)
+    m = rng.uniform(0.5, 2, n).astype(dt).astype(np.float64)
+    lam = rng.standard_normal(b)
+    out = np.zeros(2 * b)
+    L.check(H().trl_hip_residual_sq(ctx.h, L.dtype_code(dt), n, b, dev(ctx, X, dt).ptr, dev(ctx, Y, dt).ptr, dev(ctx, m, dt).ptr, lam.ctypes.data,
+                                    out.ctypes.data), "res")
+    r = Y - lam * m[:, None] * X
+    assert np.allclose(out[:b], (r * r).sum(0), rtol=1e-12)
+    assert np.allclose(out[b:], (Y * Y).sum(0), rtol=1e-12)
+
+
+@pytest.mark.parametrize("b", [1, 4, 8])
+def test_sddmm(ctx, rng, b):
+    A = P.laplacian_2d(13, 11)
+    nblk = 3
+    X = rng.standard_normal((nblk, A.n, b))
+    w = rng.uniform(0, 1, nblk * b)
+    lam = rng.uniform(1, 2, nblk * b)
+    G = ctx.empty(A.nnz)
+    gm = ctx.empty(A.n)
+    L.check(H().trl_hip_sddmm_adjoint(ctx.h, L.F64, A.n, dev(ctx, A.rowptr, np.int32).ptr, dev(ctx, A.col, np.int32).ptr, b, nblk,
+                                      dev(ctx, X, np.float64).ptr, dev(ctx, w, np.float64).ptr, dev(ctx, lam, np.float64).ptr, G.ptr, gm.ptr), "sddmm")
+    Xf = np.concatenate(list(X), axis=1)  # n x nvec
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    ref = ((Xf[rows] * w) * Xf[A.col]).sum(1)
+    assert rel(G.download(), ref) < 1e-13
+    assert rel(gm.download(), -((Xf**2) * (w * lam)).sum(1)) < 1e-13
+
+
+def test_convert_and_fill(ctx, rng):
+    x = rng.standard_normal(1001)
+    d = dev(ctx, x, np.float64)
+    f = ctx.empty(1001, np.float32)
+    L.check(H().trl_hip_convert(ctx.h, L.F64, d.ptr, L.F32, f.ptr, 1001), "convert")
+    assert np.array_equal(f.download(), x.astype(np.float32))
+    L.check(H().trl_hip_fill_zero(ctx.h, d.ptr, 1001 * 8), "fill")
+    assert not d.download().any()
+
+
+def test_invalid_arguments(ctx):
+    assert H().trl_hip_gram(ctx.h, L.F64, 10, 3, 1, None, None, None, None) == -1  # b = 3 unsupported
+    assert H().trl_hip_spmm_csr(ctx.h, L.F64, 10, 4, None, None, None, None, None, None, 3) == -1
+    assert H().trl_hip_cholqr(ctx.h, L.F64, 10, 4, None, None, None, 3) == -1

@@ -1,0 +1,123 @@
+"""Whole-solve parity of the HIP backend (libtrl_hip_solver.so) against the
+CPU oracle and the closed forms, on the same seeded inputs.
+
+Bar (from the benchmark definition): eigenvalues within 1e-10 relative of
+the oracle's FP64 run, true residuals ||Ax - lambda M x|| / ||Ax|| at the
+solver tolerance, principal angles between the HIP and oracle invariant
+subspaces reported and bounded.
+"""
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import oracle
+from trl_amd import _lib as L
+from trl_amd import problems as P
+from trl_amd.solver import DeviceCSR, EigensolverParams, solve, solve_mixed
+
+pytestmark = pytest.mark.gpu
+
+
+def v0(ctx, n, b, dt=np.float64, seed=42):
+    return ctx.to_device(P.initial_block(n, b, seed).astype(dt))
+
+
+def max_angle(Xa, Xb):
+    Qa, _ = np.linalg.qr(Xa)
+    Qb, _ = np.linalg.qr(Xb)
+    return float(np.max(sla.subspace_angles(Qa, Qb)))
+
+
+@pytest.mark.parametrize("reorth", ["mgs", "cgs2"])
+@pytest.mark.parametrize("b", [1, 2, 4, 8])
+@pytest.mark.parametrize("kind", ["diagonal", "laplace1d"])
+def test_reference_convergence(ctx, kind, b, reorth):
+    # tests/openmp/test_lanczos_convergence.cc: N = 128, nev = 16, ncv = 64,
+    # largest |lambda| first, relative 1e-8 against the exact values
+    print(f"convergence {kind} b={b} {reorth}", flush=True)
+    N, nev = 128, 16
+    A = P.diagonal(N) if kind == "diagonal" else P.laplacian_1d(N)
+    exact = (N - np.arange(nev)) ** 2.0 if kind == "diagonal" else 2 - 2 * np.cos((N - np.arange(nev)) * np.pi / (N + 1))
+    dA = DeviceCSR(ctx, A, precisions=("f64",))
+    out = solve(dA, b, EigensolverParams(nev, 4 * nev), v0(ctx, N, b), order="largest_magnitude", reorth=reorth)
+    assert out.result["converged"] == 1
+    assert np.all(np.abs(out.evals - exact) <= 1e-8 * np.abs(exact))
+    ref = oracle.lanczos_solve(A, b, nev, 4 * nev, P.initial_block(N, b), order=0)
+    assert np.all(np.abs(out.evals - ref["evals"]) <= 1e-10 * np.abs(ref["evals"]))
+
+
+@pytest.mark.parametrize("reorth", ["mgs", "cgs2"])
+def test_laplace2d_128_matches_oracle(ctx, reorth):
+    # 2D 5-point Laplacian 128^2, b = 4, 8 smallest, ncv = 256, tol 1e-8
+    c = P.config_c1()
+    print(f"laplace2d_128 {reorth}", flush=True)
+    dA = DeviceCSR(ctx, c.csr, precisions=("f64",))
+    out = solve(dA, c.b, EigensolverParams(c.nev, c.ncv, tolerance=c.tol), v0(ctx, c.csr.n, c.b), order="ascending", reorth=reorth)
+    assert out.result["converged"] == 1
+    ref = oracle.lanczos_solve(c.csr, c.b, c.nev, c.ncv, P.initial_block(c.csr.n, c.b), tol=c.tol, order=1)
+    assert ref["converged"]
+    rel_or = np.abs(out.evals - ref["evals"]) / np.abs(ref["evals"])
+    rel_ex = np.abs(out.evals - c.exact) / c.exact
+    print("rel vs oracle", rel_or.max(), "rel vs closed form", rel_ex.max(), "restarts", out.result["iterations"], ref["iterations"])
+    assert rel_or.max() <= 1e-10
+    assert rel_ex.max() <= 1e-10
+    assert out.resid.max() <= 1e-7
+    X = out.X.download()  # (nblk, n, b)
+    Xh = np.concatenate(list(X), axis=1)[:, : c.nev]
+    ang = max_angle(Xh, ref["X"])
+    print("max principal angle", ang)
+    assert ang < 1e-6
+
+
+def test_generalized_matches_scipy_golden(ctx):
+    import json
+    from pathlib import Path
+
+    g = json.loads((Path(__file__).parent / "golden" / "stiffness_mass_eigsh.json").read_text())
+    c = P.config_c2()
+    assert c.csr.digest() == g["csr_sha256"]
+    dA = DeviceCSR(ctx, c.csr, mass=c.mass, precisions=("f64",))
+    out = solve(dA, c.b, EigensolverParams(c.nev, c.ncv), v0(ctx, c.csr.n, c.b), order="ascending", reorth="cgs2")
+    assert out.result["converged"] == 1
+    ref = np.array(g["eigenvalues"])
+    print("rel vs scipy", (np.abs(out.evals - ref) / ref).max())
+    assert np.all(np.abs(out.evals - ref) <= 1e-10 * ref)
+    assert out.resid.max() < 1e-7
+
+
+def test_fp32_and_mixed_laplace2d(ctx):
+    c = P.config_c1()
+    dA = DeviceCSR(ctx, c.csr)
+    o32 = solve(dA, c.b, EigensolverParams(c.nev, c.ncv, tolerance=1e-5), v0(ctx, c.csr.n, c.b, np.float32), dtype="f32", order="ascending")
+    assert o32.result["converged"] == 1
+    assert np.all(np.abs(o32.evals - c.exact) <= 1e-4 * c.exact)
+    om = solve_mixed(dA, c.b, EigensolverParams(c.nev, c.ncv, tolerance=1e-8), v0(ctx, c.csr.n, c.b), tol_fp32=1e-5)
+    print("mixed", om.result, om.result32, (np.abs(om.evals - c.exact) / c.exact).max())
+    assert om.result["converged"] == 1
+    assert np.all(np.abs(om.evals - c.exact) <= 1e-10 * c.exact)
+    assert om.resid.max() <= 1e-7
+
+
+def test_errors(ctx):
+    A = P.diagonal(64)
+    dA = DeviceCSR(ctx, A, precisions=("f64",))
+    with pytest.raises(L.TrlHipError) as e:
+        solve(dA, 4, EigensolverParams(8, 64), v0(ctx, 64, 4))  # ncv + b > n (lanczos.hh:45-48)
+    assert e.value.code == -1
+    with pytest.raises(L.TrlHipError) as e:
+        solve(dA, 4, EigensolverParams(8, 8), v0(ctx, 64, 4))  # nev >= ncv
+    assert e.value.code == -1
+    V = P.initial_block(64, 4)
+    V[:, 1] = 0.0  # rank-deficient start block: Cholesky breakdown (evp_base.hh:55)
+    with pytest.raises(L.TrlHipError) as e:
+        solve(dA, 4, EigensolverParams(8, 32), ctx.to_device(V))
+    assert e.value.code == -2
+    assert ctx.check() == 0
+
+
+def test_max_restarts_not_converged(ctx):
+    c = P.config_c1()
+    dA = DeviceCSR(ctx, c.csr, precisions=("f64",))
+    out = solve(dA, c.b, EigensolverParams(c.nev, 32, max_restarts=2), v0(ctx, c.csr.n, c.b), order="ascending")
+    assert out.result["converged"] == 0 and out.result["iterations"] == 2

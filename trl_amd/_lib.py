@@ -52,7 +52,7 @@ HIP_API = [
     ("trl_hip_small_set_diagonal", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_small_mult", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_residual_sq", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
-    ("trl_hip_sddmm_adjoint", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_sddmm_adjoint", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 ]
 
 _hip = None
@@ -183,3 +183,48 @@ class Context:
         host = np.asarray(host)
         d = DeviceArray(self, host.shape, dtype or host.dtype)
         return d.upload(host)
+
+
+# ---------------------------------------------------------------------------
+# include/trl_hip_solver.h
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("b", c_int), ("nev", C.c_uint), ("ncv", C.c_uint), ("max_restarts", C.c_uint), ("tol", C.c_double), ("order", c_int),
+                ("reorth", c_int), ("cholqr_passes", c_int), ("lanes_per_row", c_int)]
+
+
+class SolverResult(C.Structure):
+    _fields_ = [("converged", c_int), ("iterations", C.c_uint), ("n_op_apply", C.c_uint), ("seconds", C.c_double), ("host_seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class MixedParams(C.Structure):
+    _fields_ = [("tol_fp32", C.c_double), ("max_restarts_fp32", C.c_uint), ("seed", C.c_ulonglong)]
+
+
+SOLVER_API = [
+    ("trl_hip_solve_csr", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(SolverParams), c_void_p, c_void_p,
+                                  c_void_p, c_void_p, C.POINTER(SolverResult)]),
+    ("trl_hip_solve_csr_mixed", c_int, [c_void_p, c_size, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        C.POINTER(SolverParams), C.POINTER(MixedParams), c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(SolverResult),
+                                        C.POINTER(SolverResult)]),
+    ("trl_hip_dense_syev", c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+]
+
+_solver = None
+
+
+def solver() -> C.CDLL:
+    global _solver
+    if _solver is None:
+        hip()
+        lib = _load("libtrl_hip_solver.so")
+        for name, res, args in SOLVER_API:
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _solver = lib
+    return _solver

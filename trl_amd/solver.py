@@ -1,0 +1,113 @@
+"""Python entry to whole solves on the GPU (include/trl_hip_solver.h).
+
+`DeviceCSR` keeps a CSR matrix (and an optional diagonal mass) resident in
+HBM in both panel precisions; `solve` / `solve_mixed` run the C++ driver
+(trl_amd::BlockLanczos over trl::hip) on it.  Names and parameter meaning
+follow the reference's EigensolverParams / EigensolverResult
+(include/trl/eigensolvers/params.hh:5-16).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .problems import CSR
+
+ORDER = {"largest_magnitude": 0, "ascending": 1, "descending": 2}
+REORTH = {"mgs": 0, "cgs2": 1}
+
+
+@dataclass
+class EigensolverParams:
+    nev: int
+    ncv: int
+    max_restarts: int = 1000
+    tolerance: float = 1e-8
+
+
+class DeviceCSR:
+    def __init__(self, ctx: L.Context, csr: CSR, mass: np.ndarray | None = None, precisions=("f64", "f32")):
+        self.ctx = ctx
+        self.n = csr.n
+        self.nnz = csr.nnz
+        self.rowptr = ctx.to_device(csr.rowptr.astype(np.int32))
+        self.col = ctx.to_device(csr.col.astype(np.int32))
+        self.val = {}
+        self.mass = {}
+        self.minv = {}
+        for p in precisions:
+            dt = np.float64 if p == "f64" else np.float32
+            self.val[p] = ctx.to_device(csr.val.astype(dt))
+            if mass is not None:
+                self.mass[p] = ctx.to_device(np.asarray(mass, dtype=np.float64).astype(dt))
+                self.minv[p] = ctx.to_device((1.0 / np.asarray(mass, dtype=np.float64)).astype(dt))
+        self.has_mass = mass is not None
+
+    def ptrs(self, p: str):
+        return (self.val[p].ptr, self.mass[p].ptr if self.has_mass else None, self.minv[p].ptr if self.has_mass else None)
+
+
+@dataclass
+class SolveOutput:
+    evals: np.ndarray
+    X: L.DeviceArray | None
+    resid: np.ndarray | None
+    result: dict
+    result32: dict | None = None
+
+
+def _params(b, params: EigensolverParams, order, reorth, cholqr_passes, lanes):
+    return L.SolverParams(b, params.nev, params.ncv, params.max_restarts, params.tolerance, ORDER.get(order, order), REORTH.get(reorth, reorth),
+                          cholqr_passes, lanes)
+
+
+def solve(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *, dtype="f64", order="ascending", reorth="cgs2", cholqr_passes=0,
+          lanes_per_row=0, ritz_vectors=True) -> SolveOutput:
+    ctx = A.ctx
+    dt = np.float64 if dtype == "f64" else np.float32
+    assert V0.dtype == dt and V0.nbytes == A.n * b * np.dtype(dt).itemsize
+    nblk = (params.nev + b - 1) // b
+    X = ctx.empty((nblk, A.n, b), dt) if ritz_vectors else None
+    evals = np.zeros(params.nev)
+    resid = np.zeros(params.nev)
+    res = L.SolverResult()
+    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row)
+    val, mass, minv = A.ptrs(dtype)
+    rc = L.solver().trl_hip_solve_csr(ctx.h, L.dtype_code(dt), A.n, A.rowptr.ptr, A.col.ptr, val, mass, minv, C.byref(p), V0.ptr,
+                                      evals.ctypes.data, X.ptr if X else None, resid.ctypes.data if X else None, C.byref(res))
+    L.check(rc, "trl_hip_solve_csr")
+    return SolveOutput(evals, X, resid if X else None, res.as_dict())
+
+
+def solve_mixed(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *, tol_fp32=1e-5, max_restarts_fp32=1000, seed=7, order="ascending",
+                reorth="cgs2", cholqr_passes=0, lanes_per_row=0) -> SolveOutput:
+    ctx = A.ctx
+    assert V0.dtype == np.float64
+    nblk = (params.nev + b - 1) // b
+    X = ctx.empty((nblk, A.n, b), np.float64)
+    evals = np.zeros(params.nev)
+    resid = np.zeros(params.nev)
+    res = L.SolverResult()
+    res32 = L.SolverResult()
+    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row)
+    mp = L.MixedParams(tol_fp32, max_restarts_fp32, seed)
+    v64, m64, mi64 = A.ptrs("f64")
+    v32, m32, mi32 = A.ptrs("f32")
+    rc = L.solver().trl_hip_solve_csr_mixed(ctx.h, A.n, A.rowptr.ptr, A.col.ptr, v64, v32, m64, mi64, m32, mi32, C.byref(p), C.byref(mp), V0.ptr,
+                                            evals.ctypes.data, X.ptr, resid.ctypes.data, C.byref(res), C.byref(res32))
+    L.check(rc, "trl_hip_solve_csr_mixed")
+    return SolveOutput(evals, X, resid, res.as_dict(), res32.as_dict())
+
+
+def dense_syev(A: np.ndarray):
+    """The host FP64 eigensolver of the Rayleigh-Ritz step (for tests)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.empty(n)
+    Z = np.empty((n, n))
+    L.check(L.solver().trl_hip_dense_syev(n, A.ctypes.data, w.ctypes.data, Z.ctypes.data), "trl_hip_dense_syev")
+    return w, Z
