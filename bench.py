@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: time to k eigenpairs at relative residual 1e-8, mixed FP32 vs
+FP64 on one MI355X, and SpMM GB/s.
+
+Workload (default): 3D 7-point Laplacian 160^3 (n = 4,096,000, 28,518,400
+nonzeros) + diagonal potential U[0,1], block size 8, the 32 smallest
+eigenpairs, ncv = 256, tol 1e-8.  Synthetic seeded data (trl_amd.problems).
+
+One step = one complete mixed-precision solve (FP32 thick-restart phase,
+warm start, FP64 phase to 1e-8 with the FP64 true-residual check), inputs
+resident in HBM.  `value` is the mean seconds per step.  Beside it: one
+plain FP64 solve from the same start block (speedup), the SpMM rate, the
+roofline of the dominant kernel family measured with HIP events on the
+library stream, and the CPU oracle timed on a bounded sample.
+
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "time to k eigenpairs @ rel. residual 1e-8: mixed-FP32 vs FP64 GPU; SpMM GB/s"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmm_rate(ctx, dA, b, dtype, reps=50):
+    from trl_amd import _lib as L
+
+    dt = np.float32 if dtype == "f32" else np.float64
+    X = ctx.to_device(np.random.default_rng(0).standard_normal((dA.n, b)).astype(dt))
+    Y = ctx.empty((dA.n, b), dt)
+    val, _, _ = dA.ptrs(dtype)
+    for _ in range(3):
+        L.check(L.hip().trl_hip_spmm_csr(ctx.h, L.dtype_code(dt), dA.n, b, dA.rowptr.ptr, dA.col.ptr, val, X.ptr, Y.ptr, None, 0), "spmm")
+    ctx.prof_enable(True)
+    for _ in range(reps):
+        L.check(L.hip().trl_hip_spmm_csr(ctx.h, L.dtype_code(dt), dA.n, b, dA.rowptr.ptr, dA.col.ptr, val, X.ptr, Y.ptr, None, 0), "spmm")
+    st = ctx.prof_read()["spmm"]
+    ctx.prof_enable(False)
+    avg = st["seconds"] / st["calls"]
+    per = st["bytes"] / st["calls"]
+    return {"dtype": dtype, "launches": st["calls"], "avg_us": avg * 1e6, "bytes_per_launch": per, "GBps": per / avg / 1e9, "frac_of_peak": per / avg / 1e9 / HBM_PEAK_GBPS}
+
+
+def cpu_baseline(cfg, fp64_cycles, budget_s):
+    """OpenMP CPU oracle on the box's host cores: the first blocks of one FP64
+    thick-restart cycle of the same problem, extrapolated by bytes."""
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    oracle.set_threads(cores)
+    A = cfg.csr
+    b, ncv = cfg.b, cfg.ncv
+    n = A.n
+    P = n * b * 8.0
+    SA = A.nnz * 12.0 + (n + 1) * 4.0 + 2 * P
+    m = ncv // b
+
+    def step_bytes(i):  # apply, 3-term ops, MGS over i+1 blocks, CholQR, per lanczos.hh:244-284
+        return SA + 12 * P + 5 * (i + 1) * P + 6 * P
+
+    V0 = __import__("trl_amd.problems", fromlist=["x"]).initial_block(n, b)
+    steps = 2
+    t = oracle.time_extend(A, b, cfg.nev, ncv, V0, steps)
+    # grow the sample while it stays inside the budget
+    while steps < m and t * 2.2 < budget_s:
+        steps = min(m, steps * 2)
+        t = oracle.time_extend(A, b, cfg.nev, ncv, V0, steps)
+    frac = sum(step_bytes(i) for i in range(steps)) / sum(step_bytes(i) for i in range(m))
+    t_cycle = t / frac
+    return {
+        "value": t_cycle * fp64_cycles,
+        "unit": "s (estimated time to solution)",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"OpenMP oracle, extend over the first {steps} of {m} blocks of one FP64 cycle of the same problem ({t:.2f} s), scaled by "
+        f"algorithmic bytes to a full cycle ({t_cycle:.2f} s) and multiplied by the {fp64_cycles} cycles the GPU FP64 solve needed",
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--side", type=int, default=160, help="grid side of the 3D Laplacian (160 = the named workload)")
+    ap.add_argument("--tol-fp32", type=float, default=1e-5)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported (see DESIGN.md for how the path would shard)"}))
+        return 2
+
+    from trl_amd import _lib as L
+    from trl_amd import problems as P
+    from trl_amd.solver import DeviceCSR, EigensolverParams, solve, solve_mixed
+
+    t0 = time.time()
+    cfg = P.config_c3(a.side)
+    log(f"generated {cfg.name}: n={cfg.csr.n} nnz={cfg.csr.nnz} in {time.time() - t0:.1f}s")
+    ctx = L.Context(0)
+    dA = DeviceCSR(ctx, cfg.csr)
+    V0h = P.initial_block(cfg.csr.n, cfg.b)
+    V0 = ctx.to_device(V0h)
+    params = EigensolverParams(cfg.nev, cfg.ncv, max_restarts=2000, tolerance=cfg.tol)
+    log("inputs resident in HBM")
+
+    # ---- SpMM rate (HIP events on the library stream) -------------------------
+    sp32 = spmm_rate(ctx, dA, cfg.b, "f32")
+    sp64 = spmm_rate(ctx, dA, cfg.b, "f64")
+    log(f"spmm f32 {sp32['GBps']:.0f} GB/s ({sp32['avg_us']:.1f} us), f64 {sp64['GBps']:.0f} GB/s")
+
+    # ---- warmup + profiled pass -------------------------------------------------
+    for w in range(a.warmup):
+        solve_mixed(dA, cfg.b, params, V0, tol_fp32=a.tol_fp32)
+        log(f"warmup {w} done")
+    ctx.prof_enable(True)
+    prof_out = solve_mixed(dA, cfg.b, params, V0, tol_fp32=a.tol_fp32)
+    fam = ctx.prof_read()
+    ctx.prof_enable(False)
+    log(f"profiled mixed solve: {prof_out.result['seconds']:.2f} s, fp32 cycles {prof_out.result32['iterations']}, fp64 cycles {prof_out.result['iterations']}")
+
+    # ---- timed steps --------------------------------------------------------------
+    ctx.sync()
+    times, outs = [], []
+    for s in range(a.steps):
+        ctx.sync()
+        t = time.perf_counter()
+        o = solve_mixed(dA, cfg.b, params, V0, tol_fp32=a.tol_fp32)
+        ctx.sync()
+        times.append(time.perf_counter() - t)
+        outs.append(o)
+        log(f"step {s}: {times[-1]:.3f} s, converged={o.result['converged']}, fp32 cycles {o.result32['iterations']}, fp64 cycles {o.result['iterations']}, "
+            f"max resid {o.resid.max():.2e}")
+    t_mixed = float(np.mean(times))
+    last = outs[-1]
+
+    # ---- plain FP64 solve -----------------------------------------------------------
+    fp64 = None
+    if not a.no_fp64:
+        ctx.sync()
+        t = time.perf_counter()
+        o64 = solve(dA, cfg.b, params, V0, dtype="f64", order="ascending", reorth="cgs2")
+        ctx.sync()
+        t64 = time.perf_counter() - t
+        agree = float(np.max(np.abs(o64.evals - last.evals) / np.abs(o64.evals)))
+        fp64 = {"seconds": t64, "converged": o64.result["converged"], "cycles": o64.result["iterations"], "n_op_apply": o64.result["n_op_apply"],
+                "max_resid": float(o64.resid.max()), "mixed_vs_fp64_max_rel_eig_diff": agree}
+        log(f"fp64 solve {t64:.2f} s, cycles {o64.result['iterations']}, eig agreement {agree:.2e}")
+
+    # ---- roofline of the dominant kernel family ---------------------------------------
+    busy = {k: v for k, v in fam.items() if v["calls"] and v["bytes"] > 0}
+    dom = max(busy, key=lambda k: busy[k]["seconds"])
+    d = busy[dom]
+    achieved = d["bytes"] / d["seconds"] / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "launches": d["calls"], "avg_launch_us": d["seconds"] / d["calls"] * 1e6, "algorithmic_bytes_per_launch": d["bytes"] / d["calls"]}
+    families = {k: {"calls": v["calls"], "seconds": round(v["seconds"], 6), "GBps": (v["bytes"] / v["seconds"] / 1e9) if v["seconds"] and v["bytes"] else None}
+                for k, v in fam.items() if v["calls"]}
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cycles = fp64["cycles"] if fp64 else last.result["iterations"]
+            cpu = cpu_baseline(cfg, cycles, a.cpu_budget)
+            log(f"cpu baseline {cpu}")
+        except Exception as e:  # the CPU leg is a reported baseline only
+            log(f"cpu baseline failed: {e}")
+
+    out = {
+        "metric": METRIC,
+        "value": t_mixed,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (mixed)",
+        "data": "synthetic",
+        "config": {"workload": f"{cfg.name}: 3D 7-point Laplacian {a.side}^3 + diag U[0,1]", "n": cfg.csr.n, "nnz": cfg.csr.nnz, "b": cfg.b, "nev": cfg.nev,
+                   "ncv": cfg.ncv, "tol": cfg.tol, "order": "smallest", "tol_fp32": a.tol_fp32},
+        "converged": bool(last.result["converged"]),
+        "max_true_residual": float(last.resid.max()),
+        "fp32_phase": last.result32,
+        "fp64_phase": last.result,
+        "fp64_solve": fp64,
+        "speedup_vs_fp64": (fp64["seconds"] / t_mixed) if fp64 else None,
+        "spmm": {"f32": sp32, "f64": sp64},
+        "roofline": roof,
+        "kernel_families": families,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

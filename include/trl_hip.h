@@ -44,6 +44,8 @@ extern "C" {
 
 typedef enum { TRL_HIP_F32 = 0, TRL_HIP_F64 = 1 } trl_hip_dtype;
 
+#define TRL_HIP_PROF_FAMILIES 8
+
 typedef struct trl_hip_ctx trl_hip_ctx;
 
 /* ---- context, memory ---------------------------------------------------- */
@@ -170,6 +172,28 @@ int trl_hip_sddmm_adjoint(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int
  * values (may be NULL: lambda = 0).  mdiag may be NULL.  Used for the true
  * residual of Ritz pairs (Y = A X). */
 int trl_hip_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* mdiag, const double* lambda, double* out);
+
+/* ---- profiling (no reference counterpart) --------------------------------- */
+
+/* Kernel families timed in profiling mode.  Algorithmic bytes per call
+ * (P = n*b*sizeof(scalar)): SPMM nnz*(sizeof(val)+4) + (n+1)*4 + 2P
+ * (+ n*sizeof(scalar) with a row scaling); GRAM (k+1)P (+ mass); UPDATE
+ * (k+2)P; CHOLQR 3P per pass (+ mass); ROTATE (m+kc)P; PANEL 2P or 3P. */
+#define TRL_HIP_PROF_SPMM 0
+#define TRL_HIP_PROF_GRAM 1
+#define TRL_HIP_PROF_UPDATE 2
+#define TRL_HIP_PROF_CHOLQR 3
+#define TRL_HIP_PROF_ROTATE 4
+#define TRL_HIP_PROF_PANEL 5
+#define TRL_HIP_PROF_SMALL 6
+#define TRL_HIP_PROF_OTHER 7
+
+/* on = 1 clears the counters and starts recording a HIP event pair on the
+ * context stream around every call of the families above; on = 0 stops. */
+int trl_hip_prof_enable(trl_hip_ctx* ctx, int on);
+/* Synchronous: folds finished calls in and returns the totals of one family
+ * since the last enable: calls, seconds between the event pairs, bytes. */
+int trl_hip_prof_read(trl_hip_ctx* ctx, int family, long long* calls, double* seconds, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -53,8 +53,8 @@ def lib() -> C.CDLL:
         L.oracle_lanczos_solve.restype = C.c_int
         L.oracle_lanczos_extend.argtypes = [C.c_size_t, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_uint, C.c_uint, _dp, _dp, _dp, _dp]
         L.oracle_lanczos_extend.restype = C.c_int
-        L.oracle_time_cycle.argtypes = [C.c_size_t, _ip, _ip, _dp, C.c_int, C.c_uint, C.c_uint, _dp]
-        L.oracle_time_cycle.restype = C.c_double
+        L.oracle_time_extend.argtypes = [C.c_size_t, _ip, _ip, _dp, C.c_int, C.c_uint, C.c_uint, _dp, C.c_uint]
+        L.oracle_time_extend.restype = C.c_double
         _lib = L
     return _lib
 
@@ -177,5 +177,6 @@ def lanczos_extend(csr, b, nev, ncv, V0, mass=None):
     return rc, V, T, beta
 
 
-def time_cycle(csr, b, nev, ncv, V0) -> float:
-    return float(lib().oracle_time_cycle(csr.n, _i(csr.rowptr), _i(csr.col), _d(f64(csr.val)), b, nev, ncv, _d(f64(V0).reshape(csr.n, b))))
+def time_extend(csr, b, nev, ncv, V0, steps) -> float:
+    """Seconds of the oracle's extend(0, steps) on this problem."""
+    return float(lib().oracle_time_extend(csr.n, _i(csr.rowptr), _i(csr.col), _d(f64(csr.val)), b, nev, ncv, _d(f64(V0).reshape(csr.n, b)), steps))

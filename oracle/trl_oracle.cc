@@ -705,19 +705,19 @@ int oracle_lanczos_extend(size_t n, const int32_t* rowptr, const int32_t* col, c
   return 0;
 }
 
-// One timed thick-restart cycle (extend 0 -> ncv/b plus the Rayleigh-Ritz
-// solve) for the CPU baseline; returns seconds.
-double oracle_time_cycle(size_t n, const int32_t* rowptr, const int32_t* col, const double* val, int b, unsigned nev, unsigned ncv, const double* V0)
+// Timed extend(0, steps) of an FP64 thick-restart cycle (the CPU baseline
+// sample); returns seconds.
+double oracle_time_extend(size_t n, const int32_t* rowptr, const int32_t* col, const double* val, int b, unsigned nev, unsigned ncv, const double* V0,
+                          unsigned steps)
 {
   oracle::CsrOp A{n, rowptr, col, val, nullptr, nullptr};
   oracle::Evp evp(A, b);
   evp.order = oracle::Ascending;
   oracle::Lanczos lz(evp, nev, ncv, 1);
-  if (lz.error) return -1.0;
+  if (lz.error || steps < 1 || steps > ncv / unsigned(b)) return -1.0;
   std::memcpy(lz.V.data(), V0, sizeof(double) * n * b);
   const double t0 = omp_get_wtime();
-  lz.extend(0, ncv / b);
-  evp.solve_small_dense(lz.T, lz.B.blk(0, 0), nev);
+  lz.extend(0, steps);
   return omp_get_wtime() - t0;
 }
 

@@ -31,8 +31,23 @@ def rel(a, b):
     return np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+# Every device array a test creates stays referenced until the test ends:
+# an inline temporary (dev(...).ptr) would otherwise be freed by its
+# destructor before the queued kernel reads it.
+_KEEP = []
+
+
+@pytest.fixture(autouse=True)
+def _keep_device_arrays(ctx):
+    yield
+    ctx.sync()
+    _KEEP.clear()
+
+
 def dev(ctx, a, dt):
-    return ctx.to_device(np.ascontiguousarray(a, dtype=dt))
+    d = ctx.to_device(np.ascontiguousarray(a, dtype=dt))
+    _KEEP.append(d)
+    return d
 
 
 @pytest.mark.parametrize("p", ["f64", "f32"])

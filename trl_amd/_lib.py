@@ -52,6 +52,8 @@ HIP_API = [
     ("trl_hip_small_set_diagonal", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_small_mult", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_residual_sq", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_prof_enable", c_int, [c_void_p, c_int]),
+    ("trl_hip_prof_read", c_int, [c_void_p, c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("trl_hip_sddmm_adjoint", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 ]
 
@@ -171,6 +173,19 @@ class Context:
         if rc not in (0, -2):
             check(rc, "check")
         return piv.value
+
+    PROF_FAMILIES = ["spmm", "gram", "update", "cholqr", "rotate", "panel", "small", "other"]
+
+    def prof_enable(self, on: bool = True) -> None:
+        check(hip().trl_hip_prof_enable(self.h, 1 if on else 0), "prof_enable")
+
+    def prof_read(self) -> dict:
+        out = {}
+        for f, name in enumerate(self.PROF_FAMILIES):
+            n, s, b = C.c_longlong(), C.c_double(), C.c_double()
+            check(hip().trl_hip_prof_read(self.h, f, C.byref(n), C.byref(s), C.byref(b)), "prof_read")
+            out[name] = {"calls": n.value, "seconds": s.value, "bytes": b.value}
+        return out
 
     @property
     def num_cus(self) -> int:

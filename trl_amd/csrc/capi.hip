@@ -25,6 +25,70 @@ int ensure_partials(trl_hip_ctx* ctx, size_t doubles)
   return 0;
 }
 
+// RAII event pair around one queued call (profiling mode only).
+struct Prof {
+  trl_hip_ctx* ctx;
+  int family;
+  double bytes;
+  hipEvent_t start = nullptr;
+  Prof(trl_hip_ctx* c, int fam, double b) : ctx(c), family(fam), bytes(b)
+  {
+    if (!ctx->prof_on) return;
+    start = take();
+    if (start) (void)hipEventRecord(start, ctx->stream);
+  }
+  hipEvent_t take()
+  {
+    if (!ctx->prof_pool.empty()) {
+      hipEvent_t e = ctx->prof_pool.back();
+      ctx->prof_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  ~Prof()
+  {
+    if (!ctx->prof_on || !start) return;
+    hipEvent_t stop = take();
+    if (!stop) return;
+    (void)hipEventRecord(stop, ctx->stream);
+    ctx->prof_pending.push_back({start, stop, family, bytes});
+  }
+};
+
+int prof_fold(trl_hip_ctx* ctx)
+{
+  if (ctx->prof_pending.empty()) return 0;
+  TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->prof_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) {
+      ctx->prof_seconds[r.family] += 1e-3 * ms;
+      ctx->prof_bytes[r.family] += r.bytes;
+      ctx->prof_count[r.family] += 1;
+    }
+    ctx->prof_pool.push_back(r.start);
+    ctx->prof_pool.push_back(r.stop);
+  }
+  ctx->prof_pending.clear();
+  return 0;
+}
+
+double panel_bytes(trl_hip_dtype t, size_t n, int b) { return double(n) * b * (t == TRL_HIP_F32 ? 4.0 : 8.0); }
+
+long long csr_nnz(trl_hip_ctx* ctx, const int32_t* rowptr, size_t n)
+{
+  auto it = ctx->nnz_cache.find(rowptr);
+  if (it != ctx->nnz_cache.end()) return it->second;
+  int32_t last = 0;
+  if (hipMemcpyAsync(&last, rowptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) return 0;
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->nnz_cache[rowptr] = last;
+  return last;
+}
+
 } // namespace trl_hip_impl
 
 extern "C" {
@@ -62,7 +126,10 @@ int trl_hip_ctx_create(int device, trl_hip_ctx** out)
   if (!ctx) return TRL_HIP_ENOMEM;
   ctx->device = device;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->num_cus = prop.multiProcessorCount;
+    if (prop.sharedMemPerBlock > 0) ctx->max_lds = prop.sharedMemPerBlock;
+  }
   // default-priority, non-blocking in-order stream
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&ctx->flags, 64 * sizeof(int));
@@ -91,6 +158,11 @@ int trl_hip_ctx_destroy(trl_hip_ctx* ctx)
   if (ctx->flags) (void)hipFree(ctx->flags);
   if (ctx->small) (void)hipFree(ctx->small);
   if (ctx->coef) (void)hipFree(ctx->coef);
+  for (auto& r : ctx->prof_pending) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
@@ -193,6 +265,7 @@ int trl_hip_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* 
 {
   if (!ctx || !valid_b(b) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_PANEL, 3 * panel_bytes(t, n, b));
   return launch_panel_sub(ctx, t, n, b, X, Y);
 }
 
@@ -200,12 +273,14 @@ int trl_hip_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void*
 {
   if (!ctx || !valid_b(b) || mode < 0 || mode > 3 || (n && (!X || !W || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_PANEL, ((mode == 1 || mode == 3) ? 3 : 2) * panel_bytes(t, n, b));
   return launch_panel_mult(ctx, t, n, b, X, W, Y, mode);
 }
 
 int trl_hip_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* mdiag, double* C)
 {
   if (!ctx || !valid_b(b) || k < 1 || !C || (n && (!V || !W))) return TRL_HIP_EINVAL;
+  Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0));
   return launch_gram(ctx, t, n, b, k, V, W, mdiag, C);
 }
 
@@ -213,12 +288,14 @@ int trl_hip_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
 {
   if (!ctx || !valid_b(b) || k < 1 || !C || (n && (!V || !W))) return TRL_HIP_EINVAL;
   if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * panel_bytes(t, n, b));
   return launch_update(ctx, t, n, b, k, V, C, W);
 }
 
 int trl_hip_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* mdiag, double* R, int passes)
 {
   if (!ctx || !valid_b(b) || !R || passes < 1 || passes > 2 || (n && !W)) return TRL_HIP_EINVAL;
+  Prof pr(ctx, TRL_HIP_PROF_CHOLQR, passes * (3 * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0)));
   return launch_cholqr(ctx, t, n, b, W, mdiag, R, passes);
 }
 
@@ -226,6 +303,7 @@ int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, co
 {
   if (!ctx || !valid_b(b) || m < 1 || kc < 1 || y_block_cols < kc || !Y || (n && (!V || !Out))) return TRL_HIP_EINVAL;
   if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_ROTATE, (m + kc) * panel_bytes(t, n, b));
   return launch_rotate(ctx, t, n, b, m, V, Y, y_block_cols, kc, Out);
 }
 
@@ -236,6 +314,12 @@ int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const i
   if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16) return TRL_HIP_EINVAL;
   if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
   if (!n) return 0;
+  double bytes = 0.0;
+  if (ctx->prof_on) {
+    const double s = (t == TRL_HIP_F32) ? 4.0 : 8.0;
+    bytes = double(csr_nnz(ctx, rowptr, n)) * (s + 4.0) + double(n + 1) * 4.0 + 2 * panel_bytes(t, n, b) + (dscale ? double(n) * s : 0.0);
+  }
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, bytes);
   return launch_spmm_csr(ctx, t, n, b, rowptr, col, val, X, Y, dscale, lanes_per_row);
 }
 
@@ -245,6 +329,8 @@ int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int wid
   if (!ctx || !valid_b(b) || width < 0 || (n && (!X || !Y)) || (n && width && (!ell_col || !ell_val))) return TRL_HIP_EINVAL;
   if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
   if (!n) return 0;
+  const double s = (t == TRL_HIP_F32) ? 4.0 : 8.0;
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, double(n) * width * (s + 4.0) + 2 * panel_bytes(t, n, b) + (dscale ? double(n) * s : 0.0));
   return launch_spmm_ell(ctx, t, n, b, width, ell_col, ell_val, X, Y, dscale);
 }
 
@@ -252,24 +338,28 @@ int trl_hip_small_copy(trl_hip_ctx* ctx, int b, const double* src, double* dst, 
 {
   if (!ctx || b < 1 || b > 16 || !src || !dst) return TRL_HIP_EINVAL;
   if (!transpose) return trl_hip_memcpy_d2d(ctx, dst, src, size_t(b) * b * sizeof(double));
+  Prof pr(ctx, TRL_HIP_PROF_SMALL, 0.0);
   return launch_small(ctx, 1, b, src, nullptr, dst, nullptr);
 }
 
 int trl_hip_small_zero(trl_hip_ctx* ctx, int b, double* dst)
 {
   if (!ctx || b < 1 || b > 16 || !dst) return TRL_HIP_EINVAL;
+  Prof pr(ctx, TRL_HIP_PROF_SMALL, 0.0);
   return launch_small(ctx, 0, b, nullptr, nullptr, dst, nullptr);
 }
 
 int trl_hip_small_set_diagonal(trl_hip_ctx* ctx, int b, double* dst, const double* host_values)
 {
   if (!ctx || b < 1 || b > 16 || !dst || !host_values) return TRL_HIP_EINVAL;
+  Prof pr(ctx, TRL_HIP_PROF_SMALL, 0.0);
   return launch_small(ctx, 2, b, nullptr, nullptr, dst, host_values);
 }
 
 int trl_hip_small_mult(trl_hip_ctx* ctx, int b, const double* A, const double* B, double* C)
 {
   if (!ctx || b < 1 || b > 16 || !A || !B || !C) return TRL_HIP_EINVAL;
+  Prof pr(ctx, TRL_HIP_PROF_SMALL, 0.0);
   return launch_small(ctx, 3, b, A, B, C, nullptr);
 }
 
@@ -277,6 +367,34 @@ int trl_hip_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, cons
 {
   if (!ctx || !valid_b(b) || !out || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   return launch_residual_sq(ctx, t, n, b, X, Y, mdiag, lambda, out);
+}
+
+int trl_hip_prof_enable(trl_hip_ctx* ctx, int on)
+{
+  if (!ctx) return TRL_HIP_EINVAL;
+  int rc = prof_fold(ctx);
+  if (rc) return rc;
+  if (on) {
+    ctx->nnz_cache.clear();
+    for (int f = 0; f < TRL_HIP_PROF_FAMILIES; ++f) {
+      ctx->prof_count[f] = 0;
+      ctx->prof_seconds[f] = 0.0;
+      ctx->prof_bytes[f] = 0.0;
+    }
+  }
+  ctx->prof_on = on ? 1 : 0;
+  return 0;
+}
+
+int trl_hip_prof_read(trl_hip_ctx* ctx, int family, long long* calls, double* seconds, double* bytes)
+{
+  if (!ctx || family < 0 || family >= TRL_HIP_PROF_FAMILIES) return TRL_HIP_EINVAL;
+  int rc = prof_fold(ctx);
+  if (rc) return rc;
+  if (calls) *calls = ctx->prof_count[family];
+  if (seconds) *seconds = ctx->prof_seconds[family];
+  if (bytes) *bytes = ctx->prof_bytes[family];
+  return 0;
 }
 
 } // extern "C"

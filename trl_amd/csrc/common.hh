@@ -5,6 +5,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <unordered_map>
+#include <vector>
 
 #include "trl_hip.h"
 
@@ -14,6 +16,7 @@ struct trl_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 256;
+  size_t max_lds = 65536; // dynamic LDS bytes a workgroup may request
   // per-workgroup partial sums of the block products (FP64)
   double* partials = nullptr;
   size_t partials_cap = 0; // in doubles
@@ -24,6 +27,19 @@ struct trl_hip_ctx {
   // coefficient scratch for multi-block products (grown on demand)
   double* coef = nullptr;
   size_t coef_cap = 0;
+  // profiling mode: HIP events around every queued kernel family call
+  struct ProfRec {
+    hipEvent_t start, stop;
+    int family;
+    double bytes;
+  };
+  int prof_on = 0;
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  long long prof_count[TRL_HIP_PROF_FAMILIES] = {};
+  double prof_seconds[TRL_HIP_PROF_FAMILIES] = {};
+  double prof_bytes[TRL_HIP_PROF_FAMILIES] = {};
+  std::unordered_map<const void*, long long> nnz_cache;
 };
 
 namespace trl_hip_impl {

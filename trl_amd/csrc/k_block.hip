@@ -5,8 +5,11 @@
 //   rotate   Out_j = sum_i V_i Y(i, j)          (MFMA, Y in LDS)
 //
 // MFMA operand layout used throughout (v_mfma_{f32,f64}_16x16x4): lane l
-// supplies A[i = l % 16][k = l / 16] and B[k = l / 16][j = l % 16]; the 16x16
-// result D[i][j] lives in lane j + 16 * (i / 4), element i % 4.
+// supplies A[i = l % 16][k = l / 16] and B[k = l / 16][j = l % 16].  The
+// 16x16 result D[i][j] lives in lane j + 16*q, element e, with
+//   FP32: i = 4*q + e   (four consecutive rows per lane)
+//   FP64: i = 4*e + q   (rows strided by four)
+// (the f32/f64 output groupings of the 16x16x4 instructions differ).
 #include "common.hh"
 
 using namespace trl_hip_dev;
@@ -30,6 +33,13 @@ struct Mfma<float> {
 };
 
 constexpr int pad_b(int b) { return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : 16; }
+
+// Row of the 16x16 MFMA result held by element e of a lane in row group q.
+template <typename S>
+__device__ constexpr int mfma_row(int q, int e)
+{
+  return sizeof(S) == 8 ? 4 * e + q : 4 * q + e;
+}
 
 constexpr int kThreads = 256;
 // FP32 Gram: rows folded into the compensated accumulator every kChunk quads
@@ -137,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size
     if (t >= ntile) continue;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int i = 4 * q4 + e;
+      const int i = mfma_row<S>(q4, e);
       const int blk = t * BPT + i / BP, ci = i % BP;
       if (blk < k && ci < B) {
         double v = static_cast<double>(acc[q][e]);
@@ -348,7 +358,7 @@ __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t 
       if (jn >= NC) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const size_t row = r0 + 4 * q4 + e;
+        const size_t row = r0 + mfma_row<S>(q4, e);
         if (row < n) Out[size_t(jn / B) * ldv + row * B + (jn % B)] = static_cast<S>(acc[t][e]);
       }
     }
@@ -514,7 +524,7 @@ int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, con
     TRL_DISPATCH_B(b, {
       const int K = m * B;
       // output blocks per launch: NT*16 >= nb*B, LDS = K * NT*16 * sizeof(S) <= 144 KiB
-      const size_t lds_cap = 144 * 1024;
+      const size_t lds_cap = ctx->max_lds < size_t(144 * 1024) ? ctx->max_lds : size_t(144 * 1024);
       int nt_fit = int(lds_cap / (size_t(K) * 16 * sizeof(S)));
       if (nt_fit < 1) return TRL_HIP_EINVAL;
       if (nt_fit > 4) nt_fit = 4;
