@@ -179,16 +179,18 @@ def test_gram_deterministic(ctx, rng):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
-def test_gram_fp32_compensated(ctx):
-    # a long FP32 dot whose naive float sum loses digits: sum of 2^22 ones plus
-    # a small residue per row must come out exact to FP64 rounding
+def test_gram_fp32_long_sum_exact(ctx):
+    # 2^22 rows of x = 1, y = 1 + 2^-10.  Every partial sum inside one FP32 MFMA
+    # chunk (<= 32 rows) is exactly representable, and the chunk sums are folded
+    # into an error-free (hi, lo) pair, then summed across waves in FP64: the
+    # result must be exact.  A plain FP32 running sum would drop the 2^-10
+    # residues once it passes 2^13.
     n, b = 1 << 22, 1
     x = np.ones((n, b), np.float32)
-    y = np.full((n, b), 1.0 + 2.0**-20, np.float32)
+    y = np.full((n, b), 1.0 + 2.0**-10, np.float32)
     Cd = ctx.empty((1, 1))
     L.check(H().trl_hip_gram(ctx.h, L.F32, n, b, 1, dev(ctx, x, np.float32).ptr, dev(ctx, y, np.float32).ptr, None, Cd.ptr), "gram")
-    exact = n * (1.0 + 2.0**-20)
-    assert abs(Cd.download()[0, 0] - exact) / exact < 1e-12
+    assert Cd.download()[0, 0] == n * (1.0 + 2.0**-10)
 
 
 @pytest.mark.parametrize("p", ["f64", "f32"])
