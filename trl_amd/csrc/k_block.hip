@@ -42,8 +42,10 @@ __device__ constexpr int mfma_row(int q, int e)
 }
 
 constexpr int kThreads = 256;
-// FP32 Gram: rows folded into the compensated accumulator every kChunk quads
-constexpr int kChunk = 8;
+// Gram: row quads per load group (16 rows)
+constexpr int kU = 4;
+// Gram workgroups per compute unit
+constexpr int kGramWgPerCu = 4;
 
 // ---------------------------------------------------------------------------
 // Gram.  A tile stacks 16/BP blocks of V (columns padded to BP = pow2(b));
@@ -76,11 +78,15 @@ __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size
   }
   const bool wval = i16 < B;
 
+  // Each row wave walks groups of kU consecutive row quads (4*kU rows); all
+  // loads of a group are issued before its MFMAs so that enough bytes are in
+  // flight to cover HBM latency.
   const size_t nquads = (n + 3) / 4;
-  const size_t qstride = size_t(gridDim.x) * RS;
-  size_t qd = size_t(blockIdx.x) * RS + rs;
+  const size_t ngroups = (nquads + kU - 1) / kU;
+  const size_t gstride = size_t(gridDim.x) * RS;
 
-  // FP64: one accumulator; FP32: chunk accumulator folded into (hi, lo) by TwoSum
+  // FP64: one accumulator; FP32: the MFMA accumulates one group (16 rows) in
+  // FP32 and is folded into an error-free (hi, lo) pair by TwoSum
   acc_t acc[TPW];
   acc_t lo[TPW];
 #pragma unroll
@@ -89,43 +95,35 @@ __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size
     lo[q] = acc_t{0, 0, 0, 0};
   }
 
-  if constexpr (sizeof(S) == 8) {
-    for (; qd < nquads; qd += qstride) {
-      const size_t r = qd * 4 + q4;
+  for (size_t g = size_t(blockIdx.x) * RS + rs; g < ngroups; g += gstride) {
+    S bw[kU];
+    S a[kU][TPW];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const size_t r = (g * kU + u) * 4 + q4;
       const bool rok = r < n;
-      S bw = 0;
+      bw[u] = 0;
       if (rok && wval) {
-        bw = W[r * B + i16];
-        if (md) bw *= md[r];
+        bw[u] = W[r * B + i16];
+        if (md) bw[u] *= md[r];
       }
 #pragma unroll
-      for (int q = 0; q < TPW; ++q) {
-        S a = 0;
-        if (rok && aval[q]) a = V[aoff[q] + r * B];
-        acc[q] = M::op(a, bw, acc[q]);
-      }
+      for (int q = 0; q < TPW; ++q) a[u][q] = (rok && aval[q]) ? V[aoff[q] + r * B] : S(0);
     }
-  }
-  else {
-    while (qd < nquads) {
+    if constexpr (sizeof(S) == 8) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) acc[q] = M::op(a[u][q], bw[u], acc[q]);
+    }
+    else {
       acc_t cur[TPW];
 #pragma unroll
       for (int q = 0; q < TPW; ++q) cur[q] = acc_t{0, 0, 0, 0};
-      for (int it = 0; it < kChunk && qd < nquads; ++it, qd += qstride) {
-        const size_t r = qd * 4 + q4;
-        const bool rok = r < n;
-        S bw = 0;
-        if (rok && wval) {
-          bw = W[r * B + i16];
-          if (md) bw *= md[r];
-        }
 #pragma unroll
-        for (int q = 0; q < TPW; ++q) {
-          S a = 0;
-          if (rok && aval[q]) a = V[aoff[q] + r * B];
-          cur[q] = M::op(a, bw, cur[q]);
-        }
-      }
+      for (int u = 0; u < kU; ++u)
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) cur[q] = M::op(a[u][q], bw[u], cur[q]);
 #pragma unroll
       for (int q = 0; q < TPW; ++q)
 #pragma unroll
@@ -158,46 +156,30 @@ __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size
   }
 }
 
-// out[p] = sum_{w < nw} partials[w*P + p], summed in fixed order (deterministic).
+// out[p] = sum_{w < nw} partials[w*P + p], summed in fixed order (deterministic):
+// 16 entries per workgroup, 16 lanes striding over w, combined in fixed order.
 __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ partials, int nw, int P, double* __restrict__ out)
 {
-  __shared__ double red[4][64];
-  const int pl = threadIdx.x & 63, s = threadIdx.x >> 6;
-  const int p = blockIdx.x * 64 + pl;
+  __shared__ double red[16][17];
+  const int pl = threadIdx.x & 15, s = threadIdx.x >> 4;
+  const int p = blockIdx.x * 16 + pl;
   double acc = 0.0;
   if (p < P)
-    for (int w = s; w < nw; w += 4) acc += partials[size_t(w) * P + p];
+    for (int w = s; w < nw; w += 16) acc += partials[size_t(w) * P + p];
   red[s][pl] = acc;
   __syncthreads();
-  if (s == 0 && p < P) out[p] = (red[0][pl] + red[1][pl]) + (red[2][pl] + red[3][pl]);
+  if (s == 0 && p < P) {
+    double t = 0.0;
+    for (int q = 0; q < 16; ++q) t += red[q][pl];
+    out[p] = t;
+  }
 }
 
-// Sum the b x b Gram partials, factor G = R^T R in FP64 (lower triangle of G
-// read, as an LLT does), write R (times Rprev if given) and R^{-1}.  On
-// breakdown the sticky flag receives the 1-based pivot, R and R^{-1} are set
-// to the identity so the following scale leaves W unchanged.
-__global__ __launch_bounds__(kThreads) void k_chol(const double* __restrict__ partials, int nw, int b, const double* __restrict__ Rprev,
-                                                   double* __restrict__ Rout, double* __restrict__ Rinv, int* __restrict__ flags)
+// Factor G = R^T R (G symmetric b x b in LDS, lower triangle read as an LLT
+// does), set Rout = R Rprev (or R) and Rinv = R^{-1}; on breakdown record the
+// 1-based pivot in the sticky flag and return identities.  One thread.
+__device__ void chol_bxb(const double* G, int b, const double* Rprev, double* Rout, double* Rinv, int* flags, double (*R)[16], double (*X)[16])
 {
-  __shared__ double red[4][64];
-  __shared__ double G[256];
-  const int P = b * b;
-  for (int base = 0; base < P; base += 64) {
-    const int pl = threadIdx.x & 63, s = threadIdx.x >> 6;
-    const int p = base + pl;
-    double acc = 0.0;
-    if (p < P)
-      for (int w = s; w < nw; w += 4) acc += partials[size_t(w) * P + p];
-    red[s][pl] = acc;
-    __syncthreads();
-    if (s == 0 && p < P) G[p] = (red[0][pl] + red[1][pl]) + (red[2][pl] + red[3][pl]);
-    __syncthreads();
-  }
-  __shared__ double Rl[16][16];
-  __shared__ double Xl[16][16];
-  if (threadIdx.x != 0) return;
-  auto& R = Rl;
-  auto& X = Xl;
   int fail = 0;
   for (int i = 0; i < 16; ++i)
     for (int j = 0; j < 16; ++j) R[i][j] = X[i][j] = 0.0;
@@ -225,7 +207,6 @@ __global__ __launch_bounds__(kThreads) void k_chol(const double* __restrict__ pa
       }
     return;
   }
-  // X = R^{-1} (upper triangular), column by column
   for (int j = 0; j < b; ++j) {
     X[j][j] = 1.0 / R[j][j];
     for (int i = j - 1; i >= 0; --i) {
@@ -236,17 +217,119 @@ __global__ __launch_bounds__(kThreads) void k_chol(const double* __restrict__ pa
   }
   for (int i = 0; i < b; ++i)
     for (int j = 0; j < b; ++j) Rinv[i * b + j] = X[i][j];
-  if (Rprev) {
-    for (int i = 0; i < b; ++i)
-      for (int j = 0; j < b; ++j) {
-        double s = 0.0;
-        for (int p = i; p <= j; ++p) s += R[i][p] * Rprev[p * b + j];
-        Rout[i * b + j] = (j >= i) ? s : 0.0;
+  for (int i = 0; i < b; ++i)
+    for (int j = 0; j < b; ++j) {
+      double v = 0.0;
+      if (j >= i) {
+        if (Rprev)
+          for (int p = i; p <= j; ++p) v += R[i][p] * Rprev[p * b + j];
+        else
+          v = R[i][j];
       }
+      Rout[i * b + j] = v;
+    }
+}
+
+// One Cholesky-QR pass over the panel W (n x B), one row per thread:
+//   SCALE: W <- W Rinv_prev first (in place, FP64 arithmetic);
+//   G = W^T diag(m) W accumulated in FP64 over the stored values (upper
+//   triangle, B(B+1)/2 entries), reduced per workgroup (shuffles, then LDS);
+//   the last workgroup to finish sums the workgroup partials in fixed order
+//   and factors G on the device (chol_bxb), then re-arms the counter.
+template <typename S, int B, bool SCALE>
+__global__ __launch_bounds__(kThreads) void k_cholqr_pass(S* __restrict__ W, const S* __restrict__ md, size_t n, const double* __restrict__ Rinv_prev,
+                                                          double* __restrict__ partials, unsigned* __restrict__ counter, const double* __restrict__ Rprev,
+                                                          double* __restrict__ Rout, double* __restrict__ Rinv_out, int* __restrict__ flags)
+{
+  constexpr int NP = B * (B + 1) / 2;
+  __shared__ double Rs[B * B];
+  __shared__ double red[kThreads / 64][NP];
+  __shared__ double Gs[B * B];
+  __shared__ double Rl[16][16], Xl[16][16];
+  __shared__ unsigned ticket;
+  if constexpr (SCALE) {
+    for (int i = threadIdx.x; i < B * B; i += blockDim.x) Rs[i] = Rinv_prev[i];
+    __syncthreads();
   }
-  else {
-    for (int i = 0; i < b; ++i)
-      for (int j = 0; j < b; ++j) Rout[i * b + j] = R[i][j];
+  double g[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) g[i] = 0.0;
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t row = size_t(blockIdx.x) * blockDim.x + threadIdx.x; row < n; row += stride) {
+    S w[B];
+    RowIO<S, B>::load(W + row * B, w);
+    double wd[B];
+    if constexpr (SCALE) {
+      double acc[B];
+#pragma unroll
+      for (int c = 0; c < B; ++c) acc[c] = 0.0;
+#pragma unroll
+      for (int cp = 0; cp < B; ++cp) {
+        const double x = static_cast<double>(w[cp]);
+#pragma unroll
+        for (int c = cp; c < B; ++c) acc[c] = fma(x, Rs[cp * B + c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        w[c] = static_cast<S>(acc[c]);
+        wd[c] = static_cast<double>(w[c]);
+      }
+      RowIO<S, B>::store(W + row * B, w);
+    }
+    else {
+#pragma unroll
+      for (int c = 0; c < B; ++c) wd[c] = static_cast<double>(w[c]);
+    }
+    const double m = md ? static_cast<double>(md[row]) : 1.0;
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const double wi = wd[i] * m;
+#pragma unroll
+      for (int j = i; j < B; ++j) {
+        g[idx] = fma(wi, wd[j], g[idx]);
+        ++idx;
+      }
+    }
+  }
+  // workgroup reduction: butterfly inside each wave, then waves in order
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    double v = g[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
+    double t = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q][i];
+    partials[size_t(blockIdx.x) * NP + i] = t;
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
+  __syncthreads();
+  if (ticket != gridDim.x - 1) return;
+  __threadfence();
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
+    double t = 0.0;
+    for (unsigned w = 0; w < gridDim.x; ++w) t += partials[size_t(w) * NP + i];
+    // unpack the upper-triangle index i into (r, c) and mirror
+    int r = 0, rem = i;
+    while (rem >= B - r) {
+      rem -= B - r;
+      ++r;
+    }
+    const int c = r + rem;
+    Gs[r * B + c] = t;
+    Gs[c * B + r] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    chol_bxb(Gs, B, Rprev, Rout, Rinv_out, flags, Rl, Xl);
+    *counter = 0u;
   }
 }
 
@@ -367,10 +450,7 @@ __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t 
 
 inline int pick_tpw(int per_wave)
 {
-  if (per_wave <= 1) return 1;
-  if (per_wave <= 2) return 2;
-  if (per_wave <= 4) return 4;
-  if (per_wave <= 6) return 6;
+  if (per_wave <= 6) return per_wave < 1 ? 1 : per_wave;
   if (per_wave <= 8) return 8;
   if (per_wave <= 12) return 12;
   return 16;
@@ -387,9 +467,9 @@ int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* 
   const int RS = 4 / TG;
   const int tpw = pick_tpw((ntile + TG - 1) / TG);
   if ((ntile + TG - 1) / TG > 16) return TRL_HIP_EINVAL;
-  const size_t nquads = (n + 3) / 4;
-  size_t g = (nquads + RS - 1) / RS;
-  const size_t gmax = size_t(2) * ctx->num_cus;
+  const size_t ngroups = ((n + 3) / 4 + kU - 1) / kU;
+  size_t g = (ngroups + RS - 1) / RS;
+  const size_t gmax = size_t(kGramWgPerCu) * ctx->num_cus;
   if (g > gmax) g = gmax;
   if (g < 1) g = 1;
   const size_t P = size_t(k) * B * B;
@@ -404,8 +484,14 @@ int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* 
   case 2:
     hipLaunchKernelGGL((k_gram<S, B, 2>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
     break;
+  case 3:
+    hipLaunchKernelGGL((k_gram<S, B, 3>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    break;
   case 4:
     hipLaunchKernelGGL((k_gram<S, B, 4>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    break;
+  case 5:
+    hipLaunchKernelGGL((k_gram<S, B, 5>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
     break;
   case 6:
     hipLaunchKernelGGL((k_gram<S, B, 6>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
@@ -447,7 +533,7 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
       const S* w = static_cast<const S*>(W);
       const S* m = static_cast<const S*>(md);
       if (n == 0) {
-        hipLaunchKernelGGL(k_reduce, dim3(unsigned((size_t(k) * B * B + 63) / 64)), dim3(kThreads), 0, ctx->stream, ctx->partials, 0, k * B * B, C);
+        hipLaunchKernelGGL(k_reduce, dim3(unsigned((size_t(k) * B * B + 15) / 16)), dim3(kThreads), 0, ctx->stream, ctx->partials, 0, k * B * B, C);
         TRL_HIP_LAUNCH_CHECK();
         return 0;
       }
@@ -458,7 +544,7 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
         int rc = queue_gram_partials<S, B>(ctx, n, kc, v + size_t(j0) * n * B, w, m, &nw);
         if (rc) return rc;
         const int P = kc * B * B;
-        hipLaunchKernelGGL(k_reduce, dim3(unsigned((P + 63) / 64)), dim3(kThreads), 0, ctx->stream, ctx->partials, nw, P, C + size_t(j0) * B * B);
+        hipLaunchKernelGGL(k_reduce, dim3(unsigned((P + 15) / 16)), dim3(kThreads), 0, ctx->stream, ctx->partials, nw, P, C + size_t(j0) * B * B);
         TRL_HIP_LAUNCH_CHECK();
       }
     });
@@ -488,30 +574,38 @@ int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, con
 
 int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes)
 {
-  double* Rinv = ctx->small;
-  double* R1 = ctx->small + 256;
+  // scratch: [0,256) R1, [256,512) Rinv of pass 1, [512,768) Rinv of pass 2
+  double* R1 = ctx->small;
+  double* Rinv1 = ctx->small + 256;
+  double* Rinv2 = ctx->small + 512;
+  unsigned* counter = reinterpret_cast<unsigned*>(ctx->flags + 8);
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
       S* w = static_cast<S*>(W);
       const S* m = static_cast<const S*>(md);
-      for (int pass = 0; pass < passes; ++pass) {
-        int nw = 0;
-        if (n > 0) {
-          int rc = queue_gram_partials<S, B>(ctx, n, 1, w, w, m, &nw);
-          if (rc) return rc;
-        }
-        const bool last = (pass == passes - 1);
-        double* rout = last ? R : R1;
-        const double* rprev = (pass > 0) ? R1 : nullptr;
-        hipLaunchKernelGGL(k_chol, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->partials, nw, B, rprev, rout, Rinv, ctx->flags);
+      constexpr int NP = B * (B + 1) / 2;
+      size_t g = (n + kThreads - 1) / kThreads;
+      const size_t gmax = size_t(kGramWgPerCu) * ctx->num_cus;
+      if (g > gmax) g = gmax;
+      if (g < 1) g = 1;
+      int rc = ensure_partials(ctx, g * NP);
+      if (rc) return rc;
+      const dim3 grid{unsigned(g)}, blk{unsigned(kThreads)};
+      hipLaunchKernelGGL((k_cholqr_pass<S, B, false>), grid, blk, 0, ctx->stream, w, m, n, nullptr, ctx->partials, counter, nullptr, passes == 1 ? R : R1,
+                         Rinv1, ctx->flags);
+      TRL_HIP_LAUNCH_CHECK();
+      const double* last_inv = Rinv1;
+      if (passes == 2) {
+        hipLaunchKernelGGL((k_cholqr_pass<S, B, true>), grid, blk, 0, ctx->stream, w, m, n, Rinv1, ctx->partials, counter, R1, R, Rinv2, ctx->flags);
         TRL_HIP_LAUNCH_CHECK();
-        if (n > 0) {
-          size_t g = (n + kThreads - 1) / kThreads;
-          const size_t gmax = size_t(8) * ctx->num_cus;
-          if (g > gmax) g = gmax;
-          hipLaunchKernelGGL((k_scale<S, B>), dim3(unsigned(g)), dim3(kThreads), 0, ctx->stream, w, Rinv, n);
-          TRL_HIP_LAUNCH_CHECK();
-        }
+        last_inv = Rinv2;
+      }
+      if (n > 0) {
+        size_t gs = (n + kThreads - 1) / kThreads;
+        const size_t gsmax = size_t(8) * ctx->num_cus;
+        if (gs > gsmax) gs = gsmax;
+        hipLaunchKernelGGL((k_scale<S, B>), dim3(unsigned(gs)), dim3(kThreads), 0, ctx->stream, w, last_inv, n);
+        TRL_HIP_LAUNCH_CHECK();
       }
     });
   });
