@@ -134,6 +134,14 @@ int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, co
 int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
                      const void* dscale, int lanes_per_row);
 
+/* Fused three-term recurrence step on the operator:
+ * Y = alpha diag(dscale) A X + beta X + gamma Z (dscale may be NULL; Z may be
+ * NULL, or equal to Y for an in-place update of the older iterate; X must not
+ * overlap Y).  One step of a Chebyshev polynomial filter; no reference
+ * counterpart. */
+int trl_hip_spmm_csr_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
+                           const void* X, void* Y, const void* Z, double alpha, double beta, double gamma);
+
 /* The same product for a matrix in sliced-ELL form: slot s of row r at
  * index s*n + r of ell_col / ell_val (padding slots: col = r, val = 0). */
 int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* ell_col, const void* ell_val, const void* X, void* Y,

@@ -56,18 +56,35 @@ int trl_hip_solve_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t
                       trl_hip_solver_result* res);
 
 typedef struct {
-  double tol_fp32;           /* stopping tolerance of the FP32 phase          */
+  double tol_fp32;           /* stopping tolerance of the FP32 phase                  */
   unsigned max_restarts_fp32;
-  unsigned long long seed;   /* mixing of the FP32 Ritz vectors (see below)   */
+  unsigned long long seed;   /* refine = 0: mixing of the FP32 Ritz vectors           */
+  int refine;                /* 0: FP64 Lanczos warm start, 1: Chebyshev + Rayleigh-Ritz */
+  unsigned cheb_degree;      /* refine = 1: filter degree per refinement sweep        */
+  unsigned max_refine;       /* refine = 1: sweeps before falling back to refine = 0  */
+  int phase1_fp64;           /* 1: run phase 1 in FP64 too (the same schedule without
+                                the precision change, for comparisons)                */
+  unsigned refine_blocks;    /* refine = 1: Ritz blocks refined together (0: nev/b + 7);
+                                the blocks beyond nev/b + 1 are a buffer that moves
+                                the filter cutoff away from the wanted values         */
 } trl_hip_mixed_params;
 
-/* Mixed precision: an FP32 solve (FP32 basis, SpMM and Gram products, FP64
- * coefficients, Cholesky and Rayleigh-Ritz) to tol_fp32, then an FP64 solve
- * to p->tol started from V0 = X32 G, where X32 are the nev/b + 1 FP32 Ritz
- * blocks converted to FP64 and G is a seeded Gaussian (nev/b+1)*b x b
- * mixing matrix, so the FP64 Krylov space opens on the FP32 Ritz subspace.
+/* Mixed precision.  Phase 1: an FP32 thick-restart solve (FP32 basis, SpMM
+ * and Gram products with compensated sums, FP64 coefficients, Cholesky and
+ * Rayleigh-Ritz) to tol_fp32, keeping its leading Ritz blocks X.
+ * Phase 2 (FP64), by mp->refine:
+ *   1: block refinement of all of X together: FP64 block CGS2 + CholQR2,
+ *      Rayleigh-Ritz, and while a true residual exceeds p->tol, a scaled
+ *      Chebyshev filter of degree cheb_degree (Zhou & Saad, J. Comput. Phys.
+ *      2007) that damps the spectrum beyond the kept Ritz values, then
+ *      Rayleigh-Ritz again; after max_refine sweeps it falls back to 0;
+ *   0: an FP64 thick-restart solve started from V0 = X G, G a seeded
+ *      Gaussian (nev/b+1)*b x b mixing matrix.
+ * Order must be ascending or descending for refine = 1.
  * val32/mass32/minv32 are FP32 copies of the FP64 arrays.  evals, X, resid
- * and res refer to the final FP64 phase; res32 (optional) to the FP32 one. */
+ * refer to the final FP64 result; res->seconds is the wall time of both
+ * phases, res->iterations the FP64 sweeps or cycles; res32 (optional)
+ * describes the FP32 phase. */
 int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, const double* val64, const float* val32,
                             const double* mass64, const double* minv64, const float* mass32, const float* minv32, const trl_hip_solver_params* p,
                             const trl_hip_mixed_params* mp, const double* V0, double* evals, double* X, double* resid, trl_hip_solver_result* res,

@@ -341,3 +341,26 @@ def test_invalid_arguments(ctx):
     assert H().trl_hip_gram(ctx.h, L.F64, 10, 3, 1, None, None, None, None) == -1  # b = 3 unsupported
     assert H().trl_hip_spmm_csr(ctx.h, L.F64, 10, 4, None, None, None, None, None, None, 3) == -1
     assert H().trl_hip_cholqr(ctx.h, L.F64, 10, 4, None, None, None, 3) == -1
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", [1, 4, 6, 8, 16])
+@pytest.mark.parametrize("variant", ["z", "noz", "inplace", "scaled"])
+def test_spmm_axpby(ctx, rng, p, b, variant):
+    # Y = alpha diag(d) A X + beta X + gamma Z
+    A = P.laplacian_3d(6, 5, 7, rng.uniform(0, 1, 210))
+    dt = DT[p]
+    X = rng.standard_normal((A.n, b)).astype(dt).astype(np.float64)
+    Z = rng.standard_normal((A.n, b)).astype(dt).astype(np.float64)
+    d = rng.uniform(0.5, 2, A.n).astype(dt).astype(np.float64)
+    al, be, ga = 0.7, -1.3, 0.4
+    rp, ci, va = dev(ctx, A.rowptr, np.int32), dev(ctx, A.col, np.int32), dev(ctx, A.val, dt)
+    Xd, Zd = dev(ctx, X, dt), dev(ctx, Z, dt)
+    Yd = Zd if variant == "inplace" else dev(ctx, np.zeros_like(X), dt)
+    zp = None if variant == "noz" else Zd.ptr
+    dp = dev(ctx, d, dt).ptr if variant == "scaled" else None
+    L.check(H().trl_hip_spmm_csr_axpby(ctx.h, L.dtype_code(dt), A.n, b, rp.ptr, ci.ptr, va.ptr, dp, Xd.ptr, Yd.ptr, zp, al, be, ga), "axpby")
+    AX = oracle.spmm(A, X, d if variant == "scaled" else None)
+    ref = al * AX + be * X + (0.0 if variant == "noz" else ga * Z)
+    assert rel(Yd.download(), ref) < TOL[p]
+    assert H().trl_hip_spmm_csr_axpby(ctx.h, L.dtype_code(dt), A.n, b, rp.ptr, ci.ptr, va.ptr, None, Xd.ptr, Xd.ptr, None, 1.0, 0.0, 0.0) == -1

@@ -121,3 +121,40 @@ def test_max_restarts_not_converged(ctx):
     dA = DeviceCSR(ctx, c.csr, precisions=("f64",))
     out = solve(dA, c.b, EigensolverParams(c.nev, 32, max_restarts=2), v0(ctx, c.csr.n, c.b), order="ascending")
     assert out.result["converged"] == 0 and out.result["iterations"] == 2
+
+
+@pytest.mark.parametrize("phase1_fp64", [False, True])
+def test_mixed_refine_laplace3d(ctx, phase1_fp64):
+    # 3D Laplacian 24^3 + diag U[0,1] (a small stand-in for 160^3), b = 8, 32 smallest:
+    # FP32 block Lanczos to 1e-3, then FP64 Chebyshev/Rayleigh-Ritz sweeps; the
+    # result must match the plain FP64 solve and the CPU oracle
+    c = P.config_c3(24)
+    dA = DeviceCSR(ctx, c.csr)
+    V0 = v0(ctx, c.csr.n, c.b)
+    prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
+    o64 = solve(dA, c.b, prm, V0)
+    om = solve_mixed(dA, c.b, prm, V0, tol_fp32=1e-3, cheb_degree=60, phase1_fp64=phase1_fp64)
+    print("mixed", om.result, om.result32)
+    assert om.result["converged"] == 1
+    assert om.resid.max() <= 1e-8
+    assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))
+    ref = oracle.lanczos_solve(c.csr, c.b, c.nev, c.ncv, P.initial_block(c.csr.n, c.b), order=1, ritz_vectors=True)
+    assert np.all(np.abs(om.evals - ref["evals"]) <= 1e-10 * np.abs(ref["evals"]))
+    Xh = np.concatenate(list(om.X.download()), axis=1)[:, : c.nev]
+    print("max principal angle vs oracle", max_angle(Xh, ref["X"]))
+    assert max_angle(Xh, ref["X"]) < 1e-6
+
+
+def test_mixed_generalized(ctx):
+    # the structural pair K x = lambda M x in mixed precision, against scipy
+    import json
+    from pathlib import Path
+
+    g = json.loads((Path(__file__).parent / "golden" / "stiffness_mass_eigsh.json").read_text())
+    c = P.config_c2()
+    dA = DeviceCSR(ctx, c.csr, mass=c.mass)
+    om = solve_mixed(dA, c.b, EigensolverParams(c.nev, c.ncv), v0(ctx, c.csr.n, c.b), tol_fp32=1e-4, cheb_degree=40)
+    ref = np.array(g["eigenvalues"])
+    print("mixed generalized", om.result, om.result32, (np.abs(om.evals - ref) / ref).max())
+    assert np.all(np.abs(om.evals - ref) <= 1e-10 * ref)
+    assert om.resid.max() <= 1e-8

@@ -46,6 +46,8 @@ HIP_API = [
     ("trl_hip_cholqr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
     ("trl_hip_rotate", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("trl_hip_spmm_csr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    ("trl_hip_spmm_csr_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.c_double,
+                                       C.c_double, C.c_double]),
     ("trl_hip_spmm_ell", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_small_copy", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     ("trl_hip_small_zero", c_int, [c_void_p, c_int, c_void_p]),
@@ -217,7 +219,8 @@ class SolverResult(C.Structure):
 
 
 class MixedParams(C.Structure):
-    _fields_ = [("tol_fp32", C.c_double), ("max_restarts_fp32", C.c_uint), ("seed", C.c_ulonglong)]
+    _fields_ = [("tol_fp32", C.c_double), ("max_restarts_fp32", C.c_uint), ("seed", C.c_ulonglong), ("refine", c_int), ("cheb_degree", C.c_uint),
+                ("max_refine", C.c_uint), ("phase1_fp64", c_int), ("refine_blocks", C.c_uint)]
 
 
 SOLVER_API = [

@@ -323,6 +323,21 @@ int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const i
   return launch_spmm_csr(ctx, t, n, b, rowptr, col, val, X, Y, dscale, lanes_per_row);
 }
 
+int trl_hip_spmm_csr_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
+                           const void* X, void* Y, const void* Z, double alpha, double beta, double gamma)
+{
+  if (!ctx || !valid_b(b) || (n && (!rowptr || !X || !Y)) || X == Y) return TRL_HIP_EINVAL;
+  if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  double bytes = 0.0;
+  if (ctx->prof_on) {
+    const double s = (t == TRL_HIP_F32) ? 4.0 : 8.0;
+    bytes = double(csr_nnz(ctx, rowptr, n)) * (s + 4.0) + double(n + 1) * 4.0 + (Z ? 4 : 3) * panel_bytes(t, n, b);
+  }
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, bytes);
+  return launch_spmm_axpby(ctx, t, n, b, rowptr, col, val, dscale, X, Y, Z, alpha, beta, gamma);
+}
+
 int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* ell_col, const void* ell_val, const void* X, void* Y,
                      const void* dscale)
 {

@@ -62,6 +62,8 @@ int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, c
 int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int ybc, int kc, void* Out);
 int launch_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
                     const void* dscale, int lanes);
+int launch_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
+                      const void* X, void* Y, const void* Z, double alpha, double beta, double gamma);
 int launch_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* col, const void* val, const void* X, void* Y, const void* dscale);
 int launch_small(trl_hip_ctx* ctx, int op, int b, const double* A, const double* B, double* C, const double* hostvals);
 int launch_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* md, const double* lambda, double* out);

@@ -59,6 +59,50 @@ __global__ __launch_bounds__(kThreads) void k_spmm_csr(const int32_t* __restrict
   RowIO<S, B>::store(Y + size_t(row) * B, acc);
 }
 
+// Fused three-term form: Y = alpha D A X + beta X + gamma Z (D = diag(dscale)
+// or I; Z may be Y, or NULL), one Chebyshev-recurrence step per launch.
+template <typename S, int B, int L>
+__global__ __launch_bounds__(kThreads) void k_spmm_csr_axpby(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
+                                                             const S* __restrict__ X, S* Y, const S* Z, const S* __restrict__ dscale, double alpha,
+                                                             double beta, double gamma, int n)
+{
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int gid = int(blk * blockDim.x + threadIdx.x);
+  const int row = gid / L;
+  const int sub = int(threadIdx.x) % L;
+  if (row >= n) return;
+  const int beg = rowptr[row], end = rowptr[row + 1];
+  S acc[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) acc[c] = S(0);
+  for (int p = beg + sub; p < end; p += L) {
+    const int c = col[p];
+    const S v = val[p];
+    S x[B];
+    RowIO<S, B>::load(X + size_t(c) * B, x);
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
+  }
+  if constexpr (L > 1) {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
+  }
+  if (sub != 0) return;
+  S xr[B], zr[B], out[B];
+  RowIO<S, B>::load(X + size_t(row) * B, xr);
+  if (Z) RowIO<S, B>::load(Z + size_t(row) * B, zr);
+  const double a = dscale ? alpha * double(dscale[row]) : alpha;
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    double v = a * double(acc[q]) + beta * double(xr[q]);
+    if (Z) v += gamma * double(zr[q]);
+    out[q] = S(v);
+  }
+  RowIO<S, B>::store(Y + size_t(row) * B, out);
+}
+
 template <typename S, int B>
 __global__ __launch_bounds__(kThreads) void k_spmm_ell(int width, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X,
                                                        S* __restrict__ Y, const S* __restrict__ dscale, int n)
@@ -98,9 +142,37 @@ int launch_csr_l(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_
   return 0;
 }
 
+template <typename S, int B, int L>
+int launch_axpby_l(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* X, S* Y, const S* Z, const S* ds, double a,
+                   double b, double g)
+{
+  size_t grid = (n * L + kThreads - 1) / kThreads;
+  grid = (grid + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL((k_spmm_csr_axpby<S, B, L>), dim3(unsigned(grid)), dim3(kThreads), 0, ctx->stream, rowptr, col, val, X, Y, Z, ds, a, b, g, int(n));
+  TRL_HIP_LAUNCH_CHECK();
+  return 0;
+}
+
 } // namespace
 
 namespace trl_hip_impl {
+
+int launch_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
+                      const void* X, void* Y, const void* Z, double alpha, double beta, double gamma)
+{
+  TRL_DISPATCH_T(t, {
+    TRL_DISPATCH_B(b, {
+      const S* v = static_cast<const S*>(val);
+      const S* x = static_cast<const S*>(X);
+      S* y = static_cast<S*>(Y);
+      const S* z = static_cast<const S*>(Z);
+      const S* ds = static_cast<const S*>(dscale);
+      if constexpr (B >= 8) return launch_axpby_l<S, B, 1>(ctx, n, rowptr, col, v, x, y, z, ds, alpha, beta, gamma);
+      else return launch_axpby_l<S, B, 4>(ctx, n, rowptr, col, v, x, y, z, ds, alpha, beta, gamma);
+    });
+  });
+  return 0;
+}
 
 int launch_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
                     const void* dscale, int lanes)

@@ -1,7 +1,10 @@
 // libtrl_hip_solver.so: whole-solve C-ABI (include/trl_hip_solver.h) over the
 // trl::hip backend types and the trl_amd::BlockLanczos driver.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -64,7 +67,8 @@ struct SolveOut {
 // receives the first nblk_out Ritz blocks.
 template <class S, unsigned B>
 void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* mass, const S* minv,
-                 const trl_hip_solver_params& p, const S* V0, double* evals, S* X_out, unsigned nblk_out, double* resid, trl_hip_solver_result& res)
+                 const trl_hip_solver_params& p, const S* V0, double* evals, S* X_out, unsigned nblk_out, double* resid, trl_hip_solver_result& res,
+                 std::vector<double>* all_evals = nullptr, unsigned* nblk_eff = nullptr)
 {
   using EVP = trl::hip::CSREVP<S, B>;
   auto evp = std::make_shared<EVP>(c, n, rowptr, col, val, mass, minv);
@@ -87,10 +91,15 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
     res.host_seconds = evp->host_seconds();
     const auto& ev = evp->get_current_eigenvalues();
     for (unsigned i = 0; i < p.nev && i < ev.size(); ++i) evals[i] = ev[i];
+    if (all_evals) *all_evals = ev;
     if (X_out) {
       const unsigned kr = p.nev / B + 1;
-      if (lz.basis_holds_ritz_blocks() && nblk_out <= kr) {
-        throw_on(trl_hip_memcpy_d2d(c.get(), X_out, lz.get_basis().data(), std::size_t(nblk_out) * n * B * sizeof(S)), "ritz vectors");
+      if (nblk_eff) *nblk_eff = nblk_out;
+      if (lz.basis_holds_ritz_blocks()) {
+        // after a final restart only the leading kr blocks are Ritz vectors
+        const unsigned cnt = std::min(nblk_out, kr);
+        if (nblk_eff) *nblk_eff = cnt;
+        throw_on(trl_hip_memcpy_d2d(c.get(), X_out, lz.get_basis().data(), std::size_t(cnt) * n * B * sizeof(S)), "ritz vectors");
       }
       else {
         const auto& Y = evp->get_current_eigenvectors();
@@ -110,6 +119,111 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
     run(lz);
   }
 }
+
+// FP64 block refinement of nb Ritz panels toward one end of the spectrum:
+// block CGS2 + CholQR2 orthonormalisation, Rayleigh-Ritz over the nb*B
+// vectors, and a scaled Chebyshev filter (Zhou & Saad, "Self-consistent-field
+// calculations using Chebyshev-filtered subspace iteration", J. Comput. Phys.
+// 226 (2007), Algorithm 3.2) that keeps the value 1 at the far wanted end a0
+// and damps [lo, hi].
+template <unsigned B>
+struct BlockRefiner {
+  const Context& c;
+  std::size_t n;
+  const int32_t *rowptr, *col;
+  const double *val, *mass, *minv;
+  int lanes;
+  unsigned nb;
+  bool ascending;
+  trl::hip::DeviceBuffer<double> Xa, Xb, AX, coef;
+  double* X; // current Ritz panels (Xa or Xb)
+  double* spare;
+
+  BlockRefiner(const Context& ctx, std::size_t n_, const int32_t* rp, const int32_t* ci, const double* v, const double* m, const double* mi, int l, unsigned nb_,
+               bool asc)
+      : c(ctx), n(n_), rowptr(rp), col(ci), val(v), mass(m), minv(mi), lanes(l), nb(nb_), ascending(asc), Xa(ctx, std::size_t(nb_) * n_ * B),
+        Xb(ctx, std::size_t(nb_) * n_ * B), AX(ctx, std::size_t(nb_) * n_ * B), coef(ctx, std::size_t(nb_) * nb_ * B * B + 2 * B * B)
+  {
+    X = Xa.data();
+    spare = Xb.data();
+  }
+
+  double* panel(double* base, unsigned j) const { return base + std::size_t(j) * n * B; }
+
+  void orthonormalize()
+  {
+    double* C = coef.data();
+    double* R = coef.data() + std::size_t(nb) * nb * B * B;
+    for (unsigned j = 0; j < nb; ++j) {
+      if (j > 0)
+        for (int pass = 0; pass < 2; ++pass) {
+          throw_on(trl_hip_gram(c.get(), TRL_HIP_F64, n, B, int(j), X, panel(X, j), mass, C), "refine gram");
+          throw_on(trl_hip_update(c.get(), TRL_HIP_F64, n, B, int(j), X, C, panel(X, j)), "refine update");
+        }
+      throw_on(trl_hip_cholqr(c.get(), TRL_HIP_F64, n, B, panel(X, j), mass, R, 2), "refine cholqr");
+    }
+    c.sync(); // raises a breakdown
+  }
+
+  // Rayleigh-Ritz; theta receives the nb*B Ritz values in the wanted order.
+  void rayleigh_ritz(std::vector<double>& theta)
+  {
+    const unsigned nt = nb * B;
+    for (unsigned j = 0; j < nb; ++j)
+      throw_on(trl_hip_spmm_csr(c.get(), TRL_HIP_F64, n, B, rowptr, col, val, panel(X, j), panel(AX.data(), j), minv, lanes), "refine apply");
+    double* C = coef.data();
+    for (unsigned j = 0; j < nb; ++j)
+      throw_on(trl_hip_gram(c.get(), TRL_HIP_F64, n, B, int(nb), X, panel(AX.data(), j), mass, C + std::size_t(j) * nt * B), "refine projection");
+    std::vector<double> Cj(std::size_t(nt) * nt);
+    throw_on(trl_hip_memcpy_d2h(c.get(), Cj.data(), C, Cj.size() * sizeof(double)), "refine download");
+    // C + j*nt*B holds rows r = 0..nt-1 (panel i = r/B, column r%B) of H[:, j*B:(j+1)*B]
+    std::vector<double> H(std::size_t(nt) * nt), w(nt), Z(std::size_t(nt) * nt);
+    for (unsigned j = 0; j < nb; ++j)
+      for (unsigned r = 0; r < nt; ++r)
+        for (unsigned q = 0; q < B; ++q) H[std::size_t(r) * nt + j * B + q] = Cj[std::size_t(j) * nt * B + std::size_t(r) * B + q];
+    for (unsigned r = 0; r < nt; ++r)
+      for (unsigned q = r + 1; q < nt; ++q) {
+        const double a = 0.5 * (H[std::size_t(r) * nt + q] + H[std::size_t(q) * nt + r]);
+        H[std::size_t(r) * nt + q] = H[std::size_t(q) * nt + r] = a;
+      }
+    if (trl_amd::dense::syev(int(nt), H.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
+    std::vector<unsigned> idx(nt);
+    for (unsigned i = 0; i < nt; ++i) idx[i] = ascending ? i : nt - 1 - i;
+    theta.resize(nt);
+    std::vector<double> Y(std::size_t(nt) * nt);
+    for (unsigned i = 0; i < nt; ++i) theta[i] = w[idx[i]];
+    for (unsigned bi = 0; bi < nb; ++bi)
+      for (unsigned bj = 0; bj < nb; ++bj)
+        for (unsigned ci = 0; ci < B; ++ci)
+          for (unsigned cj = 0; cj < B; ++cj) Y[((std::size_t(bi) * nb + bj) * B + ci) * B + cj] = Z[std::size_t(bi * B + ci) * nt + idx[bj * B + cj]];
+    throw_on(trl_hip_memcpy_h2d(c.get(), C, Y.data(), Y.size() * sizeof(double)), "refine upload");
+    throw_on(trl_hip_rotate(c.get(), TRL_HIP_F64, n, B, int(nb), X, C, int(nb), int(nb), spare), "refine rotate");
+    std::swap(X, spare);
+  }
+
+  // Chebyshev filter of the given degree on every panel; damps [lo, hi],
+  // value 1 at a0.
+  void filter(unsigned degree, double lo, double hi, double a0)
+  {
+    if (degree == 0) return;
+    const double e = 0.5 * (hi - lo), cc = 0.5 * (hi + lo);
+    const double s1 = e / (a0 - cc);
+    for (unsigned j = 0; j < nb; ++j) {
+      double* prev = panel(X, j);
+      double* cur = panel(spare, j);
+      throw_on(trl_hip_spmm_csr_axpby(c.get(), TRL_HIP_F64, n, B, rowptr, col, val, minv, prev, cur, nullptr, s1 / e, -s1 * cc / e, 0.0), "filter");
+      double sg = s1;
+      for (unsigned i = 2; i <= degree; ++i) {
+        const double sn = 1.0 / (2.0 / s1 - sg);
+        throw_on(trl_hip_spmm_csr_axpby(c.get(), TRL_HIP_F64, n, B, rowptr, col, val, minv, cur, prev, prev, 2.0 * sn / e, -2.0 * sn * cc / e, -sg * sn),
+                 "filter");
+        std::swap(prev, cur);
+        sg = sn;
+      }
+    }
+    if (degree % 2 == 1) std::swap(X, spare); // odd degree leaves the result in the spare buffer
+  }
+};
 
 template <class F>
 int dispatch_b(int b, F&& f)
@@ -193,37 +307,115 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
     return TRL_HIP_EINVAL;
   *res = trl_hip_solver_result{};
   trl_hip_solver_result r32{};
+  if (mp->refine == 1 && p->order == 0) return TRL_HIP_EINVAL;
   try {
     const Context c = Context::borrow(ctx);
     const int rc = dispatch_b(p->b, [&](auto Bc) {
       constexpr unsigned B = decltype(Bc)::value;
       const unsigned kr = p->nev / B + 1;
+      const unsigned nblk = (p->nev + B - 1) / B;
       c.sync();
       const double t0 = now();
       // ---- FP32 phase -------------------------------------------------------
       trl::hip::DeviceBuffer<float> V0f(c, n * B);
       throw_on(trl_hip_convert(c.get(), TRL_HIP_F64, V0, TRL_HIP_F32, V0f.data(), n * B), "convert V0");
-      trl::hip::DeviceBuffer<float> X32(c, std::size_t(kr) * n * B);
-      std::vector<double> ev32(std::max(p->nev, kr * B), 0.0);
+      const unsigned m = p->ncv / B;
+      unsigned nref = mp->refine == 1 ? (mp->refine_blocks ? mp->refine_blocks : kr + 6) : kr;
+      if (nref < kr) nref = kr;
+      if (nref > m) nref = m;
+      trl::hip::DeviceBuffer<float> X32(c, std::size_t(nref) * n * B);
+      std::vector<double> ev32(std::max(p->nev, nref * B), 0.0), all32;
       trl_hip_solver_params p32 = *p;
       p32.tol = mp->tol_fp32;
       p32.max_restarts = mp->max_restarts_fp32;
       if (p32.cholqr_passes == 0) p32.cholqr_passes = 2;
-      solve_typed<float, B>(c, n, rowptr, col, val32, mass32, minv32, p32, V0f.data(), ev32.data(), X32.data(), kr, nullptr, r32);
-      // ---- warm start: V0_64 = X32 G ---------------------------------------
-      trl::hip::DeviceBuffer<double> X64(c, std::size_t(kr) * n * B);
-      throw_on(trl_hip_convert(c.get(), TRL_HIP_F32, X32.data(), TRL_HIP_F64, X64.data(), std::size_t(kr) * n * B), "convert X32");
-      std::mt19937_64 g(mp->seed);
-      std::normal_distribution<double> nd;
-      std::vector<double> G(std::size_t(kr) * B * B);
-      for (auto& x : G) x = nd(g);
-      trl::hip::DeviceBuffer<double> Gd(c, G.size());
-      throw_on(trl_hip_memcpy_h2d(c.get(), Gd.data(), G.data(), G.size() * sizeof(double)), "upload G");
-      trl::hip::DeviceBuffer<double> V064(c, n * B);
-      throw_on(trl_hip_rotate(c.get(), TRL_HIP_F64, n, B, int(kr), X64.data(), Gd.data(), 1, 1, V064.data()), "mix Ritz vectors");
-      // ---- FP64 phase --------------------------------------------------------
-      const unsigned nblk = (p->nev + B - 1) / B;
-      solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, *p, V064.data(), evals, X, nblk, resid, *res);
+      unsigned got = nref;
+      trl::hip::DeviceBuffer<double> X1(c, mp->phase1_fp64 ? std::size_t(nref) * n * B : 1);
+      if (mp->phase1_fp64) {
+        p32.cholqr_passes = p->cholqr_passes;
+        solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, p32, V0, ev32.data(), X1.data(), nref, nullptr, r32, &all32, &got);
+      }
+      else {
+        solve_typed<float, B>(c, n, rowptr, col, val32, mass32, minv32, p32, V0f.data(), ev32.data(), X32.data(), nref, nullptr, r32, &all32, &got);
+      }
+      nref = got;
+
+      bool done = false;
+      if (mp->refine == 1) {
+        // ---- FP64 phase: block refinement of the FP32 Ritz blocks -----------
+        const bool asc = (p->order == 1);
+        BlockRefiner<B> rf(c, n, rowptr, col, val64, mass64, minv64, p->lanes_per_row, nref, asc);
+        if (mp->phase1_fp64)
+          throw_on(trl_hip_memcpy_d2d(c.get(), rf.X, X1.data(), std::size_t(nref) * n * B * sizeof(double)), "copy X1");
+        else
+          throw_on(trl_hip_convert(c.get(), TRL_HIP_F32, X32.data(), TRL_HIP_F64, rf.X, std::size_t(nref) * n * B), "convert X32");
+        // spectrum ends seen by the FP32 Krylov space
+        double lo_all = all32.front(), hi_all = all32.front();
+        for (double v : all32) {
+          lo_all = std::min(lo_all, v);
+          hi_all = std::max(hi_all, v);
+        }
+        const double margin = 0.01 * (hi_all - lo_all);
+        std::vector<double> theta, rs(p->nev);
+        unsigned sweep = 0;
+        for (;;) {
+          rf.orthonormalize();
+          rf.rayleigh_ritz(theta);
+          true_residuals<double, B>(c, n, rowptr, col, val64, mass64, rf.X, p->nev, theta.data(), rs.data(), p->lanes_per_row);
+          double worst = 0.0;
+          for (double r : rs) worst = std::max(worst, r);
+          if (std::getenv("TRL_HIP_VERBOSE")) {
+            std::vector<double> srt(rs);
+            std::sort(srt.begin(), srt.end());
+            std::fprintf(stderr, "[refine] sweep %u: resid max %.3e median %.3e min %.3e theta0 %.9g theta_kept %.9g\n", sweep, worst,
+                         srt[srt.size() / 2], srt.front(), theta.front(), theta.back());
+          }
+          if (worst <= p->tol) {
+            done = true;
+            break;
+          }
+          if (sweep >= mp->max_refine) break;
+          ++sweep;
+          const double far = theta.back(); // last kept Ritz value
+          if (asc)
+            rf.filter(mp->cheb_degree, far, hi_all + margin, theta.front());
+          else
+            rf.filter(mp->cheb_degree, lo_all - margin, far, theta.front());
+        }
+        if (done) {
+          for (unsigned i = 0; i < p->nev; ++i) evals[i] = theta[i];
+          if (X) throw_on(trl_hip_memcpy_d2d(c.get(), X, rf.X, std::size_t(nblk) * n * B * sizeof(double)), "copy X");
+          if (resid)
+            for (unsigned i = 0; i < p->nev; ++i) resid[i] = rs[i];
+          res->converged = 1;
+          res->iterations = sweep;
+          res->n_op_apply = (sweep + 1) * nref * 2 + sweep * nref * mp->cheb_degree;
+        }
+        else {
+          // fall back to an FP64 Lanczos warm start from the refined block
+          trl::hip::DeviceBuffer<double> V064(c, n * B);
+          throw_on(trl_hip_memcpy_d2d(c.get(), V064.data(), rf.X, n * B * sizeof(double)), "seed");
+          solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, *p, V064.data(), evals, X, nblk, resid, *res);
+          done = true;
+        }
+      }
+      if (!done) {
+        // ---- FP64 phase: thick-restart Lanczos from V0 = X32 G -------------
+        trl::hip::DeviceBuffer<double> X64(c, std::size_t(kr) * n * B);
+        if (mp->phase1_fp64)
+          throw_on(trl_hip_memcpy_d2d(c.get(), X64.data(), X1.data(), std::size_t(kr) * n * B * sizeof(double)), "copy X1");
+        else
+          throw_on(trl_hip_convert(c.get(), TRL_HIP_F32, X32.data(), TRL_HIP_F64, X64.data(), std::size_t(kr) * n * B), "convert X32");
+        std::mt19937_64 g(mp->seed);
+        std::normal_distribution<double> nd;
+        std::vector<double> G(std::size_t(kr) * B * B);
+        for (auto& x : G) x = nd(g);
+        trl::hip::DeviceBuffer<double> Gd(c, G.size());
+        throw_on(trl_hip_memcpy_h2d(c.get(), Gd.data(), G.data(), G.size() * sizeof(double)), "upload G");
+        trl::hip::DeviceBuffer<double> V064(c, n * B);
+        throw_on(trl_hip_rotate(c.get(), TRL_HIP_F64, n, B, int(kr), X64.data(), Gd.data(), 1, 1, V064.data()), "mix Ritz vectors");
+        solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, *p, V064.data(), evals, X, nblk, resid, *res);
+      }
       c.sync();
       res->seconds = now() - t0;
     });
