@@ -1,10 +1,11 @@
 // Sparse operator kernels: Y = diag(d) A X with X, Y n x b row-major panels.
 //
-// CSR: L lanes of a wavefront share one row (L in {1,2,4,8,16}); each lane
-// walks nonzeros beg+sub, beg+sub+L, ... with coalesced column-index and value
-// loads, gathers the b-wide row of X for each nonzero into registers, and the
-// L partial rows are combined by a butterfly (shfl_xor) reduction inside the
-// lane group.  ELL: one row per lane, slot-major arrays (fully coalesced).
+// CSR: L lanes of a wavefront share one row (L in {1,2,4,8,16}).  With L = 1
+// a lane owns a row and loads its indices/values in chunks before gathering
+// the b-wide rows of X into registers; with L > 1 the lanes of a group walk
+// the row's nonzeros with coalesced index/value loads and the partial rows are
+// combined by a butterfly (shfl_xor) reduction inside the group.  ELL: one row
+// per lane, slot-major arrays (fully coalesced), chunked the same way.
 //
 // Workgroups are renumbered so that each XCD walks a contiguous range of rows:
 // the stencil neighbours of a row (r +- 1, r +- nx, ...) then share that XCD's L2.
@@ -23,6 +24,56 @@ __device__ inline unsigned xcd_block(unsigned bid, unsigned nb)
   return (bid % kXcds) * (nb / kXcds) + bid / kXcds;
 }
 
+// Nonzeros whose indices and values are loaded before any of their gathers.
+constexpr int kChunk = 8;
+
+// Accumulates row `row` of A X into acc.  L == 1: one lane per row, the
+// row's column indices and values are loaded kChunk at a time before the
+// gathers of X (two memory latencies per chunk instead of two per nonzero);
+// padding slots gather the row's own X row with value 0.  L > 1: lanes
+// sub, sub+L, ... of the row, combined by a butterfly in the lane group.
+template <typename S, int B, int L>
+__device__ inline void csr_row(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X, int row,
+                               int sub, S (&acc)[B])
+{
+#pragma unroll
+  for (int c = 0; c < B; ++c) acc[c] = S(0);
+  const int beg = rowptr[row], end = rowptr[row + 1];
+  if constexpr (L == 1) {
+    for (int p0 = beg; p0 < end; p0 += kChunk) {
+      int ci[kChunk];
+      S vi[kChunk];
+#pragma unroll
+      for (int q = 0; q < kChunk; ++q) {
+        const bool ok = p0 + q < end;
+        ci[q] = ok ? col[p0 + q] : row;
+        vi[q] = ok ? val[p0 + q] : S(0);
+      }
+      S x[kChunk][B];
+#pragma unroll
+      for (int q = 0; q < kChunk; ++q) RowIO<S, B>::load(X + size_t(ci[q]) * B, x[q]);
+#pragma unroll
+      for (int q = 0; q < kChunk; ++q)
+#pragma unroll
+        for (int c = 0; c < B; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
+    }
+  }
+  else {
+    for (int p = beg + sub; p < end; p += L) {
+      const int c = col[p];
+      const S v = val[p];
+      S x[B];
+      RowIO<S, B>::load(X + size_t(c) * B, x);
+#pragma unroll
+      for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
+  }
+}
+
 template <typename S, int B, int L>
 __global__ __launch_bounds__(kThreads) void k_spmm_csr(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
                                                        const S* __restrict__ X, S* __restrict__ Y, const S* __restrict__ dscale, int n)
@@ -32,24 +83,8 @@ __global__ __launch_bounds__(kThreads) void k_spmm_csr(const int32_t* __restrict
   const int row = gid / L;
   const int sub = int(threadIdx.x) % L;
   if (row >= n) return; // whole lane groups leave together
-  const int beg = rowptr[row], end = rowptr[row + 1];
   S acc[B];
-#pragma unroll
-  for (int c = 0; c < B; ++c) acc[c] = S(0);
-  for (int p = beg + sub; p < end; p += L) {
-    const int c = col[p];
-    const S v = val[p];
-    S x[B];
-    RowIO<S, B>::load(X + size_t(c) * B, x);
-#pragma unroll
-    for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
-  }
-  if constexpr (L > 1) {
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1)
-#pragma unroll
-      for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
-  }
+  csr_row<S, B, L>(rowptr, col, val, X, row, sub, acc);
   if (sub != 0) return;
   if (dscale) {
     const S d = dscale[row];
@@ -71,24 +106,8 @@ __global__ __launch_bounds__(kThreads) void k_spmm_csr_axpby(const int32_t* __re
   const int row = gid / L;
   const int sub = int(threadIdx.x) % L;
   if (row >= n) return;
-  const int beg = rowptr[row], end = rowptr[row + 1];
   S acc[B];
-#pragma unroll
-  for (int c = 0; c < B; ++c) acc[c] = S(0);
-  for (int p = beg + sub; p < end; p += L) {
-    const int c = col[p];
-    const S v = val[p];
-    S x[B];
-    RowIO<S, B>::load(X + size_t(c) * B, x);
-#pragma unroll
-    for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
-  }
-  if constexpr (L > 1) {
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1)
-#pragma unroll
-      for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
-  }
+  csr_row<S, B, L>(rowptr, col, val, X, row, sub, acc);
   if (sub != 0) return;
   S xr[B], zr[B], out[B];
   RowIO<S, B>::load(X + size_t(row) * B, xr);
@@ -113,14 +132,23 @@ __global__ __launch_bounds__(kThreads) void k_spmm_ell(int width, const int32_t*
   S acc[B];
 #pragma unroll
   for (int c = 0; c < B; ++c) acc[c] = S(0);
-  for (int s = 0; s < width; ++s) {
-    const size_t p = size_t(s) * n + row;
-    const int c = col[p];
-    const S v = val[p];
-    S x[B];
-    RowIO<S, B>::load(X + size_t(c) * B, x);
+  for (int s0 = 0; s0 < width; s0 += kChunk) {
+    int ci[kChunk];
+    S vi[kChunk];
 #pragma unroll
-    for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
+    for (int q = 0; q < kChunk; ++q) {
+      const bool ok = s0 + q < width;
+      const size_t p = size_t(s0 + q) * n + row;
+      ci[q] = ok ? col[p] : row;
+      vi[q] = ok ? val[p] : S(0);
+    }
+    S x[kChunk][B];
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) RowIO<S, B>::load(X + size_t(ci[q]) * B, x[q]);
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q)
+#pragma unroll
+      for (int c = 0; c < B; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
   }
   if (dscale) {
     const S d = dscale[row];
