@@ -41,23 +41,23 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmm_rate(ctx, dA, b, dtype, reps=50):
-    from trl_amd import _lib as L
-
-    dt = np.float32 if dtype == "f32" else np.float64
+def spmm_rate(ctx, dA, b, dtype, tune, reps=50):
+    """Mean time and algorithmic rate of the operator kernel the solve uses."""
+    op = dA.operator(b, dtype, tune=tune)
+    dt = op.dtype
     X = ctx.to_device(np.random.default_rng(0).standard_normal((dA.n, b)).astype(dt))
     Y = ctx.empty((dA.n, b), dt)
-    val, _, _ = dA.ptrs(dtype)
     for _ in range(3):
-        L.check(L.hip().trl_hip_spmm_csr(ctx.h, L.dtype_code(dt), dA.n, b, dA.rowptr.ptr, dA.col.ptr, val, X.ptr, Y.ptr, None, 0), "spmm")
+        op.apply(X, Y)
     ctx.prof_enable(True)
     for _ in range(reps):
-        L.check(L.hip().trl_hip_spmm_csr(ctx.h, L.dtype_code(dt), dA.n, b, dA.rowptr.ptr, dA.col.ptr, val, X.ptr, Y.ptr, None, 0), "spmm")
+        op.apply(X, Y)
     st = ctx.prof_read()["spmm"]
     ctx.prof_enable(False)
     avg = st["seconds"] / st["calls"]
     per = st["bytes"] / st["calls"]
-    return {"dtype": dtype, "launches": st["calls"], "avg_us": avg * 1e6, "bytes_per_launch": per, "GBps": per / avg / 1e9, "frac_of_peak": per / avg / 1e9 / HBM_PEAK_GBPS}
+    return {"dtype": dtype, "kernel": op.name, "launches": st["calls"], "avg_us": avg * 1e6, "bytes_per_launch": per, "GBps": per / avg / 1e9,
+            "frac_of_peak": per / avg / 1e9 / HBM_PEAK_GBPS}
 
 
 def cpu_baseline(cfg, fp64_cycles, budget_s):
@@ -108,6 +108,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-tune", action="store_true", help="untuned CSR operator kernel instead of the autotuned one")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported (see DESIGN.md for how the path would shard)"}))
@@ -128,12 +129,13 @@ def main(argv=None) -> int:
     log("inputs resident in HBM")
 
     # ---- SpMM rate (HIP events on the library stream) -------------------------
-    sp32 = spmm_rate(ctx, dA, cfg.b, "f32")
-    sp64 = spmm_rate(ctx, dA, cfg.b, "f64")
-    log(f"spmm f32 {sp32['GBps']:.0f} GB/s ({sp32['avg_us']:.1f} us), f64 {sp64['GBps']:.0f} GB/s")
+    tune = not a.no_tune
+    sp32 = spmm_rate(ctx, dA, cfg.b, "f32", tune)
+    sp64 = spmm_rate(ctx, dA, cfg.b, "f64", tune)
+    log(f"spmm f32 {sp32['kernel']} {sp32['GBps']:.0f} GB/s ({sp32['avg_us']:.1f} us), f64 {sp64['kernel']} {sp64['GBps']:.0f} GB/s ({sp64['avg_us']:.1f} us)")
 
     # ---- warmup + profiled pass -------------------------------------------------
-    mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks)
+    mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks, spmm_tune=tune)
     for w in range(a.warmup):
         solve_mixed(dA, cfg.b, params, V0, **mixed_kw)
         log(f"warmup {w} done")
@@ -163,7 +165,7 @@ def main(argv=None) -> int:
     if not a.no_fp64:
         ctx.sync()
         t = time.perf_counter()
-        o64 = solve(dA, cfg.b, params, V0, dtype="f64", order="ascending", reorth="cgs2")
+        o64 = solve(dA, cfg.b, params, V0, dtype="f64", order="ascending", reorth="cgs2", spmm_tune=tune)
         ctx.sync()
         t64 = time.perf_counter() - t
         agree = float(np.max(np.abs(o64.evals - last.evals) / np.abs(o64.evals)))
@@ -214,7 +216,7 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"{cfg.name}: 3D 7-point Laplacian {a.side}^3 + diag U[0,1]", "n": cfg.csr.n, "nnz": cfg.csr.nnz, "b": cfg.b, "nev": cfg.nev,
                    "ncv": cfg.ncv, "tol": cfg.tol, "order": "smallest", "tol_fp32": a.tol_fp32, "cheb_degree": a.cheb_degree,
-                   "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7},
+                   "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "spmm_tune": tune},
         "converged": bool(last.result["converged"]),
         "max_true_residual": float(last.resid.max()),
         "fp32_phase": last.result32,

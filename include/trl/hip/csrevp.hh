@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "evp_base.hh"
+#include "sparse_operator.hh"
 
 namespace trl::hip {
 
@@ -25,7 +26,8 @@ public:
   // reciprocal optional, both or neither).
   CSREVP(const Context& ctx, std::size_t n, const int32_t* rowptr, const int32_t* col, const PanelScalar* val, const PanelScalar* mass = nullptr,
          const PanelScalar* mass_inv = nullptr)
-      : Base(ctx, n), rowptr_(rowptr), col_(col), val_(val), minv_(mass_inv)
+      : Base(ctx, n), rowptr_(rowptr), col_(col), val_(val), minv_(mass_inv),
+        op_(std::make_shared<SparseOperator<PanelScalar>>(ctx, n, rowptr, col, val, mass_inv))
   {
     if ((mass == nullptr) != (mass_inv == nullptr)) throw std::invalid_argument("CSREVP: give both the mass diagonal and its reciprocal, or neither");
     this->set_mass(mass);
@@ -44,14 +46,18 @@ public:
     return e;
   }
 
-  void apply(BlockView X, BlockView Y) override
-  {
-    throw_on(trl_hip_spmm_csr(this->ctx_.get(), Base::dtype, this->N, block_size, rowptr_, col_, val_, X.data_, Y.data_, minv_, lanes_per_row_),
-             "apply");
-  }
+  void apply(BlockView X, BlockView Y) override { op_->apply(int(block_size), X.data_, Y.data_); }
 
-  // 0 = tuned default, else 1, 2, 4, 8 or 16 lanes per row
-  void set_lanes_per_row(int l) { lanes_per_row_ = l; }
+  // Fixed kernel variant of the CSR operator (trl_hip_spmat::variant).
+  void set_lanes_per_row(int l) { op_ = std::make_shared<SparseOperator<PanelScalar>>(this->ctx_, this->N, rowptr_, col_, val_, minv_, l); }
+
+  // Times the operator kernels (CSR variants and ELL) for this block size and
+  // keeps the fastest; returns its time per apply in microseconds.
+  double autotune(bool allow_ell = true) { return op_->tune(int(block_size), allow_ell); }
+
+  // Shares an operator (for instance one tuned before) over the same matrix.
+  void set_operator(std::shared_ptr<SparseOperator<PanelScalar>> op) { op_ = std::move(op); }
+  const std::shared_ptr<SparseOperator<PanelScalar>>& op() const { return op_; }
 
   const int32_t* rowptr() const { return rowptr_; }
   const int32_t* col() const { return col_; }
@@ -89,7 +95,7 @@ private:
   const int32_t* col_;
   const PanelScalar* val_;
   const PanelScalar* minv_;
-  int lanes_per_row_ = 0;
+  std::shared_ptr<SparseOperator<PanelScalar>> op_;
   std::shared_ptr<Owned> owned_;
 };
 

@@ -1,0 +1,97 @@
+#pragma once
+// A device-resident sparse operator Y = diag(d) A X with its kernel choice.
+//
+// The reference's CSR operator is a fixed loop over rows
+// (examples/openmp/csrevp.hh:36-44; one work-item per row in the SYCL form,
+// examples/sycl/csrevp.hh:159-187).  Here the same product has several
+// kernels (one row per lane, column-split rows, nonzero lane groups) and two
+// storage forms (CSR, slot-major ELL).  tune() times them for one block size
+// on this device and keeps the fastest; the ELL copy it may build is owned by
+// the operator.  The CSR arrays are borrowed.
+
+#include <cstdint>
+#include <memory>
+
+#include "context.hh"
+
+namespace trl::hip {
+
+template <class S>
+class SparseOperator {
+public:
+  static constexpr trl_hip_dtype dtype = dtype_of<S>();
+
+  // Borrows rowptr[n+1], col[nnz], val[nnz] (device) and the optional row
+  // scaling dscale[n]; variant as in trl_hip_spmat (0 = untuned default).
+  SparseOperator(const Context& ctx, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* dscale = nullptr, int variant = 0)
+      : ctx_(ctx), n_(n), csr_{0, rowptr, 0, col, val, dscale, variant}, op_(csr_)
+  {
+  }
+
+  // Times every kernel for block size b, on CSR and, when the longest row
+  // pads the ELL form by at most max_padding (slots / nnz - 1), on ELL, and
+  // keeps the fastest.  Returns its time per call in microseconds.
+  double tune(int b, bool allow_ell = true, double max_padding = 0.25)
+  {
+    if (n_ == 0) return 0.0;
+    const trl_hip_spmat* ell = nullptr;
+    if (allow_ell) {
+      int width = 0;
+      throw_on(trl_hip_csr_max_row(ctx_.get(), n_, csr_.rowptr, &width), "csr_max_row");
+      int32_t nnz = 0;
+      throw_on(trl_hip_memcpy_d2h(ctx_.get(), &nnz, csr_.rowptr + n_, sizeof(int32_t)), "nnz");
+      if (width > 0 && double(width) * double(n_) <= (1.0 + max_padding) * double(nnz)) {
+        if (!ell_ || ell_->width != width) {
+          ell_ = std::make_shared<Ell>(ctx_, n_, width);
+          throw_on(trl_hip_csr_to_ell(ctx_.get(), dtype, n_, csr_.rowptr, csr_.col, csr_.val, width, ell_->col.data(), ell_->val.data()), "csr_to_ell");
+        }
+        ell_desc_ = trl_hip_spmat{1, nullptr, width, ell_->col.data(), ell_->val.data(), csr_.dscale, 0};
+        ell = &ell_desc_;
+      }
+    }
+    double us = 0.0;
+    throw_on(trl_hip_spmm_tune(ctx_.get(), dtype, n_, b, &csr_, ell, &op_, &us), "spmm_tune");
+    if (op_.format == 0) ell_.reset(); // the ELL copy is only kept while it is used
+    tuned_b_ = b;
+    tuned_us_ = us;
+    return us;
+  }
+
+  // Y = diag(dscale) A X (scaled = false: Y = A X).
+  void apply(int b, const S* X, S* Y, bool scaled = true) const
+  {
+    trl_hip_spmat d = op_;
+    if (!scaled) d.dscale = nullptr;
+    throw_on(trl_hip_spmm(ctx_.get(), dtype, n_, b, &d, X, Y), "apply");
+  }
+
+  // Y = alpha diag(dscale) A X + beta X + gamma Z.
+  void axpby(int b, const S* X, S* Y, const S* Z, double alpha, double beta, double gamma) const
+  {
+    throw_on(trl_hip_spmm_axpby(ctx_.get(), dtype, n_, b, &op_, X, Y, Z, alpha, beta, gamma), "apply_axpby");
+  }
+
+  const trl_hip_spmat& descriptor() const { return op_; }
+  std::size_t size() const { return n_; }
+  int tuned_block_size() const { return tuned_b_; }
+  double tuned_us() const { return tuned_us_; }
+
+private:
+  struct Ell {
+    int width;
+    DeviceBuffer<int32_t> col;
+    DeviceBuffer<S> val;
+    Ell(const Context& c, std::size_t n, int w) : width(w), col(c, std::size_t(w) * n), val(c, std::size_t(w) * n) {}
+  };
+
+  Context ctx_;
+  std::size_t n_;
+  trl_hip_spmat csr_;
+  trl_hip_spmat op_;
+  trl_hip_spmat ell_desc_{};
+  std::shared_ptr<Ell> ell_;
+  int tuned_b_ = 0;
+  double tuned_us_ = 0.0;
+};
+
+} // namespace trl::hip

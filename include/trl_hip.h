@@ -125,27 +125,58 @@ int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, co
 
 /* ---- operator ------------------------------------------------------------ */
 
-/* Y = diag(dscale) A X for a CSR matrix A (n x n, int32 rowptr[n+1] and
- * col[nnz], values of the panel dtype): EVP::apply of CSREVP
- * (examples/openmp/csrevp.hh:36-44; SYCL kernel examples/sycl/csrevp.hh:159-187).
- * dscale may be NULL; dscale = 1/diag(M) applies M^{-1}K for a generalized
- * problem with a diagonal M.  lanes_per_row: 0 = tuned default, else one of
- * 1, 2, 4, 8, 16 lanes of a wavefront cooperating on one row. */
-int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
-                     const void* dscale, int lanes_per_row);
+/* A sparse operator as the kernels see it (plain data, caller-owned device
+ * arrays).  format 0 = CSR (rowptr[n+1], col/val[nnz], sorted columns),
+ * 1 = slot-major ELL (col/val[width*n], slot s of row r at s*n + r, padding
+ * slots col = r, val = 0).  dscale (optional, panel dtype, n entries) scales
+ * the rows of the product: dscale = 1/diag(M) applies M^{-1}K for a
+ * generalized problem with a diagonal M.  variant selects the kernel:
+ *   0 default, 1 one row per lane, -2 / -4 two / four lanes per row splitting
+ *   the b columns, 2 / 4 / 8 / 16 lane groups splitting the nonzeros (CSR).
+ * Indices and values are loaded in chunks of 8 nonzeros before their gathers. */
+typedef struct {
+  int format;
+  const int32_t* rowptr;
+  int width;
+  const int32_t* col;
+  const void* val;
+  const void* dscale;
+  int variant;
+} trl_hip_spmat;
 
-/* Fused three-term recurrence step on the operator:
- * Y = alpha diag(dscale) A X + beta X + gamma Z (dscale may be NULL; Z may be
- * NULL, or equal to Y for an in-place update of the older iterate; X must not
- * overlap Y).  One step of a Chebyshev polynomial filter; no reference
- * counterpart. */
+/* Y = diag(dscale) A X: EVP::apply of CSREVP (examples/openmp/csrevp.hh:36-44;
+ * SYCL kernel examples/sycl/csrevp.hh:159-187). */
+int trl_hip_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, const void* X, void* Y);
+
+/* Fused three-term recurrence step: Y = alpha diag(dscale) A X + beta X +
+ * gamma Z (Z may be NULL, or equal to Y for an in-place update of the older
+ * iterate; X must not overlap Y).  One step of a Chebyshev polynomial filter;
+ * no reference counterpart. */
+int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, const void* X, void* Y, const void* Z, double alpha,
+                       double beta, double gamma);
+
+/* CSR convenience forms of the two calls above (variant as in trl_hip_spmat). */
+int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
+                     const void* dscale, int variant);
 int trl_hip_spmm_csr_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
                            const void* X, void* Y, const void* Z, double alpha, double beta, double gamma);
-
-/* The same product for a matrix in sliced-ELL form: slot s of row r at
- * index s*n + r of ell_col / ell_val (padding slots: col = r, val = 0). */
+/* ELL convenience form (default variant). */
 int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* ell_col, const void* ell_val, const void* X, void* Y,
                      const void* dscale);
+
+/* Synchronous: the longest row of a CSR matrix (the ELL width). */
+int trl_hip_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* width);
+/* Queued: CSR -> slot-major ELL of the given width (>= the longest row);
+ * ell_col/ell_val hold width*n entries. */
+int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ell_col,
+                       void* ell_val);
+/* Synchronous autotuner of the operator kernel for one matrix and block
+ * size: times every applicable variant of `csr` and, if ell is not NULL, of
+ * `ell` (same matrix in ELL form) with HIP events on scratch panels, and
+ * writes the fastest descriptor to *best (dscale copied from csr) and its
+ * mean time per call to *best_us (may be NULL). */
+int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
+                      double* best_us);
 
 /* ---- small b x b blocks (MatrixBlockViewConcept, concepts.hh:26-39) ----- */
 

@@ -33,7 +33,11 @@ typedef struct {
   int reorth;            /* 0 ModifiedGS (reorthogonalization.hh:12-22),
                             1 block CGS2 (two multi-block Gram/update passes)    */
   int cholqr_passes;     /* 1 CholQR (evp_base.hh:39-71), 2 CholQR2, 0 default  */
-  int lanes_per_row;     /* SpMM lanes per CSR row, 0 = default                  */
+  int lanes_per_row;     /* operator kernel variant (trl_hip_spmat::variant),
+                            0 = default                                          */
+  int spmm_tune;         /* 1: time the operator kernels (CSR variants, ELL) for
+                            b at the start of the solve and use the fastest;
+                            lanes_per_row is then ignored                        */
 } trl_hip_solver_params;
 
 typedef struct {

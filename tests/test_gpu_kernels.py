@@ -27,6 +27,16 @@ def H():
     return L.hip()
 
 
+def split_ok(p, b, cs):
+    # a column split of cs lanes needs b % cs == 0 and at least 8 bytes per lane
+    return b % cs == 0 and (b // cs) * np.dtype(DT[p]).itemsize >= 8
+
+
+def spmat(fmt, rowptr=None, width=0, col=None, val=None, dscale=None, variant=0):
+    ptr = lambda a: a.ptr if a is not None else None  # noqa: E731
+    return L.SpMat(fmt, ptr(rowptr), width, ptr(col), ptr(val), ptr(dscale), variant)
+
+
 def rel(a, b):
     return np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300)
 
@@ -52,7 +62,7 @@ def dev(ctx, a, dt):
 
 @pytest.mark.parametrize("p", ["f64", "f32"])
 @pytest.mark.parametrize("b", BS)
-@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 8, 16])
+@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 8, 16, -2, -4])
 def test_spmm_csr(ctx, rng, p, b, lanes):
     print(f"spmm_csr {p} b={b} lanes={lanes}", flush=True)
     A = P.laplacian_3d(7, 5, 6, rng.uniform(0, 1, 210))
@@ -62,6 +72,9 @@ def test_spmm_csr(ctx, rng, p, b, lanes):
     rp, ci = dev(ctx, A.rowptr, np.int32), dev(ctx, A.col, np.int32)
     va, Xd, dd = dev(ctx, A.val, dt), dev(ctx, X, dt), dev(ctx, d, dt)
     Y = ctx.empty((A.n, b), dt)
+    if lanes < 0 and not split_ok(p, b, -lanes):
+        assert H().trl_hip_spmm_csr(ctx.h, L.dtype_code(dt), A.n, b, rp.ptr, ci.ptr, va.ptr, Xd.ptr, Y.ptr, None, lanes) == -1
+        return
     for scale in (None, dd):
         L.check(H().trl_hip_spmm_csr(ctx.h, L.dtype_code(dt), A.n, b, rp.ptr, ci.ptr, va.ptr, Xd.ptr, Y.ptr, scale.ptr if scale else None, lanes), "spmm")
         ref = oracle.spmm(A, X.astype(dt).astype(np.float64), d.astype(dt).astype(np.float64) if scale else None)
@@ -81,6 +94,66 @@ def test_spmm_ell(ctx, rng, p, b):
     assert rel(Y.download(), oracle.spmm(A, X.astype(dt).astype(np.float64))) < TOL[p]
 
 
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", BS)
+@pytest.mark.parametrize("variant", [0, 1, -2, -4])
+def test_spmm_ell_variants(ctx, rng, p, b, variant):
+    # slot-major ELL built on the device from CSR, through the generic entry
+    # point, with and without row scaling and in the fused axpby form
+    print(f"spmm_ell {p} b={b} variant={variant}", flush=True)
+    A = P.laplacian_3d(7, 5, 6, rng.uniform(0, 1, 210))
+    dt = DT[p]
+    X = rng.standard_normal((A.n, b)).astype(dt).astype(np.float64)
+    Z = rng.standard_normal((A.n, b)).astype(dt).astype(np.float64)
+    d = rng.uniform(0.5, 2.0, A.n).astype(dt).astype(np.float64)
+    rp, ci, va = dev(ctx, A.rowptr, np.int32), dev(ctx, A.col, np.int32), dev(ctx, A.val, dt)
+    w = C.c_int(-1)
+    L.check(H().trl_hip_csr_max_row(ctx.h, A.n, rp.ptr, C.byref(w)), "max_row")
+    assert w.value == int(np.diff(A.rowptr).max()) == 7
+    ec, ev = ctx.empty((w.value, A.n), np.int32), ctx.empty((w.value, A.n), dt)
+    L.check(H().trl_hip_csr_to_ell(ctx.h, L.dtype_code(dt), A.n, rp.ptr, ci.ptr, va.ptr, w.value, ec.ptr, ev.ptr), "to_ell")
+    width, hcol, hval = csr_to_ell(A)
+    assert width == w.value
+    assert np.array_equal(ec.download().ravel(), hcol.ravel())
+    assert np.array_equal(ev.download().ravel(), hval.astype(dt).ravel())
+    Xd, Zd, dd = dev(ctx, X, dt), dev(ctx, Z, dt), dev(ctx, d, dt)
+    Y = ctx.empty((A.n, b), dt)
+    for scale in (None, dd):
+        M = spmat(1, None, w.value, ec, ev, scale, variant)
+        rc = H().trl_hip_spmm(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), Xd.ptr, Y.ptr)
+        if variant < 0 and not split_ok(p, b, -variant):
+            assert rc == -1
+            return
+        L.check(rc, "spmm")
+        AX = oracle.spmm(A, X, d if scale else None)
+        assert rel(Y.download(), AX) < TOL[p]
+        L.check(H().trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), Xd.ptr, Y.ptr, Zd.ptr, 0.7, -1.3, 0.4), "axpby")
+        assert rel(Y.download(), 0.7 * AX - 1.3 * X + 0.4 * Z) < TOL[p]
+    M = spmat(1, None, w.value, ec, ev, None, 4)  # nonzero lane groups need CSR
+    assert H().trl_hip_spmm(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), Xd.ptr, Y.ptr) == -1
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", [1, 8])
+def test_spmm_tune(ctx, rng, p, b):
+    # the tuner returns a descriptor that computes the same product
+    A = P.laplacian_3d(20, 18, 16, rng.uniform(0, 1, 20 * 18 * 16))
+    dt = DT[p]
+    X = rng.standard_normal((A.n, b)).astype(dt).astype(np.float64)
+    rp, ci, va = dev(ctx, A.rowptr, np.int32), dev(ctx, A.col, np.int32), dev(ctx, A.val, dt)
+    width, hcol, hval = csr_to_ell(A)
+    ec, ev = dev(ctx, hcol, np.int32), dev(ctx, hval, dt)
+    csr, ell = spmat(0, rp, 0, ci, va), spmat(1, None, width, ec, ev)
+    for e in (None, ell):
+        best, us = L.SpMat(), C.c_double(0)
+        L.check(H().trl_hip_spmm_tune(ctx.h, L.dtype_code(dt), A.n, b, C.byref(csr), C.byref(e) if e else None, C.byref(best), C.byref(us)), "tune")
+        print("tuned", p, b, "ell" if best.format else "csr", best.variant, us.value)
+        assert us.value > 0 and best.format in ((0,) if e is None else (0, 1)) and best.variant != 0
+        Xd, Y = dev(ctx, X, dt), ctx.empty((A.n, b), dt)
+        L.check(H().trl_hip_spmm(ctx.h, L.dtype_code(dt), A.n, b, C.byref(best), Xd.ptr, Y.ptr), "spmm")
+        assert rel(Y.download(), oracle.spmm(A, X)) < TOL[p]
+
+
 def test_spmm_empty_rows(ctx, rng):
     # rows with no entries give zero rows (ragged CSR)
     n, b = 50, 8
@@ -94,7 +167,7 @@ def test_spmm_empty_rows(ctx, rng):
         rowptr[r + 1] = rowptr[r] + k
     A = P.CSR(n, rowptr, np.array(cols, np.int32), np.array(vals))
     X = rng.standard_normal((n, b))
-    for lanes in (1, 4, 16):
+    for lanes in (1, 4, 16, -2, -4):
         Y = ctx.empty((n, b))
         L.check(H().trl_hip_spmm_csr(ctx.h, L.F64, n, b, dev(ctx, A.rowptr, np.int32).ptr, dev(ctx, A.col, np.int32).ptr, dev(ctx, A.val, np.float64).ptr,
                                      dev(ctx, X, np.float64).ptr, Y.ptr, None, lanes), "spmm")
@@ -340,6 +413,10 @@ def test_convert_and_fill(ctx, rng):
 def test_invalid_arguments(ctx):
     assert H().trl_hip_gram(ctx.h, L.F64, 10, 3, 1, None, None, None, None) == -1  # b = 3 unsupported
     assert H().trl_hip_spmm_csr(ctx.h, L.F64, 10, 4, None, None, None, None, None, None, 3) == -1
+    assert H().trl_hip_spmm_csr(ctx.h, L.F64, 10, 4, None, None, None, None, None, None, -3) == -1
+    assert H().trl_hip_spmm(ctx.h, L.F64, 10, 4, None, None, None) == -1
+    bad = L.SpMat(2, None, 0, None, None, None, 0)
+    assert H().trl_hip_spmm(ctx.h, L.F64, 10, 4, C.byref(bad), None, None) == -1
     assert H().trl_hip_cholqr(ctx.h, L.F64, 10, 4, None, None, None, 3) == -1
 
 

@@ -70,6 +70,22 @@ def test_laplace2d_128_matches_oracle(ctx, reorth):
     assert ang < 1e-6
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_tuned_operator_solve(ctx, dtype):
+    # the autotuned operator (CSR variants or ELL) gives the same solve
+    c = P.config_c1()
+    print(f"tuned operator {dtype}", flush=True)
+    dA = DeviceCSR(ctx, c.csr)
+    dt = np.float64 if dtype == "f64" else np.float32
+    tol = c.tol if dtype == "f64" else 1e-5
+    o = solve(dA, c.b, EigensolverParams(c.nev, c.ncv, tolerance=tol), v0(ctx, c.csr.n, c.b, dt), dtype=dtype, order="ascending", spmm_tune=True)
+    assert o.result["converged"] == 1
+    assert np.all(np.abs(o.evals - c.exact) <= (1e-10 if dtype == "f64" else 1e-4) * c.exact)
+    om = solve_mixed(dA, c.b, EigensolverParams(c.nev, c.ncv, tolerance=c.tol), v0(ctx, c.csr.n, c.b), tol_fp32=1e-3, spmm_tune=True)
+    assert om.result["converged"] == 1
+    assert np.all(np.abs(om.evals - c.exact) <= 1e-10 * c.exact)
+
+
 def test_generalized_matches_scipy_golden(ctx):
     import json
     from pathlib import Path

@@ -23,6 +23,19 @@ c_void_p = C.c_void_p
 c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)
 
+
+
+class SpMat(C.Structure):
+    """trl_hip_spmat: a device-resident sparse operator and its kernel variant.
+
+    format 0 = CSR (rowptr, col, val), 1 = slot-major ELL (width, col, val)."""
+
+    _fields_ = [("format", c_int), ("rowptr", c_void_p), ("width", c_int), ("col", c_void_p), ("val", c_void_p), ("dscale", c_void_p),
+                ("variant", c_int)]
+
+
+c_spmat_p = C.POINTER(SpMat)
+
 # (name, restype, argtypes) of every symbol declared in include/trl_hip.h
 HIP_API = [
     ("trl_hip_ctx_create", c_int, [c_int, C.POINTER(c_void_p)]),
@@ -45,6 +58,11 @@ HIP_API = [
     ("trl_hip_update", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_cholqr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
     ("trl_hip_rotate", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    ("trl_hip_spmm", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_void_p, c_void_p]),
+    ("trl_hip_spmm_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_void_p, c_void_p, c_void_p, C.c_double, C.c_double, C.c_double]),
+    ("trl_hip_csr_max_row", c_int, [c_void_p, c_size, c_void_p, c_int_p]),
+    ("trl_hip_csr_to_ell", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    ("trl_hip_spmm_tune", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_spmat_p, c_spmat_p, c_double_p]),
     ("trl_hip_spmm_csr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     ("trl_hip_spmm_csr_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.c_double,
                                        C.c_double, C.c_double]),
@@ -208,7 +226,8 @@ class Context:
 
 class SolverParams(C.Structure):
     _fields_ = [("b", c_int), ("nev", C.c_uint), ("ncv", C.c_uint), ("max_restarts", C.c_uint), ("tol", C.c_double), ("order", c_int),
-                ("reorth", c_int), ("cholqr_passes", c_int), ("lanes_per_row", c_int)]
+                ("reorth", c_int), ("cholqr_passes", c_int), ("lanes_per_row", c_int),
+                ("spmm_tune", c_int)]
 
 
 class SolverResult(C.Structure):

@@ -307,46 +307,143 @@ int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, co
   return launch_rotate(ctx, t, n, b, m, V, Y, y_block_cols, kc, Out);
 }
 
-int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
-                     const void* dscale, int lanes_per_row)
+namespace {
+
+double spmm_bytes(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int panels)
 {
-  if (!ctx || !valid_b(b) || (n && (!rowptr || !X || !Y))) return TRL_HIP_EINVAL;
-  if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16) return TRL_HIP_EINVAL;
+  const double s = (t == TRL_HIP_F32) ? 4.0 : 8.0;
+  double bytes = panels * panel_bytes(t, n, b) + (A->dscale ? double(n) * s : 0.0);
+  if (A->format == 1) bytes += double(n) * A->width * (s + 4.0);
+  else bytes += double(csr_nnz(ctx, A->rowptr, n)) * (s + 4.0) + double(n + 1) * 4.0;
+  return bytes;
+}
+
+int check_spmat(size_t n, const trl_hip_spmat* A)
+{
+  if (!A || (A->format != 0 && A->format != 1) || !valid_spmm_variant(A->variant)) return TRL_HIP_EINVAL;
+  if (A->format == 0 && n && !A->rowptr) return TRL_HIP_EINVAL;
+  if (A->format == 1 && (A->width < 0 || (A->variant > 1))) return TRL_HIP_EINVAL;
+  if (n && (A->format == 0 || A->width > 0) && (!A->col || !A->val)) return TRL_HIP_EINVAL;
   if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
+  return 0;
+}
+
+} // namespace
+
+int trl_hip_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, const void* X, void* Y)
+{
+  if (!ctx || !valid_b(b) || check_spmat(n, A) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  double bytes = 0.0;
-  if (ctx->prof_on) {
-    const double s = (t == TRL_HIP_F32) ? 4.0 : 8.0;
-    bytes = double(csr_nnz(ctx, rowptr, n)) * (s + 4.0) + double(n + 1) * 4.0 + 2 * panel_bytes(t, n, b) + (dscale ? double(n) * s : 0.0);
-  }
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, bytes);
-  return launch_spmm_csr(ctx, t, n, b, rowptr, col, val, X, Y, dscale, lanes_per_row);
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 2) : 0.0);
+  return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, nullptr, false, 0, 0, 0, A->variant);
+}
+
+int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, const void* X, void* Y, const void* Z, double alpha,
+                       double beta, double gamma)
+{
+  if (!ctx || !valid_b(b) || check_spmat(n, A) || (n && (!X || !Y)) || X == Y) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, Z ? 4 : 3) : 0.0);
+  return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant);
+}
+
+int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
+                     const void* dscale, int variant)
+{
+  const trl_hip_spmat A{0, rowptr, 0, col, val, dscale, variant};
+  return trl_hip_spmm(ctx, t, n, b, &A, X, Y);
 }
 
 int trl_hip_spmm_csr_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
                            const void* X, void* Y, const void* Z, double alpha, double beta, double gamma)
 {
-  if (!ctx || !valid_b(b) || (n && (!rowptr || !X || !Y)) || X == Y) return TRL_HIP_EINVAL;
-  if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
-  if (!n) return 0;
-  double bytes = 0.0;
-  if (ctx->prof_on) {
-    const double s = (t == TRL_HIP_F32) ? 4.0 : 8.0;
-    bytes = double(csr_nnz(ctx, rowptr, n)) * (s + 4.0) + double(n + 1) * 4.0 + (Z ? 4 : 3) * panel_bytes(t, n, b);
-  }
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, bytes);
-  return launch_spmm_axpby(ctx, t, n, b, rowptr, col, val, dscale, X, Y, Z, alpha, beta, gamma);
+  const trl_hip_spmat A{0, rowptr, 0, col, val, dscale, 0};
+  return trl_hip_spmm_axpby(ctx, t, n, b, &A, X, Y, Z, alpha, beta, gamma);
 }
 
 int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* ell_col, const void* ell_val, const void* X, void* Y,
                      const void* dscale)
 {
-  if (!ctx || !valid_b(b) || width < 0 || (n && (!X || !Y)) || (n && width && (!ell_col || !ell_val))) return TRL_HIP_EINVAL;
-  if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
+  const trl_hip_spmat A{1, nullptr, width, ell_col, ell_val, dscale, 0};
+  return trl_hip_spmm(ctx, t, n, b, &A, X, Y);
+}
+
+int trl_hip_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* width)
+{
+  if (!ctx || !width || (n && !rowptr) || n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
+  *width = 0;
   if (!n) return 0;
-  const double s = (t == TRL_HIP_F32) ? 4.0 : 8.0;
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, double(n) * width * (s + 4.0) + 2 * panel_bytes(t, n, b) + (dscale ? double(n) * s : 0.0));
-  return launch_spmm_ell(ctx, t, n, b, width, ell_col, ell_val, X, Y, dscale);
+  int* d = ctx->flags + 16;
+  TRL_HIP_TRY(hipMemsetAsync(d, 0, sizeof(int), ctx->stream));
+  int rc = launch_csr_max_row(ctx, n, rowptr, d);
+  if (rc) return rc;
+  TRL_HIP_TRY(hipMemcpyAsync(width, d, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ell_col,
+                       void* ell_val)
+{
+  if (!ctx || width < 0 || n > size_t(INT32_MAX) || (n && (!rowptr || (width && (!ell_col || !ell_val))))) return TRL_HIP_EINVAL;
+  if (!n || !width) return 0;
+  return launch_csr_to_ell(ctx, t, n, rowptr, col, val, width, ell_col, ell_val);
+}
+
+int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
+                      double* best_us)
+{
+  if (!ctx || !valid_b(b) || !best || check_spmat(n, csr) || csr->format != 0 || (ell && (check_spmat(n, ell) || ell->format != 1))) return TRL_HIP_EINVAL;
+  *best = *csr;
+  if (best_us) *best_us = 0.0;
+  if (!n) return 0;
+  const size_t bytes = n * size_t(b) * (t == TRL_HIP_F32 ? 4 : 8);
+  void *X = nullptr, *Y = nullptr;
+  int rc = trl_hip_malloc(ctx, bytes, &X);
+  if (!rc) rc = trl_hip_malloc(ctx, bytes, &Y);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc) rc = trl_hip_fill_zero(ctx, X, bytes);
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = TRL_HIP_ENOMEM;
+  double best_t = 1e300;
+  const int csr_variants[] = {1, -2, -4, 2, 4, 8};
+  const int ell_variants[] = {1, -2, -4};
+  auto time_one = [&](const trl_hip_spmat& A) -> double {
+    for (int w = 0; w < 2; ++w)
+      if (launch_spmm(ctx, t, n, b, A.format == 0 ? A.rowptr : nullptr, A.width, A.col, A.val, A.dscale, X, Y, nullptr, false, 0, 0, 0, A.variant))
+        return -1.0; // variant not applicable to this b / dtype
+    const int reps = 5;
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) return -1.0;
+    for (int r = 0; r < reps; ++r)
+      launch_spmm(ctx, t, n, b, A.format == 0 ? A.rowptr : nullptr, A.width, A.col, A.val, A.dscale, X, Y, nullptr, false, 0, 0, 0, A.variant);
+    if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+    return 1e3 * ms / reps;
+  };
+  for (int pass = 0; pass < 2 && !rc; ++pass) {
+    const trl_hip_spmat* base = pass == 0 ? csr : ell;
+    if (!base) continue;
+    const int* vs = pass == 0 ? csr_variants : ell_variants;
+    const int nv = pass == 0 ? 6 : 3;
+    for (int i = 0; i < nv; ++i) {
+      trl_hip_spmat A = *base;
+      A.variant = vs[i];
+      A.dscale = csr->dscale;
+      (void)hipGetLastError();
+      const double us = time_one(A);
+      (void)hipGetLastError();
+      if (us > 0.0 && us < best_t) {
+        best_t = us;
+        *best = A;
+      }
+    }
+  }
+  if (best_us && best_t < 1e300) *best_us = best_t;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (X) trl_hip_free(ctx, X);
+  if (Y) trl_hip_free(ctx, Y);
+  return rc;
 }
 
 int trl_hip_small_copy(trl_hip_ctx* ctx, int b, const double* src, double* dst, int transpose)

@@ -60,11 +60,12 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
 int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W);
 int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes);
 int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int ybc, int kc, void* Out);
-int launch_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
-                    const void* dscale, int lanes);
-int launch_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
-                      const void* X, void* Y, const void* Z, double alpha, double beta, double gamma);
-int launch_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* col, const void* val, const void* X, void* Y, const void* dscale);
+bool valid_spmm_variant(int v);
+int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, int width, const int32_t* col, const void* val, const void* dscale,
+                const void* X, void* Y, const void* Z, bool axpby, double alpha, double beta, double gamma, int variant);
+int launch_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* dev_out);
+int launch_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ecol,
+                      void* eval);
 int launch_small(trl_hip_ctx* ctx, int op, int b, const double* A, const double* B, double* C, const double* hostvals);
 int launch_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* md, const double* lambda, double* out);
 

@@ -1,14 +1,23 @@
-// Sparse operator kernels: Y = diag(d) A X with X, Y n x b row-major panels.
+// Sparse operator kernels: Y = diag(d) A X (or the fused three-term form
+// Y = alpha diag(d) A X + beta X + gamma Z) for n x b row-major panels.
 //
-// CSR: L lanes of a wavefront share one row (L in {1,2,4,8,16}).  With L = 1
-// a lane owns a row and loads its indices/values in chunks before gathering
-// the b-wide rows of X into registers; with L > 1 the lanes of a group walk
-// the row's nonzeros with coalesced index/value loads and the partial rows are
-// combined by a butterfly (shfl_xor) reduction inside the group.  ELL: one row
-// per lane, slot-major arrays (fully coalesced), chunked the same way.
+// Row kernels (one row per CS lanes, CS in {1, 2, 4}; lane `part` owns
+// columns [part*b/CS, (part+1)*b/CS) of the row): the row's column indices and
+// values are loaded kChunk at a time before the gathers of X, so a chunk costs
+// two memory latencies instead of two per nonzero; padding slots gather the
+// row's own X row with value 0.  Storage is CSR (int32 rowptr/col) or
+// slot-major ELL (slot s of row r at s*n + r; fully coalesced index/value
+// streams).  Splitting columns instead of nonzeros keeps every lane's
+// registers small (b/CS values per gathered row) without a reduction.
+//
+// Lane-group kernel (CSR only, L in {2,4,8,16} lanes per row): lanes walk the
+// row's nonzeros sub, sub+L, ... and the partial rows are combined by a
+// butterfly (shfl_xor) inside the group; for long rows.
 //
 // Workgroups are renumbered so that each XCD walks a contiguous range of rows:
 // the stencil neighbours of a row (r +- 1, r +- nx, ...) then share that XCD's L2.
+#include <algorithm>
+
 #include "common.hh"
 
 using namespace trl_hip_dev;
@@ -17,6 +26,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr unsigned kXcds = 8;
+constexpr int kChunk = 8;
 
 __device__ inline unsigned xcd_block(unsigned bid, unsigned nb)
 {
@@ -24,59 +34,78 @@ __device__ inline unsigned xcd_block(unsigned bid, unsigned nb)
   return (bid % kXcds) * (nb / kXcds) + bid / kXcds;
 }
 
-// Nonzeros whose indices and values are loaded before any of their gathers.
-constexpr int kChunk = 8;
+struct Epi {
+  const void* dscale;
+  const void* Z;
+  double alpha, beta, gamma;
+};
 
-// Accumulates row `row` of A X into acc.  L == 1: one lane per row, the
-// row's column indices and values are loaded kChunk at a time before the
-// gathers of X (two memory latencies per chunk instead of two per nonzero);
-// padding slots gather the row's own X row with value 0.  L > 1: lanes
-// sub, sub+L, ... of the row, combined by a butterfly in the lane group.
-template <typename S, int B, int L>
-__device__ inline void csr_row(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X, int row,
-                               int sub, S (&acc)[B])
+// AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z
+template <typename S, int BC, bool AXPBY>
+__device__ inline void epilogue(const S (&acc)[BC], int row, int B, int off, const S* __restrict__ X, S* Y, const Epi& e)
 {
+  const S* ds = static_cast<const S*>(e.dscale);
+  S out[BC];
+  if constexpr (!AXPBY) {
+    const S d = ds ? ds[row] : S(1);
 #pragma unroll
-  for (int c = 0; c < B; ++c) acc[c] = S(0);
-  const int beg = rowptr[row], end = rowptr[row + 1];
-  if constexpr (L == 1) {
-    for (int p0 = beg; p0 < end; p0 += kChunk) {
-      int ci[kChunk];
-      S vi[kChunk];
-#pragma unroll
-      for (int q = 0; q < kChunk; ++q) {
-        const bool ok = p0 + q < end;
-        ci[q] = ok ? col[p0 + q] : row;
-        vi[q] = ok ? val[p0 + q] : S(0);
-      }
-      S x[kChunk][B];
-#pragma unroll
-      for (int q = 0; q < kChunk; ++q) RowIO<S, B>::load(X + size_t(ci[q]) * B, x[q]);
-#pragma unroll
-      for (int q = 0; q < kChunk; ++q)
-#pragma unroll
-        for (int c = 0; c < B; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
-    }
+    for (int q = 0; q < BC; ++q) out[q] = ds ? acc[q] * d : acc[q];
   }
   else {
-    for (int p = beg + sub; p < end; p += L) {
-      const int c = col[p];
-      const S v = val[p];
-      S x[B];
-      RowIO<S, B>::load(X + size_t(c) * B, x);
+    const S* Z = static_cast<const S*>(e.Z);
+    S xr[BC], zr[BC];
+    RowIO<S, BC>::load(X + size_t(row) * B + off, xr);
+    if (Z) RowIO<S, BC>::load(Z + size_t(row) * B + off, zr);
+    const double a = ds ? e.alpha * double(ds[row]) : e.alpha;
 #pragma unroll
-      for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
+    for (int q = 0; q < BC; ++q) {
+      double v = a * double(acc[q]) + e.beta * double(xr[q]);
+      if (Z) v += e.gamma * double(zr[q]);
+      out[q] = S(v);
     }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1)
-#pragma unroll
-      for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
   }
+  RowIO<S, BC>::store(Y + size_t(row) * B + off, out);
 }
 
-template <typename S, int B, int L>
-__global__ __launch_bounds__(kThreads) void k_spmm_csr(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
-                                                       const S* __restrict__ X, S* __restrict__ Y, const S* __restrict__ dscale, int n)
+template <typename S, int B, int CS, bool ELL, bool AXPBY>
+__global__ __launch_bounds__(kThreads) void k_spmm_rows(const int32_t* __restrict__ rowptr, int width, const int32_t* __restrict__ col,
+                                                        const S* __restrict__ val, const S* __restrict__ X, S* Y, Epi e, int n)
+{
+  constexpr int BC = B / CS;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int gid = int(blk * blockDim.x + threadIdx.x);
+  const int row = gid / CS;
+  const int off = (gid % CS) * BC;
+  if (row >= n) return;
+  S acc[BC];
+#pragma unroll
+  for (int c = 0; c < BC; ++c) acc[c] = S(0);
+  const int beg = ELL ? 0 : rowptr[row];
+  const int end = ELL ? width : rowptr[row + 1];
+  for (int p0 = beg; p0 < end; p0 += kChunk) {
+    int ci[kChunk];
+    S vi[kChunk];
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      const bool ok = p0 + q < end;
+      const size_t p = ELL ? size_t(p0 + q) * n + row : size_t(p0 + q);
+      ci[q] = ok ? col[p] : row;
+      vi[q] = ok ? val[p] : S(0);
+    }
+    S x[kChunk][BC];
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) RowIO<S, BC>::load(X + size_t(ci[q]) * B + off, x[q]);
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q)
+#pragma unroll
+      for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
+  }
+  epilogue<S, BC, AXPBY>(acc, row, B, off, X, Y, e);
+}
+
+template <typename S, int B, int L, bool AXPBY>
+__global__ __launch_bounds__(kThreads) void k_spmm_lanes(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
+                                                         const S* __restrict__ X, S* Y, Epi e, int n)
 {
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const int gid = int(blk * blockDim.x + threadIdx.x);
@@ -84,161 +113,164 @@ __global__ __launch_bounds__(kThreads) void k_spmm_csr(const int32_t* __restrict
   const int sub = int(threadIdx.x) % L;
   if (row >= n) return; // whole lane groups leave together
   S acc[B];
-  csr_row<S, B, L>(rowptr, col, val, X, row, sub, acc);
-  if (sub != 0) return;
-  if (dscale) {
-    const S d = dscale[row];
-#pragma unroll
-    for (int q = 0; q < B; ++q) acc[q] *= d;
-  }
-  RowIO<S, B>::store(Y + size_t(row) * B, acc);
-}
-
-// Fused three-term form: Y = alpha D A X + beta X + gamma Z (D = diag(dscale)
-// or I; Z may be Y, or NULL), one Chebyshev-recurrence step per launch.
-template <typename S, int B, int L>
-__global__ __launch_bounds__(kThreads) void k_spmm_csr_axpby(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
-                                                             const S* __restrict__ X, S* Y, const S* Z, const S* __restrict__ dscale, double alpha,
-                                                             double beta, double gamma, int n)
-{
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  const int gid = int(blk * blockDim.x + threadIdx.x);
-  const int row = gid / L;
-  const int sub = int(threadIdx.x) % L;
-  if (row >= n) return;
-  S acc[B];
-  csr_row<S, B, L>(rowptr, col, val, X, row, sub, acc);
-  if (sub != 0) return;
-  S xr[B], zr[B], out[B];
-  RowIO<S, B>::load(X + size_t(row) * B, xr);
-  if (Z) RowIO<S, B>::load(Z + size_t(row) * B, zr);
-  const double a = dscale ? alpha * double(dscale[row]) : alpha;
-#pragma unroll
-  for (int q = 0; q < B; ++q) {
-    double v = a * double(acc[q]) + beta * double(xr[q]);
-    if (Z) v += gamma * double(zr[q]);
-    out[q] = S(v);
-  }
-  RowIO<S, B>::store(Y + size_t(row) * B, out);
-}
-
-template <typename S, int B>
-__global__ __launch_bounds__(kThreads) void k_spmm_ell(int width, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X,
-                                                       S* __restrict__ Y, const S* __restrict__ dscale, int n)
-{
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  const int row = int(blk * blockDim.x + threadIdx.x);
-  if (row >= n) return;
-  S acc[B];
 #pragma unroll
   for (int c = 0; c < B; ++c) acc[c] = S(0);
-  for (int s0 = 0; s0 < width; s0 += kChunk) {
-    int ci[kChunk];
-    S vi[kChunk];
+  const int beg = rowptr[row], end = rowptr[row + 1];
+  for (int p = beg + sub; p < end; p += L) {
+    const int c = col[p];
+    const S v = val[p];
+    S x[B];
+    RowIO<S, B>::load(X + size_t(c) * B, x);
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q) {
-      const bool ok = s0 + q < width;
-      const size_t p = size_t(s0 + q) * n + row;
-      ci[q] = ok ? col[p] : row;
-      vi[q] = ok ? val[p] : S(0);
-    }
-    S x[kChunk][B];
-#pragma unroll
-    for (int q = 0; q < kChunk; ++q) RowIO<S, B>::load(X + size_t(ci[q]) * B, x[q]);
-#pragma unroll
-    for (int q = 0; q < kChunk; ++q)
-#pragma unroll
-      for (int c = 0; c < B; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
+    for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
   }
-  if (dscale) {
-    const S d = dscale[row];
 #pragma unroll
-    for (int q = 0; q < B; ++q) acc[q] *= d;
-  }
-  RowIO<S, B>::store(Y + size_t(row) * B, acc);
+  for (int off = L / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
+  if (sub != 0) return;
+  epilogue<S, B, AXPBY>(acc, row, B, 0, X, Y, e);
 }
 
-template <typename S, int B, int L>
-int launch_csr_l(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* X, S* Y, const S* ds)
+inline unsigned xcd_grid(size_t threads)
 {
-  const size_t threads = n * L;
   size_t g = (threads + kThreads - 1) / kThreads;
-  // round the grid up to a multiple of the XCD count so the renumbering applies
-  g = (g + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL((k_spmm_csr<S, B, L>), dim3(unsigned(g)), dim3(kThreads), 0, ctx->stream, rowptr, col, val, X, Y, ds, int(n));
+  return unsigned((g + kXcds - 1) / kXcds * kXcds); // a multiple of the XCD count so the renumbering applies
+}
+
+// variant: 1 = one row per lane; -2 / -4 = 2 / 4 lanes per row splitting the
+// columns; 2, 4, 8, 16 = lane groups splitting the nonzeros (CSR only).
+// rowptr == NULL selects ELL storage (width slots).
+template <typename S, int B, bool AXPBY>
+int launch_rows(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, int width, const int32_t* col, const S* val, const S* X, S* Y, const Epi& e)
+{
+  const bool ell = rowptr == nullptr;
+  const int cs = variant < 0 ? -variant : 1;
+  const size_t lanes = cs > 1 ? size_t(cs) : size_t(variant > 1 ? variant : 1);
+  const dim3 g{xcd_grid(n * lanes)}, blk{unsigned(kThreads)};
+  if (variant > 1) {
+    if (ell) return TRL_HIP_EINVAL;
+    switch (variant) {
+    case 2:
+      hipLaunchKernelGGL((k_spmm_lanes<S, B, 2, AXPBY>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_spmm_lanes<S, B, 4, AXPBY>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_spmm_lanes<S, B, 8, AXPBY>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
+      break;
+    case 16:
+      hipLaunchKernelGGL((k_spmm_lanes<S, B, 16, AXPBY>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
+      break;
+    default:
+      return TRL_HIP_EINVAL;
+    }
+    TRL_HIP_LAUNCH_CHECK();
+    return 0;
+  }
+#define TRL_ROWS(CS_, ELL_) hipLaunchKernelGGL((k_spmm_rows<S, B, CS_, ELL_, AXPBY>), g, blk, 0, ctx->stream, rowptr, width, col, val, X, Y, e, int(n))
+  if (cs == 1) {
+    if (ell) TRL_ROWS(1, true);
+    else TRL_ROWS(1, false);
+  }
+  else if (cs == 2) {
+    // each lane moves at least 8 bytes of a row
+    if constexpr (B % 2 == 0 && (B / 2) * sizeof(S) >= 8) {
+      if (ell) TRL_ROWS(2, true);
+      else TRL_ROWS(2, false);
+    }
+    else
+      return TRL_HIP_EINVAL;
+  }
+  else if (cs == 4) {
+    if constexpr (B % 4 == 0 && (B / 4) * sizeof(S) >= 8) {
+      if (ell) TRL_ROWS(4, true);
+      else TRL_ROWS(4, false);
+    }
+    else
+      return TRL_HIP_EINVAL;
+  }
+  else
+    return TRL_HIP_EINVAL;
+#undef TRL_ROWS
   TRL_HIP_LAUNCH_CHECK();
   return 0;
 }
 
-template <typename S, int B, int L>
-int launch_axpby_l(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* X, S* Y, const S* Z, const S* ds, double a,
-                   double b, double g)
+// untuned default: one row per lane; FP64 rows of 8+ columns split in two
+template <typename S, int B>
+constexpr int default_variant()
 {
-  size_t grid = (n * L + kThreads - 1) / kThreads;
-  grid = (grid + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL((k_spmm_csr_axpby<S, B, L>), dim3(unsigned(grid)), dim3(kThreads), 0, ctx->stream, rowptr, col, val, X, Y, Z, ds, a, b, g, int(n));
-  TRL_HIP_LAUNCH_CHECK();
-  return 0;
+  return (B * sizeof(S) >= 64) ? -2 : 1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_csr_max_row(const int32_t* __restrict__ rowptr, int n, int* __restrict__ out)
+{
+  __shared__ int red[kThreads];
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, rowptr[i + 1] - rowptr[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < unsigned(s)) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_csr_to_ell(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
+                                                         int width, int n, int32_t* __restrict__ ecol, S* __restrict__ eval)
+{
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const int beg = rowptr[row], len = rowptr[row + 1] - beg;
+  for (int s = 0; s < width; ++s) {
+    const size_t p = size_t(s) * n + row;
+    const bool ok = s < len;
+    ecol[p] = ok ? col[beg + s] : row;
+    eval[p] = ok ? val[beg + s] : S(0);
+  }
 }
 
 } // namespace
 
 namespace trl_hip_impl {
 
-int launch_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
-                      const void* X, void* Y, const void* Z, double alpha, double beta, double gamma)
+int launch_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* dev_out)
 {
-  TRL_DISPATCH_T(t, {
-    TRL_DISPATCH_B(b, {
-      const S* v = static_cast<const S*>(val);
-      const S* x = static_cast<const S*>(X);
-      S* y = static_cast<S*>(Y);
-      const S* z = static_cast<const S*>(Z);
-      const S* ds = static_cast<const S*>(dscale);
-      if constexpr (B >= 8) return launch_axpby_l<S, B, 1>(ctx, n, rowptr, col, v, x, y, z, ds, alpha, beta, gamma);
-      else return launch_axpby_l<S, B, 4>(ctx, n, rowptr, col, v, x, y, z, ds, alpha, beta, gamma);
-    });
-  });
+  hipLaunchKernelGGL(k_csr_max_row, dim3(unsigned(std::min<size_t>((n + kThreads - 1) / kThreads, 1024))), dim3(kThreads), 0, ctx->stream, rowptr, int(n),
+                     dev_out);
+  TRL_HIP_LAUNCH_CHECK();
   return 0;
 }
 
-int launch_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
-                    const void* dscale, int lanes)
+int launch_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ecol,
+                      void* eval)
 {
+  const dim3 g{unsigned((n + kThreads - 1) / kThreads)}, blk{unsigned(kThreads)};
   TRL_DISPATCH_T(t, {
-    TRL_DISPATCH_B(b, {
-      const S* v = static_cast<const S*>(val);
-      const S* x = static_cast<const S*>(X);
-      S* y = static_cast<S*>(Y);
-      const S* d = static_cast<const S*>(dscale);
-      int L = lanes;
-      if (L == 0) L = (B >= 8) ? 1 : 4;
-      switch (L) {
-      case 1:
-        return launch_csr_l<S, B, 1>(ctx, n, rowptr, col, v, x, y, d);
-      case 2:
-        return launch_csr_l<S, B, 2>(ctx, n, rowptr, col, v, x, y, d);
-      case 4:
-        return launch_csr_l<S, B, 4>(ctx, n, rowptr, col, v, x, y, d);
-      case 8:
-        return launch_csr_l<S, B, 8>(ctx, n, rowptr, col, v, x, y, d);
-      default:
-        return launch_csr_l<S, B, 16>(ctx, n, rowptr, col, v, x, y, d);
-      }
-    });
+    hipLaunchKernelGGL((k_csr_to_ell<S>), g, blk, 0, ctx->stream, rowptr, col, static_cast<const S*>(val), width, int(n), ecol, static_cast<S*>(eval));
   });
+  TRL_HIP_LAUNCH_CHECK();
   return 0;
 }
 
-int launch_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* col, const void* val, const void* X, void* Y, const void* dscale)
+bool valid_spmm_variant(int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == -2 || v == -4; }
+
+int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, int width, const int32_t* col, const void* val, const void* dscale,
+                const void* X, void* Y, const void* Z, bool axpby, double alpha, double beta, double gamma, int variant)
 {
+  const Epi e{dscale, Z, alpha, beta, gamma};
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
-      size_t g = (n + kThreads - 1) / kThreads;
-      g = (g + kXcds - 1) / kXcds * kXcds;
-      hipLaunchKernelGGL((k_spmm_ell<S, B>), dim3(unsigned(g)), dim3(kThreads), 0, ctx->stream, width, col, static_cast<const S*>(val),
-                         static_cast<const S*>(X), static_cast<S*>(Y), static_cast<const S*>(dscale), int(n));
-      TRL_HIP_LAUNCH_CHECK();
+      const int v = variant == 0 ? default_variant<S, B>() : variant;
+      const S* vv = static_cast<const S*>(val);
+      const S* x = static_cast<const S*>(X);
+      S* y = static_cast<S*>(Y);
+      if (axpby) return launch_rows<S, B, true>(ctx, n, v, rowptr, width, col, vv, x, y, e);
+      return launch_rows<S, B, false>(ctx, n, v, rowptr, width, col, vv, x, y, e);
     });
   });
   return 0;

@@ -21,6 +21,7 @@
 namespace {
 
 using trl::hip::Context;
+using trl::hip::SparseOperator;
 using trl::hip::throw_on;
 
 int code_of(const std::exception& e)
@@ -42,15 +43,14 @@ double now()
 // True residuals ||K x - lambda M x|| / ||K x|| of nev Ritz vectors stored as
 // ceil(nev/B) panels at X (device), computed with the undivided operator K.
 template <class S, unsigned B>
-void true_residuals(const Context& c, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* mass, const S* X, unsigned nev,
-                    const double* evals, double* out, int lanes)
+void true_residuals(const Context& c, std::size_t n, const SparseOperator<S>& op, const S* mass, const S* X, unsigned nev, const double* evals, double* out)
 {
   constexpr trl_hip_dtype t = trl::hip::dtype_of<S>();
   trl::hip::DeviceBuffer<S> KX(c, n * B);
   const unsigned nblk = (nev + B - 1) / B;
   for (unsigned j = 0; j < nblk; ++j) {
     const S* Xj = X + std::size_t(j) * n * B;
-    throw_on(trl_hip_spmm_csr(c.get(), t, n, B, rowptr, col, val, Xj, KX.data(), nullptr, lanes), "residual spmm");
+    op.apply(int(B), Xj, KX.data(), false);
     double lam[16] = {0.0};
     for (unsigned q = 0; q < B && j * B + q < nev; ++q) lam[q] = evals[j * B + q];
     double sums[32];
@@ -59,21 +59,32 @@ void true_residuals(const Context& c, std::size_t n, const int32_t* rowptr, cons
   }
 }
 
-struct SolveOut {
-  trl_hip_solver_result res{};
-};
+// The operator of one solve: the CSR arrays with the caller's kernel
+// variant, or the fastest kernel and storage for block size B when tuning
+// is asked for.
+template <class S, unsigned B>
+std::shared_ptr<SparseOperator<S>> make_operator(const Context& c, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* minv,
+                                                 const trl_hip_solver_params& p)
+{
+  auto op = std::make_shared<SparseOperator<S>>(c, n, rowptr, col, val, minv, p.spmm_tune ? 0 : p.lanes_per_row);
+  if (p.spmm_tune) op->tune(int(B));
+  if (std::getenv("TRL_HIP_VERBOSE"))
+    std::fprintf(stderr, "[operator] %s b=%u format=%s variant=%d (%.1f us)\n", sizeof(S) == 4 ? "f32" : "f64", B, op->descriptor().format ? "ell" : "csr",
+                 op->descriptor().variant, op->tuned_us());
+  return op;
+}
 
 // Runs one solve with panel type S and block size B.  If X_out is given it
 // receives the first nblk_out Ritz blocks.
 template <class S, unsigned B>
 void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* mass, const S* minv,
-                 const trl_hip_solver_params& p, const S* V0, double* evals, S* X_out, unsigned nblk_out, double* resid, trl_hip_solver_result& res,
-                 std::vector<double>* all_evals = nullptr, unsigned* nblk_eff = nullptr)
+                 const std::shared_ptr<SparseOperator<S>>& op, const trl_hip_solver_params& p, const S* V0, double* evals, S* X_out, unsigned nblk_out,
+                 double* resid, trl_hip_solver_result& res, std::vector<double>* all_evals = nullptr, unsigned* nblk_eff = nullptr)
 {
   using EVP = trl::hip::CSREVP<S, B>;
   auto evp = std::make_shared<EVP>(c, n, rowptr, col, val, mass, minv);
   evp->set_order(static_cast<trl::hip::EigenvalueOrder>(p.order));
-  evp->set_lanes_per_row(p.lanes_per_row);
+  evp->set_operator(op);
   if (p.cholqr_passes > 0) evp->set_cholqr_passes(p.cholqr_passes);
   trl_amd::EigensolverParams ep{p.nev, p.ncv, p.max_restarts, p.tol};
 
@@ -106,7 +117,7 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
         throw_on(trl_hip_rotate(c.get(), EVP::dtype, n, B, int(p.ncv / B), lz.get_basis().data(), Y.data(), int(Y.block_cols()), int(nblk_out), X_out),
                  "ritz vectors");
       }
-      if (resid) true_residuals<S, B>(c, n, rowptr, col, val, mass, X_out, p.nev, evals, resid, p.lanes_per_row);
+      if (resid) true_residuals<S, B>(c, n, *op, mass, X_out, p.nev, evals, resid);
       c.sync();
     }
   };
@@ -130,18 +141,16 @@ template <unsigned B>
 struct BlockRefiner {
   const Context& c;
   std::size_t n;
-  const int32_t *rowptr, *col;
-  const double *val, *mass, *minv;
-  int lanes;
+  const SparseOperator<double>& op;
+  const double* mass;
   unsigned nb;
   bool ascending;
   trl::hip::DeviceBuffer<double> Xa, Xb, AX, coef;
   double* X; // current Ritz panels (Xa or Xb)
   double* spare;
 
-  BlockRefiner(const Context& ctx, std::size_t n_, const int32_t* rp, const int32_t* ci, const double* v, const double* m, const double* mi, int l, unsigned nb_,
-               bool asc)
-      : c(ctx), n(n_), rowptr(rp), col(ci), val(v), mass(m), minv(mi), lanes(l), nb(nb_), ascending(asc), Xa(ctx, std::size_t(nb_) * n_ * B),
+  BlockRefiner(const Context& ctx, std::size_t n_, const SparseOperator<double>& o, const double* m, unsigned nb_, bool asc)
+      : c(ctx), n(n_), op(o), mass(m), nb(nb_), ascending(asc), Xa(ctx, std::size_t(nb_) * n_ * B),
         Xb(ctx, std::size_t(nb_) * n_ * B), AX(ctx, std::size_t(nb_) * n_ * B), coef(ctx, std::size_t(nb_) * nb_ * B * B + 2 * B * B)
   {
     X = Xa.data();
@@ -170,7 +179,7 @@ struct BlockRefiner {
   {
     const unsigned nt = nb * B;
     for (unsigned j = 0; j < nb; ++j)
-      throw_on(trl_hip_spmm_csr(c.get(), TRL_HIP_F64, n, B, rowptr, col, val, panel(X, j), panel(AX.data(), j), minv, lanes), "refine apply");
+      op.apply(int(B), panel(X, j), panel(AX.data(), j));
     double* C = coef.data();
     for (unsigned j = 0; j < nb; ++j)
       throw_on(trl_hip_gram(c.get(), TRL_HIP_F64, n, B, int(nb), X, panel(AX.data(), j), mass, C + std::size_t(j) * nt * B), "refine projection");
@@ -211,12 +220,11 @@ struct BlockRefiner {
     for (unsigned j = 0; j < nb; ++j) {
       double* prev = panel(X, j);
       double* cur = panel(spare, j);
-      throw_on(trl_hip_spmm_csr_axpby(c.get(), TRL_HIP_F64, n, B, rowptr, col, val, minv, prev, cur, nullptr, s1 / e, -s1 * cc / e, 0.0), "filter");
+      op.axpby(int(B), prev, cur, nullptr, s1 / e, -s1 * cc / e, 0.0);
       double sg = s1;
       for (unsigned i = 2; i <= degree; ++i) {
         const double sn = 1.0 / (2.0 / s1 - sg);
-        throw_on(trl_hip_spmm_csr_axpby(c.get(), TRL_HIP_F64, n, B, rowptr, col, val, minv, cur, prev, prev, 2.0 * sn / e, -2.0 * sn * cc / e, -sg * sn),
-                 "filter");
+        op.axpby(int(B), cur, prev, prev, 2.0 * sn / e, -2.0 * sn * cc / e, -sg * sn);
         std::swap(prev, cur);
         sg = sn;
       }
@@ -277,14 +285,18 @@ int trl_hip_solve_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t
     if (t == TRL_HIP_F64) {
       rc = dispatch_b(p->b, [&](auto Bc) {
         constexpr unsigned B = decltype(Bc)::value;
-        solve_typed<double, B>(c, n, rowptr, col, static_cast<const double*>(val), static_cast<const double*>(mass), static_cast<const double*>(mass_inv),
-                               *p, static_cast<const double*>(V0), evals, static_cast<double*>(X), nblk, resid, *res);
+        const auto* v = static_cast<const double*>(val);
+        const auto* mi = static_cast<const double*>(mass_inv);
+        solve_typed<double, B>(c, n, rowptr, col, v, static_cast<const double*>(mass), mi, make_operator<double, B>(c, n, rowptr, col, v, mi, *p), *p,
+                               static_cast<const double*>(V0), evals, static_cast<double*>(X), nblk, resid, *res);
       });
     }
     else if (t == TRL_HIP_F32) {
       rc = dispatch_b(p->b, [&](auto Bc) {
         constexpr unsigned B = decltype(Bc)::value;
-        solve_typed<float, B>(c, n, rowptr, col, static_cast<const float*>(val), static_cast<const float*>(mass), static_cast<const float*>(mass_inv), *p,
+        const auto* v = static_cast<const float*>(val);
+        const auto* mi = static_cast<const float*>(mass_inv);
+        solve_typed<float, B>(c, n, rowptr, col, v, static_cast<const float*>(mass), mi, make_operator<float, B>(c, n, rowptr, col, v, mi, *p), *p,
                               static_cast<const float*>(V0), evals, static_cast<float*>(X), nblk, resid, *res);
       });
     }
@@ -316,6 +328,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       const unsigned nblk = (p->nev + B - 1) / B;
       c.sync();
       const double t0 = now();
+      const auto op64 = make_operator<double, B>(c, n, rowptr, col, val64, minv64, *p);
       // ---- FP32 phase -------------------------------------------------------
       trl::hip::DeviceBuffer<float> V0f(c, n * B);
       throw_on(trl_hip_convert(c.get(), TRL_HIP_F64, V0, TRL_HIP_F32, V0f.data(), n * B), "convert V0");
@@ -333,10 +346,11 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       trl::hip::DeviceBuffer<double> X1(c, mp->phase1_fp64 ? std::size_t(nref) * n * B : 1);
       if (mp->phase1_fp64) {
         p32.cholqr_passes = p->cholqr_passes;
-        solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, p32, V0, ev32.data(), X1.data(), nref, nullptr, r32, &all32, &got);
+        solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, op64, p32, V0, ev32.data(), X1.data(), nref, nullptr, r32, &all32, &got);
       }
       else {
-        solve_typed<float, B>(c, n, rowptr, col, val32, mass32, minv32, p32, V0f.data(), ev32.data(), X32.data(), nref, nullptr, r32, &all32, &got);
+        const auto op32 = make_operator<float, B>(c, n, rowptr, col, val32, minv32, *p);
+        solve_typed<float, B>(c, n, rowptr, col, val32, mass32, minv32, op32, p32, V0f.data(), ev32.data(), X32.data(), nref, nullptr, r32, &all32, &got);
       }
       nref = got;
 
@@ -344,7 +358,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       if (mp->refine == 1) {
         // ---- FP64 phase: block refinement of the FP32 Ritz blocks -----------
         const bool asc = (p->order == 1);
-        BlockRefiner<B> rf(c, n, rowptr, col, val64, mass64, minv64, p->lanes_per_row, nref, asc);
+        BlockRefiner<B> rf(c, n, *op64, mass64, nref, asc);
         if (mp->phase1_fp64)
           throw_on(trl_hip_memcpy_d2d(c.get(), rf.X, X1.data(), std::size_t(nref) * n * B * sizeof(double)), "copy X1");
         else
@@ -361,7 +375,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         for (;;) {
           rf.orthonormalize();
           rf.rayleigh_ritz(theta);
-          true_residuals<double, B>(c, n, rowptr, col, val64, mass64, rf.X, p->nev, theta.data(), rs.data(), p->lanes_per_row);
+          true_residuals<double, B>(c, n, *op64, mass64, rf.X, p->nev, theta.data(), rs.data());
           double worst = 0.0;
           for (double r : rs) worst = std::max(worst, r);
           if (std::getenv("TRL_HIP_VERBOSE")) {
@@ -395,7 +409,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
           // fall back to an FP64 Lanczos warm start from the refined block
           trl::hip::DeviceBuffer<double> V064(c, n * B);
           throw_on(trl_hip_memcpy_d2d(c.get(), V064.data(), rf.X, n * B * sizeof(double)), "seed");
-          solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, *p, V064.data(), evals, X, nblk, resid, *res);
+          solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, op64, *p, V064.data(), evals, X, nblk, resid, *res);
           done = true;
         }
       }
@@ -414,7 +428,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         throw_on(trl_hip_memcpy_h2d(c.get(), Gd.data(), G.data(), G.size() * sizeof(double)), "upload G");
         trl::hip::DeviceBuffer<double> V064(c, n * B);
         throw_on(trl_hip_rotate(c.get(), TRL_HIP_F64, n, B, int(kr), X64.data(), Gd.data(), 1, 1, V064.data()), "mix Ritz vectors");
-        solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, *p, V064.data(), evals, X, nblk, resid, *res);
+        solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, op64, *p, V064.data(), evals, X, nblk, resid, *res);
       }
       c.sync();
       res->seconds = now() - t0;

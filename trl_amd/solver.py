@@ -50,6 +50,55 @@ class DeviceCSR:
     def ptrs(self, p: str):
         return (self.val[p].ptr, self.mass[p].ptr if self.has_mass else None, self.minv[p].ptr if self.has_mass else None)
 
+    def operator(self, b: int, p: str = "f64", tune: bool = True, scaled: bool = False) -> "DeviceOperator":
+        """The operator kernel for block size b: the untuned CSR default, or
+        the fastest of the CSR variants and the device-built ELL form
+        (trl_hip_spmm_tune).  scaled applies diag(1/M) for a mass matrix."""
+        return DeviceOperator(self, b, p, tune, scaled)
+
+
+class DeviceOperator:
+    """A trl_hip_spmat descriptor plus the device arrays it refers to."""
+
+    def __init__(self, A: DeviceCSR, b: int, p: str, tune: bool, scaled: bool):
+        ctx, H = A.ctx, L.hip()
+        dt = np.float64 if p == "f64" else np.float32
+        self.A, self.b, self.dtype = A, b, dt
+        val, _, minv = A.ptrs(p)
+        dscale = minv if scaled else None
+        csr = L.SpMat(0, A.rowptr.ptr, 0, A.col.ptr, val, dscale, 0)
+        self.ell = None
+        self.us = None
+        self.desc = csr
+        if tune:
+            w = C.c_int()
+            L.check(H.trl_hip_csr_max_row(ctx.h, A.n, A.rowptr.ptr, C.byref(w)), "csr_max_row")
+            ell = None
+            if w.value * A.n <= 1.25 * A.nnz:
+                self.ell = (ctx.empty((w.value, A.n), np.int32), ctx.empty((w.value, A.n), dt))
+                L.check(H.trl_hip_csr_to_ell(ctx.h, L.dtype_code(dt), A.n, A.rowptr.ptr, A.col.ptr, val, w.value, self.ell[0].ptr, self.ell[1].ptr), "csr_to_ell")
+                ell = L.SpMat(1, None, w.value, self.ell[0].ptr, self.ell[1].ptr, dscale, 0)
+            best, us = L.SpMat(), C.c_double()
+            L.check(H.trl_hip_spmm_tune(ctx.h, L.dtype_code(dt), A.n, b, C.byref(csr), C.byref(ell) if ell else None, C.byref(best), C.byref(us)), "spmm_tune")
+            self.desc, self.us = best, us.value
+            if best.format == 0:
+                self.ell = None
+
+    @property
+    def name(self) -> str:
+        return f"{'ell' if self.desc.format else 'csr'}/v{self.desc.variant}"
+
+    def bytes_per_apply(self) -> float:
+        """Algorithmic HBM bytes of one apply: matrix arrays once, X read once, Y written."""
+        s = np.dtype(self.dtype).itemsize
+        panels = 2 * self.A.n * self.b * s
+        if self.desc.format == 1:
+            return panels + self.desc.width * self.A.n * (4 + s)
+        return panels + self.A.nnz * (4 + s) + (self.A.n + 1) * 4
+
+    def apply(self, X: L.DeviceArray, Y: L.DeviceArray) -> None:
+        L.check(L.hip().trl_hip_spmm(self.A.ctx.h, L.dtype_code(self.dtype), self.A.n, self.b, C.byref(self.desc), X.ptr, Y.ptr), "trl_hip_spmm")
+
 
 @dataclass
 class SolveOutput:
@@ -60,13 +109,13 @@ class SolveOutput:
     result32: dict | None = None
 
 
-def _params(b, params: EigensolverParams, order, reorth, cholqr_passes, lanes):
+def _params(b, params: EigensolverParams, order, reorth, cholqr_passes, lanes, tune):
     return L.SolverParams(b, params.nev, params.ncv, params.max_restarts, params.tolerance, ORDER.get(order, order), REORTH.get(reorth, reorth),
-                          cholqr_passes, lanes)
+                          cholqr_passes, lanes, 1 if tune else 0)
 
 
 def solve(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *, dtype="f64", order="ascending", reorth="cgs2", cholqr_passes=0,
-          lanes_per_row=0, ritz_vectors=True) -> SolveOutput:
+          lanes_per_row=0, spmm_tune=False, ritz_vectors=True) -> SolveOutput:
     ctx = A.ctx
     dt = np.float64 if dtype == "f64" else np.float32
     assert V0.dtype == dt and V0.nbytes == A.n * b * np.dtype(dt).itemsize
@@ -75,7 +124,7 @@ def solve(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *,
     evals = np.zeros(params.nev)
     resid = np.zeros(params.nev)
     res = L.SolverResult()
-    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row)
+    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row, spmm_tune)
     val, mass, minv = A.ptrs(dtype)
     rc = L.solver().trl_hip_solve_csr(ctx.h, L.dtype_code(dt), A.n, A.rowptr.ptr, A.col.ptr, val, mass, minv, C.byref(p), V0.ptr,
                                       evals.ctypes.data, X.ptr if X else None, resid.ctypes.data if X else None, C.byref(res))
@@ -84,7 +133,8 @@ def solve(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *,
 
 
 def solve_mixed(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *, tol_fp32=1e-5, max_restarts_fp32=1000, seed=7, order="ascending",
-                reorth="cgs2", cholqr_passes=0, lanes_per_row=0, refine=1, cheb_degree=80, max_refine=8, refine_blocks=0, phase1_fp64=False) -> SolveOutput:
+                reorth="cgs2", cholqr_passes=0, lanes_per_row=0, spmm_tune=False, refine=1, cheb_degree=80, max_refine=8, refine_blocks=0,
+                phase1_fp64=False) -> SolveOutput:
     ctx = A.ctx
     assert V0.dtype == np.float64
     nblk = (params.nev + b - 1) // b
@@ -93,7 +143,7 @@ def solve_mixed(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArr
     resid = np.zeros(params.nev)
     res = L.SolverResult()
     res32 = L.SolverResult()
-    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row)
+    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row, spmm_tune)
     mp = L.MixedParams(tol_fp32, max_restarts_fp32, seed, refine, cheb_degree, max_refine, 1 if phase1_fp64 else 0, refine_blocks)
     v64, m64, mi64 = A.ptrs("f64")
     v32, m32, mi32 = A.ptrs("f32")
