@@ -163,10 +163,16 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ 
   __shared__ double red[16][17];
   const int pl = threadIdx.x & 15, s = threadIdx.x >> 4;
   const int p = blockIdx.x * 16 + pl;
-  double acc = 0.0;
-  if (p < P)
-    for (int w = s; w < nw; w += 16) acc += partials[size_t(w) * P + p];
-  red[s][pl] = acc;
+  double acc = 0.0, acc1 = 0.0;
+  if (p < P) {
+    int w = s;
+    for (; w + 16 < nw; w += 32) {
+      acc += partials[size_t(w) * P + p];
+      acc1 += partials[size_t(w + 16) * P + p];
+    }
+    if (w < nw) acc += partials[size_t(w) * P + p];
+  }
+  red[s][pl] = acc + acc1;
   __syncthreads();
   if (s == 0 && p < P) {
     double t = 0.0;
@@ -254,46 +260,60 @@ __global__ __launch_bounds__(kThreads) void k_cholqr_pass(S* __restrict__ W, con
   double g[NP];
 #pragma unroll
   for (int i = 0; i < NP; ++i) g[i] = 0.0;
+  // two rows per iteration: both loads are in flight before the arithmetic
   const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t row = size_t(blockIdx.x) * blockDim.x + threadIdx.x; row < n; row += stride) {
-    S w[B];
-    RowIO<S, B>::load(W + row * B, w);
-    double wd[B];
-    if constexpr (SCALE) {
-      double acc[B];
-#pragma unroll
-      for (int c = 0; c < B; ++c) acc[c] = 0.0;
-#pragma unroll
-      for (int cp = 0; cp < B; ++cp) {
-        const double x = static_cast<double>(w[cp]);
-#pragma unroll
-        for (int c = cp; c < B; ++c) acc[c] = fma(x, Rs[cp * B + c], acc[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < B; ++c) {
-        w[c] = static_cast<S>(acc[c]);
-        wd[c] = static_cast<double>(w[c]);
-      }
-      RowIO<S, B>::store(W + row * B, w);
+  for (size_t row = size_t(blockIdx.x) * blockDim.x + threadIdx.x; row < n; row += 2 * stride) {
+    const size_t rows[2] = {row, row + stride};
+    const bool has2 = rows[1] < n;
+    S w[2][B];
+    RowIO<S, B>::load(W + rows[0] * B, w[0]);
+    if (has2) RowIO<S, B>::load(W + rows[1] * B, w[1]);
+    double mm[2] = {1.0, 1.0};
+    if (md) {
+      mm[0] = static_cast<double>(md[rows[0]]);
+      if (has2) mm[1] = static_cast<double>(md[rows[1]]);
     }
-    else {
 #pragma unroll
-      for (int c = 0; c < B; ++c) wd[c] = static_cast<double>(w[c]);
-    }
-    const double m = md ? static_cast<double>(md[row]) : 1.0;
-    int idx = 0;
+    for (int u = 0; u < 2; ++u) {
+      if (u == 1 && !has2) break;
+      double wd[B];
+      if constexpr (SCALE) {
+        double acc[B];
 #pragma unroll
-    for (int i = 0; i < B; ++i) {
-      const double wi = wd[i] * m;
+        for (int c = 0; c < B; ++c) acc[c] = 0.0;
 #pragma unroll
-      for (int j = i; j < B; ++j) {
-        g[idx] = fma(wi, wd[j], g[idx]);
-        ++idx;
+        for (int cp = 0; cp < B; ++cp) {
+          const double x = static_cast<double>(w[u][cp]);
+#pragma unroll
+          for (int c = cp; c < B; ++c) acc[c] = fma(x, Rs[cp * B + c], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          w[u][c] = static_cast<S>(acc[c]);
+          wd[c] = static_cast<double>(w[u][c]);
+        }
+        RowIO<S, B>::store(W + rows[u] * B, w[u]);
+      }
+      else {
+#pragma unroll
+        for (int c = 0; c < B; ++c) wd[c] = static_cast<double>(w[u][c]);
+      }
+      int idx = 0;
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const double wi = wd[i] * mm[u];
+#pragma unroll
+        for (int j = i; j < B; ++j) {
+          g[idx] = fma(wi, wd[j], g[idx]);
+          ++idx;
+        }
       }
     }
   }
-  // workgroup reduction: butterfly inside each wave, then waves in order
+  // workgroup reduction: butterfly inside each wave, then waves in order;
+  // partials are stored entry-major (entry i of workgroup w at i*G + w)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned G = gridDim.x;
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     double v = g[i];
@@ -305,26 +325,39 @@ __global__ __launch_bounds__(kThreads) void k_cholqr_pass(S* __restrict__ W, con
   for (int i = threadIdx.x; i < NP; i += blockDim.x) {
     double t = 0.0;
     for (int q = 0; q < kThreads / 64; ++q) t += red[q][i];
-    partials[size_t(blockIdx.x) * NP + i] = t;
+    partials[size_t(i) * G + blockIdx.x] = t;
   }
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
   __syncthreads();
-  if (ticket != gridDim.x - 1) return;
+  if (ticket != G - 1) return;
   __threadfence();
-  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
-    double t = 0.0;
-    for (unsigned w = 0; w < gridDim.x; ++w) t += partials[size_t(w) * NP + i];
-    // unpack the upper-triangle index i into (r, c) and mirror
-    int r = 0, rem = i;
-    while (rem >= B - r) {
-      rem -= B - r;
-      ++r;
+  // last workgroup: one wave per entry, lanes stride over the workgroups,
+  // then a butterfly (fixed order: deterministic for a given grid)
+  for (int i = wave; i < NP; i += kThreads / 64) {
+    const double* src = partials + size_t(i) * G;
+    double t0 = 0.0, t1 = 0.0;
+    unsigned w = lane;
+    for (; w + 64 < G; w += 128) {
+      t0 += src[w];
+      t1 += src[w + 64];
     }
-    const int c = r + rem;
-    Gs[r * B + c] = t;
-    Gs[c * B + r] = t;
+    if (w < G) t0 += src[w];
+    double t = t0 + t1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0) {
+      // unpack the upper-triangle index i into (r, c) and mirror
+      int r = 0, rem = i;
+      while (rem >= B - r) {
+        rem -= B - r;
+        ++r;
+      }
+      const int c = r + rem;
+      Gs[r * B + c] = t;
+      Gs[c * B + r] = t;
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
