@@ -25,7 +25,7 @@
 #include <stdexcept>
 #include <vector>
 
-#include "trl_amd/dense.hh"
+#include "trl_amd/rayleigh_ritz.hh"
 
 #include "blockmatrix.hh"
 #include "context.hh"
@@ -80,54 +80,15 @@ public:
     ctx_.sync(); // raises a pending Cholesky breakdown
     const auto t0 = std::chrono::steady_clock::now();
     const std::size_t nb = T.block_rows();
-    const std::size_t nt = nb * blocksize;
     const std::vector<double> tb = T.to_host();
     const std::vector<double> bh = beta.to_host();
-    dense_.resize(nt * nt);
-    for (std::size_t i = 0; i < nb; ++i)
-      for (std::size_t j = 0; j < T.block_cols(); ++j)
-        for (unsigned bi = 0; bi < blocksize; ++bi)
-          for (unsigned bj = 0; bj < blocksize; ++bj)
-            dense_[(i * blocksize + bi) * nt + (j * blocksize + bj)] = tb[((i * T.block_cols() + j) * blocksize + bi) * blocksize + bj];
-    std::vector<double> w(nt), Z(nt * nt);
-    if (trl_amd::dense::syev(int(nt), dense_.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
-
-    std::vector<unsigned> idx(nt);
-    std::iota(idx.begin(), idx.end(), 0u);
-    if (order_ == EigenvalueOrder::LargestMagnitude)
-      std::stable_sort(idx.begin(), idx.end(), [&](unsigned a, unsigned b) { return std::abs(w[a]) > std::abs(w[b]); });
-    else if (order_ == EigenvalueOrder::Descending)
-      std::reverse(idx.begin(), idx.end());
-
-    eigenvalues.resize(nt);
-    for (std::size_t i = 0; i < nt; ++i) eigenvalues[i] = w[idx[i]];
+    trl_amd::rayleigh_ritz(blocksize, nb, T.block_cols(), tb.data(), bh.data(), nev, static_cast<int>(order_), tolerance_, rr_, dense_);
+    eigenvalues = rr_.theta;
+    evec_host_ = rr_.Y;
+    last_residuals_ = rr_.residual;
     if (!eigenvectors || eigenvectors->block_rows() != nb) eigenvectors = std::make_unique<BlockMatrix>(ctx_, nb, T.block_cols());
-    evec_host_.assign(nb * T.block_cols() * blocksize * blocksize, 0.0);
-    for (std::size_t i = 0; i < nb; ++i)
-      for (std::size_t j = 0; j < T.block_cols(); ++j)
-        for (unsigned bi = 0; bi < blocksize; ++bi)
-          for (unsigned bj = 0; bj < blocksize; ++bj)
-            evec_host_[((i * T.block_cols() + j) * blocksize + bi) * blocksize + bj] = Z[(i * blocksize + bi) * nt + idx[j * blocksize + bj]];
     eigenvectors->from_host(evec_host_);
-
-    std::size_t n_converged = 0;
-    const std::size_t last = nb - 1;
-    const double eps = std::numeric_limits<double>::epsilon();
-    const std::size_t n_check = std::min<std::size_t>(nev, nt);
-    last_residuals_.assign(n_check, 0.0);
-    for (std::size_t col = 0; col < n_check; ++col) {
-      const std::size_t bc = col / blocksize, cb = col % blocksize;
-      const double* vl = evec_host_.data() + (last * T.block_cols() + bc) * blocksize * blocksize;
-      double nsq = 0.0;
-      for (unsigned i = 0; i < blocksize; ++i) {
-        double s = 0.0;
-        for (unsigned k = 0; k < blocksize; ++k) s += bh[i * blocksize + k] * vl[k * blocksize + cb];
-        nsq += s * s;
-      }
-      const double rel = std::sqrt(nsq) / std::max(std::abs(eigenvalues[col]), eps);
-      last_residuals_[col] = rel;
-      if (rel < tolerance_) ++n_converged;
-    }
+    const std::size_t n_converged = rr_.nconv;
     host_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return n_converged;
   }
@@ -202,6 +163,7 @@ protected:
   std::vector<double> evec_host_;
   std::vector<double> dense_;
   std::vector<double> last_residuals_;
+  trl_amd::RitzResult rr_;
   double tolerance_ = 1e-8;
   EigenvalueOrder order_ = EigenvalueOrder::LargestMagnitude;
   double host_seconds_ = 0.0;

@@ -1,0 +1,80 @@
+#pragma once
+// Host FP64 Rayleigh-Ritz step of the thick-restart block Lanczos driver,
+// following impl/openmp/evp_base.hh:79-154: eigen-decompose the projected
+// block matrix T, order the pairs, keep the eigenvectors in block layout and
+// count the first nev pairs whose residual estimate
+// ||beta y_last|| / max(|theta|, eps) is below tol.  Shared by the
+// single-problem eigenproblem (trl::hip::EVPBase) and the batched driver.
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+#include "trl_amd/dense.hh"
+
+namespace trl_amd {
+
+// 0 = |lambda| descending (evp_base.hh:103), 1 = ascending, 2 = descending
+struct RitzResult {
+  std::vector<double> theta;     // nt Ritz values in the chosen order
+  std::vector<double> Y;         // nb x bc blocks of b x b: block (i, j) at ((i*bc + j)*b*b), row-major
+  std::vector<double> residual;  // estimates of the first min(nev, nt) pairs
+  std::size_t nconv = 0;
+};
+
+// T: nb x bc blocks of b x b in BlockMatrix layout (block (i, j) at
+// ((i*bc + j)*b*b), row-major inside); only the leading nb x nb blocks enter
+// the eigen-decomposition (bc == nb in the driver).  beta: b x b, the last
+// off-diagonal block.  dense: caller-provided workspace (resized here).
+inline void rayleigh_ritz(unsigned b, std::size_t nb, std::size_t bc, const double* T, const double* beta, std::size_t nev, int order, double tol,
+                          RitzResult& out, std::vector<double>& dense)
+{
+  const std::size_t nt = nb * b;
+  dense.resize(nt * nt);
+  for (std::size_t i = 0; i < nb; ++i)
+    for (std::size_t j = 0; j < nb; ++j)
+      for (unsigned bi = 0; bi < b; ++bi)
+        for (unsigned bj = 0; bj < b; ++bj) dense[(i * b + bi) * nt + (j * b + bj)] = T[((i * bc + j) * b + bi) * b + bj];
+  std::vector<double> w(nt), Z(nt * nt);
+  if (dense::syev(int(nt), dense.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
+
+  std::vector<unsigned> idx(nt);
+  std::iota(idx.begin(), idx.end(), 0u);
+  if (order == 0)
+    std::stable_sort(idx.begin(), idx.end(), [&](unsigned x, unsigned y) { return std::abs(w[x]) > std::abs(w[y]); });
+  else if (order == 2)
+    std::reverse(idx.begin(), idx.end());
+
+  out.theta.resize(nt);
+  for (std::size_t i = 0; i < nt; ++i) out.theta[i] = w[idx[i]];
+  out.Y.assign(nb * bc * b * b, 0.0);
+  for (std::size_t i = 0; i < nb; ++i)
+    for (std::size_t j = 0; j < nb; ++j)
+      for (unsigned bi = 0; bi < b; ++bi)
+        for (unsigned bj = 0; bj < b; ++bj) out.Y[((i * bc + j) * b + bi) * b + bj] = Z[(i * b + bi) * nt + idx[j * b + bj]];
+
+  out.nconv = 0;
+  const std::size_t last = nb - 1;
+  const double eps = std::numeric_limits<double>::epsilon();
+  const std::size_t n_check = std::min(nev, nt);
+  out.residual.assign(n_check, 0.0);
+  for (std::size_t col = 0; col < n_check; ++col) {
+    const std::size_t blk = col / b, cb = col % b;
+    const double* vl = out.Y.data() + (last * bc + blk) * b * b;
+    double nsq = 0.0;
+    for (unsigned i = 0; i < b; ++i) {
+      double s = 0.0;
+      for (unsigned k = 0; k < b; ++k) s += beta[i * b + k] * vl[k * b + cb];
+      nsq += s * s;
+    }
+    const double rel = std::sqrt(nsq) / std::max(std::abs(out.theta[col]), eps);
+    out.residual[col] = rel;
+    if (rel < tol) ++out.nconv;
+  }
+}
+
+} // namespace trl_amd

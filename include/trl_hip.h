@@ -41,6 +41,7 @@ extern "C" {
 #define TRL_HIP_ENOMEM (-3) /* device allocation failed                               */
 #define TRL_HIP_EEIG (-4)   /* dense eigensolver did not converge (evp_base.hh:96)   */
 #define TRL_HIP_ERANGE (-5) /* block index out of range (multivector.hh:87)           */
+#define TRL_HIP_EBARRIER (-6) /* a batched step's device barrier timed out             */
 
 typedef enum { TRL_HIP_F32 = 0, TRL_HIP_F64 = 1 } trl_hip_dtype;
 
@@ -211,6 +212,59 @@ int trl_hip_sddmm_adjoint(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int
  * values (may be NULL: lambda = 0).  mdiag may be NULL.  Used for the true
  * residual of Ritz pairs (Y = A X). */
 int trl_hip_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* mdiag, const double* lambda, double* out);
+
+/* ---- batched problems (no reference counterpart) --------------------------
+ *
+ * Many independent standard eigenproblems A_p x = lambda x (FP64, the same
+ * block size b and basis size ncv) advanced in lock-step by the thick-restart
+ * block Lanczos recurrence.  The problems' rows are concatenated; basis block
+ * q of all problems is one panel.  One Lanczos step of every active problem
+ * is one persistent kernel launch (rows of the new block kept in registers
+ * across the three-term recurrence, two classical Gram-Schmidt passes and
+ * Cholesky-QR, workgroups of a problem synchronised by device barriers); the
+ * steps of a restart cycle are replayed from a captured hipGraph.  The host
+ * driver (trl_hip_batch_solve_csr in trl_hip_solver.h) runs the
+ * Rayleigh-Ritz steps.  b in {1, 2, 4, 8}.  All calls are synchronous. */
+typedef struct trl_hip_batch trl_hip_batch;
+
+/* Host CSR arrays of nprob problems (n[p] rows, sorted columns); ncv a
+ * multiple of b.  Copies them into the concatenated device layout (CSR and
+ * ELL) and picks the default kernel variant. */
+int trl_hip_batch_create(trl_hip_ctx* ctx, int nprob, const size_t* n, const int32_t* const* rowptr, const int32_t* const* col, const double* const* val,
+                         int b, int ncv, trl_hip_batch** out);
+int trl_hip_batch_destroy(trl_hip_batch* batch);
+/* Kernel variant: format 0 CSR / 1 ELL, rows_per_thread (tile = 256 x that
+ * many rows); TRL_HIP_ERANGE if the grid would not fit on the device at once. */
+int trl_hip_batch_set_variant(trl_hip_batch* batch, int format, int rows_per_thread);
+/* Autotuner: times one full-width step of every variant that fits and keeps
+ * the fastest (uses block m of the basis and the coefficient arrays as
+ * scratch: call it before the first extend). */
+int trl_hip_batch_tune(trl_hip_batch* batch, double* best_us);
+int trl_hip_batch_info(const trl_hip_batch* batch, int* ntiles, int* format, int* rows_per_thread, double* tuned_us);
+/* cholqr_passes 1 (as the reference's orthonormalize) or 2; use_graph 0
+ * launches the steps directly instead of replaying a captured graph. */
+int trl_hip_batch_set_options(trl_hip_batch* batch, int cholqr_passes, int use_graph);
+/* Host <-> device copy of basis block q (n[p] x b) of problem p. */
+int trl_hip_batch_set_block(trl_hip_batch* batch, int p, int q, const double* host);
+int trl_hip_batch_get_block(trl_hip_batch* batch, int p, int q, double* host);
+/* active[p] != 0: problem p takes part in the following steps/rotations. */
+int trl_hip_batch_set_active(trl_hip_batch* batch, const int* active);
+/* Steps k..m_end-1 of every active problem (lanczos.hh:227-287): k == 0
+ * orthonormalises block 0 first; quasi = 1 makes step k the quasi-Lanczos
+ * step after a restart (lanczos.hh:150-178) with arrow[p] = T_{k,k-1} (b x b,
+ * nprob blocks).  Returns alpha (T_ii) and beta (the R of orthonormalising
+ * V_{i+1}) as nprob x m blocks (step i at [p*m + i]) and the sticky Cholesky
+ * breakdown pivots per problem (0 = none). */
+int trl_hip_batch_extend(trl_hip_batch* batch, int k, int m_end, int quasi, const double* arrow, double* alpha, double* beta, int* pflag);
+/* Thick restart of every active problem: block j < kr becomes
+ * sum_{q<m} V_q Y_p(q, j), block kr becomes V_m (lanczos.hh:105-118).  Y:
+ * nprob row-major (m b) x (kr b) matrices. */
+int trl_hip_batch_rotate(trl_hip_batch* batch, const double* Y, int kr);
+/* ||A x - lambda x|| / ||A x|| of the pairs (block q column c of every
+ * problem, lambda[(p*nblk + q)*b + c]) into out (same indexing). */
+int trl_hip_batch_residuals(trl_hip_batch* batch, int nblk, const double* lambda, double* out);
+/* Clears the breakdown pivots and the barrier error flag. */
+int trl_hip_batch_reset_flags(trl_hip_batch* batch);
 
 /* ---- profiling (no reference counterpart) --------------------------------- */
 

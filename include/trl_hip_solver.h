@@ -94,6 +94,42 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
                             const trl_hip_mixed_params* mp, const double* V0, double* evals, double* X, double* resid, trl_hip_solver_result* res,
                             trl_hip_solver_result* res32);
 
+/* ---- many independent problems in lock-step (no reference counterpart) ---- */
+
+typedef struct {
+  int b;                 /* block size: 1, 2, 4, 8                              */
+  unsigned nev, ncv, max_restarts;
+  double tol;
+  int order;             /* as trl_hip_solver_params::order                     */
+  int cholqr_passes;     /* 0/1: Cholesky-QR as the reference, 2: CholQR2       */
+  int tune;              /* 1: autotune the step kernel's tiles and format      */
+  int use_graph;         /* 1: replay each cycle's steps from a hipGraph        */
+} trl_hip_batch_params;
+
+typedef struct {
+  double seconds;        /* wall time of the restart loop (device + host)      */
+  double host_seconds;   /* of which the host Rayleigh-Ritz steps              */
+  double setup_seconds;  /* layout upload, start blocks, autotuning            */
+  unsigned cycles;       /* restart cycles of the batch (the slowest problem)  */
+  unsigned graph_launches;
+  int ntiles, format, rows_per_thread;
+  double tuned_us;       /* the chosen variant's time for one full-width step  */
+} trl_hip_batch_stats;
+
+/* nprob standard FP64 problems A_p x = lambda x (host CSR arrays, host start
+ * blocks V0[p] of n[p] x b) solved together by the thick-restart block
+ * Lanczos recurrence of trl_hip_solve_csr (block CGS2 reorthogonalisation),
+ * advanced in lock-step: one persistent kernel launch per step of every
+ * active problem, each cycle's steps replayed from a hipGraph, the
+ * Rayleigh-Ritz steps on the host in parallel threads; converged problems
+ * leave the batch.  evals: nprob x nev; resid (optional): true relative
+ * residuals, nprob x nev; X (optional): host arrays of ceil(nev/b) x n[p] x b;
+ * res[p].seconds is the time from the start of the loop to the end of the
+ * cycle in which problem p converged. */
+int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t* n, const int32_t* const* rowptr, const int32_t* const* col,
+                            const double* const* val, const double* const* V0, const trl_hip_batch_params* p, double* evals, double* resid,
+                            double* const* X, trl_hip_solver_result* res, trl_hip_batch_stats* stats);
+
 /* Host FP64 symmetric eigensolver used by the Rayleigh-Ritz step:
  * A (n x n row-major) -> ascending w, eigenvectors in the columns of Z. */
 int trl_hip_dense_syev(int n, const double* A, double* w, double* Z);

@@ -74,6 +74,19 @@ HIP_API = [
     ("trl_hip_residual_sq", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_prof_enable", c_int, [c_void_p, c_int]),
     ("trl_hip_prof_read", c_int, [c_void_p, c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("trl_hip_batch_create", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, C.POINTER(c_void_p)]),
+    ("trl_hip_batch_destroy", c_int, [c_void_p]),
+    ("trl_hip_batch_set_variant", c_int, [c_void_p, c_int, c_int]),
+    ("trl_hip_batch_tune", c_int, [c_void_p, c_double_p]),
+    ("trl_hip_batch_info", c_int, [c_void_p, c_int_p, c_int_p, c_int_p, c_double_p]),
+    ("trl_hip_batch_set_options", c_int, [c_void_p, c_int, c_int]),
+    ("trl_hip_batch_set_block", c_int, [c_void_p, c_int, c_int, c_void_p]),
+    ("trl_hip_batch_get_block", c_int, [c_void_p, c_int, c_int, c_void_p]),
+    ("trl_hip_batch_set_active", c_int, [c_void_p, c_void_p]),
+    ("trl_hip_batch_extend", c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_batch_rotate", c_int, [c_void_p, c_void_p, c_int]),
+    ("trl_hip_batch_residuals", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    ("trl_hip_batch_reset_flags", c_int, [c_void_p]),
     ("trl_hip_sddmm_adjoint", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 ]
 
@@ -242,6 +255,19 @@ class MixedParams(C.Structure):
                 ("max_refine", C.c_uint), ("phase1_fp64", c_int), ("refine_blocks", C.c_uint)]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("b", c_int), ("nev", C.c_uint), ("ncv", C.c_uint), ("max_restarts", C.c_uint), ("tol", C.c_double), ("order", c_int),
+                ("cholqr_passes", c_int), ("tune", c_int), ("use_graph", c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("host_seconds", C.c_double), ("setup_seconds", C.c_double), ("cycles", C.c_uint),
+                ("graph_launches", C.c_uint), ("ntiles", c_int), ("format", c_int), ("rows_per_thread", c_int), ("tuned_us", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 SOLVER_API = [
     ("trl_hip_solve_csr", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(SolverParams), c_void_p, c_void_p,
                                   c_void_p, c_void_p, C.POINTER(SolverResult)]),
@@ -249,6 +275,8 @@ SOLVER_API = [
                                         C.POINTER(SolverParams), C.POINTER(MixedParams), c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(SolverResult),
                                         C.POINTER(SolverResult)]),
     ("trl_hip_dense_syev", c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_batch_solve_csr", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(BatchParams), c_void_p, c_void_p,
+                                        c_void_p, c_void_p, C.POINTER(BatchStats)]),
 ]
 
 _solver = None

@@ -108,6 +108,8 @@ const char* trl_hip_strerror(int code)
     return "Eigendecomposition failed";
   case TRL_HIP_ERANGE:
     return "block index out of range";
+  case TRL_HIP_EBARRIER:
+    return "device barrier of a batched step timed out";
   default:
     if (code > 0) return hipGetErrorString(static_cast<hipError_t>(code));
     return "unknown trl_hip error";
