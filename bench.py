@@ -96,6 +96,44 @@ def cpu_baseline(cfg, fp64_cycles, budget_s):
     }
 
 
+def batched_bench(ctx, sequential: bool) -> dict:
+    """The batched workload: 64 2D Laplacians (sides U{48..256}, potential
+    U[0, 2]), b = 4, 10 smallest, ncv = 48, FP64, advanced in lock-step; and
+    the same 64 problems solved one after another with the single-problem
+    path for comparison."""
+    from trl_amd import problems as P
+    from trl_amd.batch import solve_batch
+    from trl_amd.solver import DeviceCSR, EigensolverParams, solve
+
+    cfgs = P.config_c4()
+    mats = [c.csr for c in cfgs]
+    b, nev, ncv = cfgs[0].b, cfgs[0].nev, cfgs[0].ncv
+    V0 = [P.initial_block(m.n, b, seed=42 + p) for p, m in enumerate(mats)]
+    t = time.perf_counter()
+    out = solve_batch(ctx, mats, b, nev, ncv, V0, tune=True)
+    wall = time.perf_counter() - t
+    per = np.array([r["seconds"] for r in out.results])
+    st = out.stats
+    res = {"workload": "64 x 2D 5-point Laplacian, sides U{48..256} + diag U[0,2], b=4, 10 smallest, ncv=48, FP64", "problems": len(mats),
+           "rows_total": int(sum(m.n for m in mats)), "converged": int(sum(r["converged"] for r in out.results)),
+           "max_true_residual": float(out.resid.max()), "problems_per_s": len(mats) / wall, "wall_s": wall, "loop_s": st["seconds"],
+           "setup_s": st["setup_seconds"], "host_rr_s": st["host_seconds"], "cycles": st["cycles"], "p50_ms": float(np.percentile(per, 50) * 1e3),
+           "p99_ms": float(np.percentile(per, 99) * 1e3), "kernel": {"format": "ell" if st["format"] else "csr", "rows_per_tile": 256 * st["rows_per_thread"],
+                                                                     "tiles": st["ntiles"], "tuned_step_us": st["tuned_us"]}}
+    if sequential:
+        t = time.perf_counter()
+        worst = 0.0
+        for p, m in enumerate(mats):
+            dA = DeviceCSR(ctx, m, precisions=("f64",))
+            o = solve(dA, b, EigensolverParams(nev, ncv), ctx.to_device(V0[p]), order="ascending", reorth="cgs2")
+            worst = max(worst, float(np.max(np.abs(o.evals - out.evals[p]) / np.abs(o.evals))))
+        ts = time.perf_counter() - t
+        res["sequential_single_solves_s"] = ts
+        res["speedup_vs_sequential"] = ts / wall
+        res["max_rel_eig_diff_vs_single"] = worst
+    return res
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +147,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-tune", action="store_true", help="untuned CSR operator kernel instead of the autotuned one")
+    ap.add_argument("--reorth", default="dgks", choices=["mgs", "cgs2", "dgks"], help="reorthogonalisation of the mixed and FP64 solves")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-problem workload")
+    ap.add_argument("--no-batch-sequential", action="store_true", help="skip the one-by-one comparison of the batched workload")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported (see DESIGN.md for how the path would shard)"}))
@@ -135,13 +176,14 @@ def main(argv=None) -> int:
     log(f"spmm f32 {sp32['kernel']} {sp32['GBps']:.0f} GB/s ({sp32['avg_us']:.1f} us), f64 {sp64['kernel']} {sp64['GBps']:.0f} GB/s ({sp64['avg_us']:.1f} us)")
 
     # ---- warmup + profiled pass -------------------------------------------------
-    mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks, spmm_tune=tune)
+    mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks, spmm_tune=tune, reorth=a.reorth)
     for w in range(a.warmup):
         solve_mixed(dA, cfg.b, params, V0, **mixed_kw)
         log(f"warmup {w} done")
     ctx.prof_enable(True)
     prof_out = solve_mixed(dA, cfg.b, params, V0, **mixed_kw)
     fam = ctx.prof_read()
+    reorth_stats = ctx.prof_reorth()
     ctx.prof_enable(False)
     log(f"profiled mixed solve: {prof_out.result['seconds']:.2f} s, fp32 cycles {prof_out.result32['iterations']}, fp64 cycles {prof_out.result['iterations']}")
 
@@ -165,7 +207,7 @@ def main(argv=None) -> int:
     if not a.no_fp64:
         ctx.sync()
         t = time.perf_counter()
-        o64 = solve(dA, cfg.b, params, V0, dtype="f64", order="ascending", reorth="cgs2", spmm_tune=tune)
+        o64 = solve(dA, cfg.b, params, V0, dtype="f64", order="ascending", reorth=a.reorth, spmm_tune=tune)
         ctx.sync()
         t64 = time.perf_counter() - t
         agree = float(np.max(np.abs(o64.evals - last.evals) / np.abs(o64.evals)))
@@ -192,6 +234,12 @@ def main(argv=None) -> int:
     families = {k: {"calls": v["calls"], "seconds": round(v["seconds"], 6), "GBps": (v["bytes"] / v["seconds"] / 1e9) if v["seconds"] and v["bytes"] else None}
                 for k, v in fam.items() if v["calls"]}
 
+    batched = None
+    if not a.no_batch:
+        batched = batched_bench(ctx, not a.no_batch_sequential)
+        log(f"batched: {batched['problems_per_s']:.1f} problems/s, p50 {batched['p50_ms']:.0f} ms, p99 {batched['p99_ms']:.0f} ms, "
+            f"sequential {batched.get('sequential_single_solves_s', float('nan')):.2f} s")
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -216,7 +264,7 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"{cfg.name}: 3D 7-point Laplacian {a.side}^3 + diag U[0,1]", "n": cfg.csr.n, "nnz": cfg.csr.nnz, "b": cfg.b, "nev": cfg.nev,
                    "ncv": cfg.ncv, "tol": cfg.tol, "order": "smallest", "tol_fp32": a.tol_fp32, "cheb_degree": a.cheb_degree,
-                   "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "spmm_tune": tune},
+                   "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "spmm_tune": tune, "reorth": a.reorth},
         "converged": bool(last.result["converged"]),
         "max_true_residual": float(last.resid.max()),
         "fp32_phase": last.result32,
@@ -227,6 +275,8 @@ def main(argv=None) -> int:
         "spmm": {"f32": sp32, "f64": sp64},
         "roofline": roof,
         "kernel_families": families,
+        "reorth_second_passes": reorth_stats,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

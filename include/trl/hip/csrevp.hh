@@ -115,4 +115,19 @@ struct BlockCGS2 {
   }
 };
 
+// Block classical Gram-Schmidt with a second pass only where the first one
+// cancelled (the DGKS criterion: a column whose norm fell below eta times its
+// norm before the pass is orthogonalised again), through the same Reorth hook.
+// In the Lanczos recurrence the new block is nearly orthogonal to the basis
+// already, so the second pass, and half of the reorthogonalisation traffic,
+// is usually skipped.
+struct BlockCGSAdaptive {
+  double eta = 0.70710678118654752; // 1/sqrt(2)
+  template <typename EVP, typename BMV>
+  void operator()(EVP& evp, BMV& V, unsigned int count, typename BMV::BlockView V_next, typename BMV::BlockMatrix::BlockView /*tmp*/) const
+  {
+    evp.reorth_blocks(V, count, V_next, eta);
+  }
+};
+
 } // namespace trl::hip

@@ -134,6 +134,16 @@ public:
     return C;
   }
 
+  // W orthogonalised against V[0:count] in the M-inner product by block
+  // classical Gram-Schmidt: two passes (eta <= 0), or a second pass only where
+  // the first one cancelled (the DGKS test with threshold eta), decided on the
+  // device.
+  void reorth_blocks(const BlockMultivector& V, unsigned count, const BlockView& W, double eta)
+  {
+    double* C = coef(std::size_t(2 * count + 1) * blocksize * blocksize);
+    throw_on(trl_hip_reorth(ctx_.get(), dtype, N, blocksize, int(count), V.data(), W.data_, mass_, C, eta), "reorth_blocks");
+  }
+
   // W -= V[0:count] C
   void update_blocks(const BlockMultivector& V, unsigned count, const double* C, const BlockView& W)
   {

@@ -124,6 +124,20 @@ int trl_hip_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, 
  * overlap V. */
 int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int y_block_cols, int kc, void* Out);
 
+/* Block classical Gram-Schmidt of W against V[0:k] in the M-inner product
+ * (the Reorth hook, reorthogonalization.hh:29-32; the reference's single
+ * block-MGS pass is reorthogonalization.hh:12-22): C = V^T M W,
+ * W -= V C, and a second pass with fresh coefficients
+ *   eta <= 0:     always (CGS2);
+ *   0 < eta < 1:  only if some column's norm after the first pass is below
+ *                 eta times its norm before (the DGKS test, Daniel, Gragg,
+ *                 Kaufman & Stewart 1976), decided on the device: the
+ *                 second pass's kernels return at once when it is not needed.
+ * When W is the block right after V[0:k] (W == V + k*n*b) the first pass
+ * also yields W^T M W in the same sweep.  C: device scratch of
+ * (2k+1)*b*b doubles. */
+int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, void* W, const void* mdiag, double* C, double eta);
+
 /* ---- operator ------------------------------------------------------------ */
 
 /* A sparse operator as the kernels see it (plain data, caller-owned device
@@ -287,6 +301,9 @@ int trl_hip_prof_enable(trl_hip_ctx* ctx, int on);
 /* Synchronous: folds finished calls in and returns the totals of one family
  * since the last enable: calls, seconds between the event pairs, bytes. */
 int trl_hip_prof_read(trl_hip_ctx* ctx, int family, long long* calls, double* seconds, double* bytes);
+/* Profiling mode: trl_hip_reorth calls with eta > 0 since the last enable,
+ * and how many of them took the second pass. */
+int trl_hip_prof_reorth(trl_hip_ctx* ctx, long long* calls, long long* second_passes);
 
 #ifdef __cplusplus
 }

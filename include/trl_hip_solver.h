@@ -31,7 +31,8 @@ typedef struct {
   int order;             /* 0 |lambda| descending (evp_base.hh:103), 1 ascending,
                             2 descending                                         */
   int reorth;            /* 0 ModifiedGS (reorthogonalization.hh:12-22),
-                            1 block CGS2 (two multi-block Gram/update passes)    */
+                            1 block CGS2 (two multi-block Gram/update passes),
+                            2 block CGS with the DGKS second-pass test          */
   int cholqr_passes;     /* 1 CholQR (evp_base.hh:39-71), 2 CholQR2, 0 default  */
   int lanes_per_row;     /* operator kernel variant (trl_hip_spmat::variant),
                             0 = default                                          */

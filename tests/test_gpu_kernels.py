@@ -421,6 +421,57 @@ def test_invalid_arguments(ctx):
 
 
 @pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", [1, 4, 8])
+@pytest.mark.parametrize("mass", [False, True])
+@pytest.mark.parametrize("contiguous", [True, False])
+def test_reorth_cgs_dgks(ctx, rng, p, b, mass, contiguous):
+    # W against k M-orthonormal blocks: eta <= 0 is CGS2 (compared with the
+    # oracle's gram/update applied twice); eta = 1/sqrt(2) takes the second
+    # pass only for a W with large components along V
+    dt = DT[p]
+    n, k = 3000, 5
+    m = rng.uniform(0.5, 2.0, n) if mass else np.ones(n)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, k * b)))
+    Q = Q / np.sqrt(m)[:, None]  # M-orthonormal columns: Q^T M Q = I
+    Vh = np.stack([Q[:, j * b:(j + 1) * b] for j in range(k)]).astype(dt)
+    md = dev(ctx, m, dt) if mass else None
+    tol = 1e-12 if p == "f64" else 2e-5
+    for case in ("cancel", "near"):
+        if case == "cancel":  # mostly inside span(V): the first pass cancels
+            W0 = 1e-3 * rng.standard_normal((n, b)) + 10.0 * Q[:, :b] @ rng.standard_normal((b, b))
+        else:
+            W0 = rng.standard_normal((n, b))
+            W0 -= Q @ (Q.T @ (m[:, None] * W0))
+            W0 += 1e-9 * Q[:, b:2 * b]
+        W0 = W0.astype(dt)
+        for eta in (0.0, 0.70710678118654752):
+            if contiguous:
+                Vd = dev(ctx, np.concatenate([Vh, W0[None]]), dt)
+                Wp = Vd.offset(k * n * b)
+            else:
+                Vd = dev(ctx, Vh, dt)
+                Wd = dev(ctx, W0, dt)
+                Wp = Wd.ptr
+            Cd = ctx.empty(((2 * k + 1) * b * b,))
+            ctx.prof_enable(True)
+            L.check(H().trl_hip_reorth(ctx.h, L.dtype_code(dt), n, b, k, Vd.ptr, Wp, md.ptr if md else None, Cd.ptr, eta), "reorth")
+            calls, second = C.c_longlong(), C.c_longlong()
+            L.check(H().trl_hip_prof_reorth(ctx.h, C.byref(calls), C.byref(second)), "prof_reorth")
+            ctx.prof_enable(False)
+            out = np.empty((n, b), dt)
+            L.check(H().trl_hip_memcpy_d2h(ctx.h, out.ctypes.data, Wp, out.nbytes), "d2h")
+            Vf = Vh.astype(np.float64)
+            ref = W0.astype(np.float64)
+            for _ in range(2 if (eta <= 0 or case == "cancel") else 1):
+                oracle.update(Vf, oracle.gram(Vf, ref, m if mass else None), ref)
+            assert rel(out, ref) < tol * 10
+            ortho = np.abs(Q.T @ (m[:, None] * out.astype(np.float64))).max() / np.abs(out).max()
+            assert ortho < (1e-13 if p == "f64" else 1e-5), (case, eta, ortho)
+            if eta > 0:
+                assert calls.value == 1 and second.value == (1 if case == "cancel" else 0), (case, calls.value, second.value)
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
 @pytest.mark.parametrize("b", [1, 4, 6, 8, 16])
 @pytest.mark.parametrize("variant", ["z", "noz", "inplace", "scaled"])
 def test_spmm_axpby(ctx, rng, p, b, variant):

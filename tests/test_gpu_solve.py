@@ -29,7 +29,7 @@ def max_angle(Xa, Xb):
     return float(np.max(sla.subspace_angles(Qa, Qb)))
 
 
-@pytest.mark.parametrize("reorth", ["mgs", "cgs2"])
+@pytest.mark.parametrize("reorth", ["mgs", "cgs2", "dgks"])
 @pytest.mark.parametrize("b", [1, 2, 4, 8])
 @pytest.mark.parametrize("kind", ["diagonal", "laplace1d"])
 def test_reference_convergence(ctx, kind, b, reorth):
@@ -47,7 +47,7 @@ def test_reference_convergence(ctx, kind, b, reorth):
     assert np.all(np.abs(out.evals - ref["evals"]) <= 1e-10 * np.abs(ref["evals"]))
 
 
-@pytest.mark.parametrize("reorth", ["mgs", "cgs2"])
+@pytest.mark.parametrize("reorth", ["mgs", "cgs2", "dgks"])
 def test_laplace2d_128_matches_oracle(ctx, reorth):
     # 2D 5-point Laplacian 128^2, b = 4, 8 smallest, ncv = 256, tol 1e-8
     c = P.config_c1()
@@ -86,7 +86,8 @@ def test_tuned_operator_solve(ctx, dtype):
     assert np.all(np.abs(om.evals - c.exact) <= 1e-10 * c.exact)
 
 
-def test_generalized_matches_scipy_golden(ctx):
+@pytest.mark.parametrize("reorth", ["cgs2", "dgks"])
+def test_generalized_matches_scipy_golden(ctx, reorth):
     import json
     from pathlib import Path
 
@@ -94,7 +95,7 @@ def test_generalized_matches_scipy_golden(ctx):
     c = P.config_c2()
     assert c.csr.digest() == g["csr_sha256"]
     dA = DeviceCSR(ctx, c.csr, mass=c.mass, precisions=("f64",))
-    out = solve(dA, c.b, EigensolverParams(c.nev, c.ncv), v0(ctx, c.csr.n, c.b), order="ascending", reorth="cgs2")
+    out = solve(dA, c.b, EigensolverParams(c.nev, c.ncv), v0(ctx, c.csr.n, c.b), order="ascending", reorth=reorth)
     assert out.result["converged"] == 1
     ref = np.array(g["eigenvalues"])
     print("rel vs scipy", (np.abs(out.evals - ref) / ref).max())
@@ -139,8 +140,9 @@ def test_max_restarts_not_converged(ctx):
     assert out.result["converged"] == 0 and out.result["iterations"] == 2
 
 
+@pytest.mark.parametrize("reorth", ["cgs2", "dgks"])
 @pytest.mark.parametrize("phase1_fp64", [False, True])
-def test_mixed_refine_laplace3d(ctx, phase1_fp64):
+def test_mixed_refine_laplace3d(ctx, phase1_fp64, reorth):
     # 3D Laplacian 24^3 + diag U[0,1] (a small stand-in for 160^3), b = 8, 32 smallest:
     # FP32 block Lanczos to 1e-3, then FP64 Chebyshev/Rayleigh-Ritz sweeps; the
     # result must match the plain FP64 solve and the CPU oracle
@@ -148,8 +150,8 @@ def test_mixed_refine_laplace3d(ctx, phase1_fp64):
     dA = DeviceCSR(ctx, c.csr)
     V0 = v0(ctx, c.csr.n, c.b)
     prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
-    o64 = solve(dA, c.b, prm, V0)
-    om = solve_mixed(dA, c.b, prm, V0, tol_fp32=1e-3, cheb_degree=60, phase1_fp64=phase1_fp64)
+    o64 = solve(dA, c.b, prm, V0, reorth=reorth)
+    om = solve_mixed(dA, c.b, prm, V0, tol_fp32=1e-3, cheb_degree=60, phase1_fp64=phase1_fp64, reorth=reorth)
     print("mixed", om.result, om.result32)
     assert om.result["converged"] == 1
     assert om.resid.max() <= 1e-8

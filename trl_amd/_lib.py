@@ -72,6 +72,8 @@ HIP_API = [
     ("trl_hip_small_set_diagonal", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_small_mult", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_residual_sq", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_reorth", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, C.c_double]),
+    ("trl_hip_prof_reorth", c_int, [c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     ("trl_hip_prof_enable", c_int, [c_void_p, c_int]),
     ("trl_hip_prof_read", c_int, [c_void_p, c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("trl_hip_batch_create", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, C.POINTER(c_void_p)]),
@@ -219,6 +221,13 @@ class Context:
             check(hip().trl_hip_prof_read(self.h, f, C.byref(n), C.byref(s), C.byref(b)), "prof_read")
             out[name] = {"calls": n.value, "seconds": s.value, "bytes": b.value}
         return out
+
+    def prof_reorth(self) -> dict:
+        """trl_hip_reorth calls with the second-pass test since prof_enable, and
+        how many took the second pass."""
+        c, s = C.c_longlong(), C.c_longlong()
+        check(hip().trl_hip_prof_reorth(self.h, C.byref(c), C.byref(s)), "prof_reorth")
+        return {"calls": c.value, "second_passes": s.value}
 
     @property
     def num_cus(self) -> int:

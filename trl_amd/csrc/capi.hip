@@ -294,6 +294,64 @@ int trl_hip_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
   return launch_update(ctx, t, n, b, k, V, C, W);
 }
 
+int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, void* W, const void* mdiag, double* C, double eta)
+{
+  if (!ctx || !valid_b(b) || k < 0 || !C || (n && (!V || !W)) || !(eta < 1.0)) return TRL_HIP_EINVAL;
+  if (k == 0 || n == 0) return 0;
+  const size_t BB = size_t(b) * b;
+  double* C1 = C;
+  double* C2 = C + (k + 1) * BB;
+  const double P = panel_bytes(t, n, b), Pm = mdiag ? panel_bytes(t, n, 1) : 0.0;
+  int rc = 0;
+  if (eta <= 0.0) { // two full passes
+    for (double* Cp : {C1, C2}) {
+      {
+        Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm);
+        if ((rc = launch_gram(ctx, t, n, b, k, V, W, mdiag, Cp))) return rc;
+      }
+      Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P);
+      if ((rc = launch_update(ctx, t, n, b, k, V, Cp, W))) return rc;
+    }
+    return 0;
+  }
+  const size_t es = (t == TRL_HIP_F32) ? 4 : 8;
+  const bool contiguous = static_cast<const char*>(V) + size_t(k) * n * b * es == static_cast<const char*>(W);
+  {
+    // pass 1 with W^T M W in the same sweep when W follows the k blocks
+    Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm + (contiguous ? 0.0 : 2 * P + Pm));
+    if (contiguous)
+      rc = launch_gram(ctx, t, n, b, k + 1, V, W, mdiag, C1);
+    else {
+      rc = launch_gram(ctx, t, n, b, k, V, W, mdiag, C1);
+      if (!rc) rc = launch_gram(ctx, t, n, b, 1, W, W, mdiag, C1 + k * BB);
+    }
+    if (rc) return rc;
+  }
+  int* gate = ctx->flags + 24;
+  if ((rc = launch_dgks(ctx, b, k, C1, eta * eta, gate))) return rc;
+  {
+    Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P);
+    if ((rc = launch_update(ctx, t, n, b, k, V, C1, W))) return rc;
+  }
+  if (ctx->prof_on) {
+    // profiling mode: decide on the host so that skipped work is not counted
+    int g = 1;
+    TRL_HIP_TRY(hipMemcpyAsync(&g, gate, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->reorth_calls += 1;
+    if (!g) return 0;
+    ctx->reorth_second += 1;
+    {
+      Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm);
+      if ((rc = launch_gram(ctx, t, n, b, k, V, W, mdiag, C2))) return rc;
+    }
+    Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P);
+    return launch_update(ctx, t, n, b, k, V, C2, W);
+  }
+  if ((rc = launch_gram(ctx, t, n, b, k, V, W, mdiag, C2, gate))) return rc;
+  return launch_update(ctx, t, n, b, k, V, C2, W, gate);
+}
+
 int trl_hip_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* mdiag, double* R, int passes)
 {
   if (!ctx || !valid_b(b) || !R || passes < 1 || passes > 2 || (n && !W)) return TRL_HIP_EINVAL;
@@ -495,6 +553,7 @@ int trl_hip_prof_enable(trl_hip_ctx* ctx, int on)
       ctx->prof_seconds[f] = 0.0;
       ctx->prof_bytes[f] = 0.0;
     }
+    ctx->reorth_calls = ctx->reorth_second = 0;
   }
   ctx->prof_on = on ? 1 : 0;
   return 0;
@@ -508,6 +567,14 @@ int trl_hip_prof_read(trl_hip_ctx* ctx, int family, long long* calls, double* se
   if (calls) *calls = ctx->prof_count[family];
   if (seconds) *seconds = ctx->prof_seconds[family];
   if (bytes) *bytes = ctx->prof_bytes[family];
+  return 0;
+}
+
+int trl_hip_prof_reorth(trl_hip_ctx* ctx, long long* calls, long long* second_passes)
+{
+  if (!ctx) return TRL_HIP_EINVAL;
+  if (calls) *calls = ctx->reorth_calls;
+  if (second_passes) *second_passes = ctx->reorth_second;
   return 0;
 }
 

@@ -20,7 +20,8 @@ struct trl_hip_ctx {
   // per-workgroup partial sums of the block products (FP64)
   double* partials = nullptr;
   size_t partials_cap = 0; // in doubles
-  // flags[0]: sticky Cholesky breakdown pivot (1-based, 0 = none)
+  // flags[0]: sticky Cholesky breakdown pivot (1-based, 0 = none); [8]:
+  // CholQR arrival counter; [16]: csr_max_row result; [24]: second-pass gate
   int* flags = nullptr;
   // small FP64 scratch: R^{-1}, R of the first CholQR pass, coefficient staging
   double* small = nullptr;
@@ -39,6 +40,8 @@ struct trl_hip_ctx {
   long long prof_count[TRL_HIP_PROF_FAMILIES] = {};
   double prof_seconds[TRL_HIP_PROF_FAMILIES] = {};
   double prof_bytes[TRL_HIP_PROF_FAMILIES] = {};
+  // profiling mode: trl_hip_reorth calls and how many took the second pass
+  long long reorth_calls = 0, reorth_second = 0;
   std::unordered_map<const void*, long long> nnz_cache;
 };
 
@@ -56,8 +59,11 @@ int launch_fill_zero(trl_hip_ctx* ctx, void* dst, size_t bytes);
 int launch_convert(trl_hip_ctx* ctx, trl_hip_dtype st, const void* src, trl_hip_dtype dt, void* dst, size_t count);
 int launch_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const void* Y);
 int launch_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const double* W, void* Y, int mode);
-int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* md, double* C);
-int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W);
+// gate (device int, may be NULL): the kernels return at once when *gate == 0
+int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* md, double* C,
+                const int* gate = nullptr);
+int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W, const int* gate = nullptr);
+int launch_dgks(trl_hip_ctx* ctx, int b, int k, const double* C, double eta2, int* gate);
 int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes);
 int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int ybc, int kc, void* Out);
 bool valid_spmm_variant(int v);

@@ -55,8 +55,9 @@ constexpr int kGramWgPerCu = 4;
 // writes its partial sums to partials[(blockIdx.x*RS + rs)*P + idx], P = k*b*b.
 template <typename S, int B, int TPW>
 __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size_t ldv, int k, const S* __restrict__ W, const S* __restrict__ md, size_t n,
-                                                   int TG, double* __restrict__ partials)
+                                                   int TG, double* __restrict__ partials, const int* __restrict__ gate)
 {
+  if (gate && *gate == 0) return;
   using M = Mfma<S>;
   using acc_t = typename M::acc_t;
   constexpr int BP = pad_b(B), BPT = 16 / BP;
@@ -158,8 +159,10 @@ __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size
 
 // out[p] = sum_{w < nw} partials[w*P + p], summed in fixed order (deterministic):
 // 16 entries per workgroup, 16 lanes striding over w, combined in fixed order.
-__global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ partials, int nw, int P, double* __restrict__ out)
+__global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ partials, int nw, int P, double* __restrict__ out,
+                                                     const int* __restrict__ gate)
 {
+  if (gate && *gate == 0) return;
   __shared__ double red[16][17];
   const int pl = threadIdx.x & 15, s = threadIdx.x >> 4;
   const int p = blockIdx.x * 16 + pl;
@@ -395,8 +398,10 @@ __global__ __launch_bounds__(kThreads) void k_scale(S* __restrict__ W, const dou
 
 // W -= sum_{j<k} V_j C_j ; C (k*B x B, FP64) staged in LDS in the panel type.
 template <typename S, int B>
-__global__ __launch_bounds__(kThreads) void k_update(const S* __restrict__ V, size_t ldv, int k, const double* __restrict__ C, S* __restrict__ W, size_t n)
+__global__ __launch_bounds__(kThreads) void k_update(const S* __restrict__ V, size_t ldv, int k, const double* __restrict__ C, S* __restrict__ W, size_t n,
+                                                     const int* __restrict__ gate)
 {
+  if (gate && *gate == 0) return;
   extern __shared__ unsigned char smem_raw[];
   S* Cs = reinterpret_cast<S*>(smem_raw);
   const int P = k * B * B;
@@ -492,7 +497,7 @@ inline int pick_tpw(int per_wave)
 // Queue the Gram partials for blocks [0, k) of V against W; returns the
 // number of partial rows written (nwg * RS) in *nw.
 template <typename S, int B>
-int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* W, const S* md, int* nw)
+int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* W, const S* md, int* nw, const int* gate)
 {
   constexpr int BP = pad_b(B), BPT = 16 / BP;
   const int ntile = (k + BPT - 1) / BPT;
@@ -512,31 +517,31 @@ int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* 
   const dim3 grid{unsigned(g)}, blk{unsigned(kThreads)};
   switch (tpw) {
   case 1:
-    hipLaunchKernelGGL((k_gram<S, B, 1>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 1>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   case 2:
-    hipLaunchKernelGGL((k_gram<S, B, 2>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 2>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   case 3:
-    hipLaunchKernelGGL((k_gram<S, B, 3>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 3>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   case 4:
-    hipLaunchKernelGGL((k_gram<S, B, 4>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 4>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   case 5:
-    hipLaunchKernelGGL((k_gram<S, B, 5>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 5>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   case 6:
-    hipLaunchKernelGGL((k_gram<S, B, 6>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 6>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   case 8:
-    hipLaunchKernelGGL((k_gram<S, B, 8>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 8>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   case 12:
-    hipLaunchKernelGGL((k_gram<S, B, 12>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 12>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   default:
-    hipLaunchKernelGGL((k_gram<S, B, 16>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials);
+    hipLaunchKernelGGL((k_gram<S, B, 16>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
     break;
   }
   TRL_HIP_LAUNCH_CHECK();
@@ -558,7 +563,62 @@ constexpr int gram_max_k()
 
 namespace trl_hip_impl {
 
-int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* md, double* C)
+// Second-pass test of classical Gram-Schmidt with reorthogonalisation
+// (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976)): C holds the
+// first pass's coefficients V_q^T M W (blocks q < k) and W^T M W (block k).
+// Column j needs a second pass when its norm after the first pass,
+// ||w_j||^2 - sum_q ||c_qj||^2, falls below eta2 ||w_j||^2.  gate = 1 if any
+// column does.
+template <int B>
+__global__ void k_dgks(const double* __restrict__ C, int k, double eta2, int* __restrict__ gate)
+{
+  __shared__ int need;
+  if (threadIdx.x == 0) need = 0;
+  __syncthreads();
+  const int j = threadIdx.x;
+  if (j < B) {
+    const double w2 = C[size_t(k) * B * B + j * B + j];
+    double c2 = 0.0;
+    for (int q = 0; q < k; ++q)
+      for (int c = 0; c < B; ++c) {
+        const double x = C[size_t(q) * B * B + c * B + j];
+        c2 = fma(x, x, c2);
+      }
+    if (!(w2 - c2 >= eta2 * w2) || !(w2 > 0.0)) atomicOr(&need, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *gate = need;
+}
+
+int launch_dgks(trl_hip_ctx* ctx, int b, int k, const double* C, double eta2, int* gate)
+{
+  switch (b) {
+  case 1:
+    hipLaunchKernelGGL(k_dgks<1>, dim3(1), dim3(64), 0, ctx->stream, C, k, eta2, gate);
+    break;
+  case 2:
+    hipLaunchKernelGGL(k_dgks<2>, dim3(1), dim3(64), 0, ctx->stream, C, k, eta2, gate);
+    break;
+  case 4:
+    hipLaunchKernelGGL(k_dgks<4>, dim3(1), dim3(64), 0, ctx->stream, C, k, eta2, gate);
+    break;
+  case 6:
+    hipLaunchKernelGGL(k_dgks<6>, dim3(1), dim3(64), 0, ctx->stream, C, k, eta2, gate);
+    break;
+  case 8:
+    hipLaunchKernelGGL(k_dgks<8>, dim3(1), dim3(64), 0, ctx->stream, C, k, eta2, gate);
+    break;
+  case 16:
+    hipLaunchKernelGGL(k_dgks<16>, dim3(1), dim3(64), 0, ctx->stream, C, k, eta2, gate);
+    break;
+  default:
+    return TRL_HIP_EINVAL;
+  }
+  TRL_HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* md, double* C, const int* gate)
 {
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
@@ -566,7 +626,7 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
       const S* w = static_cast<const S*>(W);
       const S* m = static_cast<const S*>(md);
       if (n == 0) {
-        hipLaunchKernelGGL(k_reduce, dim3(unsigned((size_t(k) * B * B + 15) / 16)), dim3(kThreads), 0, ctx->stream, ctx->partials, 0, k * B * B, C);
+        hipLaunchKernelGGL(k_reduce, dim3(unsigned((size_t(k) * B * B + 15) / 16)), dim3(kThreads), 0, ctx->stream, ctx->partials, 0, k * B * B, C, gate);
         TRL_HIP_LAUNCH_CHECK();
         return 0;
       }
@@ -574,10 +634,10 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
       for (int j0 = 0; j0 < k; j0 += kmax) {
         const int kc = (k - j0) < kmax ? (k - j0) : kmax;
         int nw = 0;
-        int rc = queue_gram_partials<S, B>(ctx, n, kc, v + size_t(j0) * n * B, w, m, &nw);
+        int rc = queue_gram_partials<S, B>(ctx, n, kc, v + size_t(j0) * n * B, w, m, &nw, gate);
         if (rc) return rc;
         const int P = kc * B * B;
-        hipLaunchKernelGGL(k_reduce, dim3(unsigned((P + 15) / 16)), dim3(kThreads), 0, ctx->stream, ctx->partials, nw, P, C + size_t(j0) * B * B);
+        hipLaunchKernelGGL(k_reduce, dim3(unsigned((P + 15) / 16)), dim3(kThreads), 0, ctx->stream, ctx->partials, nw, P, C + size_t(j0) * B * B, gate);
         TRL_HIP_LAUNCH_CHECK();
       }
     });
@@ -585,7 +645,7 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
   return 0;
 }
 
-int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W)
+int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W, const int* gate)
 {
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
@@ -597,7 +657,7 @@ int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, con
         const int kc = (k - j0) < kmax ? (k - j0) : kmax;
         const size_t shm = size_t(kc) * B * B * sizeof(S);
         hipLaunchKernelGGL((k_update<S, B>), dim3(unsigned(g)), dim3(kThreads), shm, ctx->stream, static_cast<const S*>(V) + size_t(j0) * n * B, n * B, kc,
-                           C + size_t(j0) * B * B, static_cast<S*>(W), n);
+                           C + size_t(j0) * B * B, static_cast<S*>(W), n, gate);
         TRL_HIP_LAUNCH_CHECK();
       }
     });

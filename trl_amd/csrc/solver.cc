@@ -125,6 +125,10 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
     trl_amd::BlockLanczos<EVP, trl::hip::BlockCGS2> lz(evp, ep);
     run(lz);
   }
+  else if (p.reorth == 2) {
+    trl_amd::BlockLanczos<EVP, trl::hip::BlockCGSAdaptive> lz(evp, ep);
+    run(lz);
+  }
   else {
     trl_amd::BlockLanczos<EVP, trl_amd::ModifiedGS> lz(evp, ep);
     run(lz);
@@ -145,12 +149,13 @@ struct BlockRefiner {
   const double* mass;
   unsigned nb;
   bool ascending;
+  double eta; // second-pass test of the block Gram-Schmidt (<= 0: always two passes)
   trl::hip::DeviceBuffer<double> Xa, Xb, AX, coef;
   double* X; // current Ritz panels (Xa or Xb)
   double* spare;
 
-  BlockRefiner(const Context& ctx, std::size_t n_, const SparseOperator<double>& o, const double* m, unsigned nb_, bool asc)
-      : c(ctx), n(n_), op(o), mass(m), nb(nb_), ascending(asc), Xa(ctx, std::size_t(nb_) * n_ * B),
+  BlockRefiner(const Context& ctx, std::size_t n_, const SparseOperator<double>& o, const double* m, unsigned nb_, bool asc, double eta_)
+      : c(ctx), n(n_), op(o), mass(m), nb(nb_), ascending(asc), eta(eta_), Xa(ctx, std::size_t(nb_) * n_ * B),
         Xb(ctx, std::size_t(nb_) * n_ * B), AX(ctx, std::size_t(nb_) * n_ * B), coef(ctx, std::size_t(nb_) * nb_ * B * B + 2 * B * B)
   {
     X = Xa.data();
@@ -164,11 +169,7 @@ struct BlockRefiner {
     double* C = coef.data();
     double* R = coef.data() + std::size_t(nb) * nb * B * B;
     for (unsigned j = 0; j < nb; ++j) {
-      if (j > 0)
-        for (int pass = 0; pass < 2; ++pass) {
-          throw_on(trl_hip_gram(c.get(), TRL_HIP_F64, n, B, int(j), X, panel(X, j), mass, C), "refine gram");
-          throw_on(trl_hip_update(c.get(), TRL_HIP_F64, n, B, int(j), X, C, panel(X, j)), "refine update");
-        }
+      if (j > 0) throw_on(trl_hip_reorth(c.get(), TRL_HIP_F64, n, B, int(j), X, panel(X, j), mass, C, eta), "refine reorth");
       throw_on(trl_hip_cholqr(c.get(), TRL_HIP_F64, n, B, panel(X, j), mass, R, 2), "refine cholqr");
     }
     c.sync(); // raises a breakdown
@@ -276,7 +277,7 @@ int trl_hip_solve_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t
 {
   if (!ctx || !p || !rowptr || !col || !val || !V0 || !evals || !res) return TRL_HIP_EINVAL;
   if ((mass == nullptr) != (mass_inv == nullptr)) return TRL_HIP_EINVAL;
-  if (p->order < 0 || p->order > 2 || p->reorth < 0 || p->reorth > 1) return TRL_HIP_EINVAL;
+  if (p->order < 0 || p->order > 2 || p->reorth < 0 || p->reorth > 2) return TRL_HIP_EINVAL;
   *res = trl_hip_solver_result{};
   try {
     const Context c = Context::borrow(ctx);
@@ -320,6 +321,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
   *res = trl_hip_solver_result{};
   trl_hip_solver_result r32{};
   if (mp->refine == 1 && p->order == 0) return TRL_HIP_EINVAL;
+  if (p->order < 0 || p->order > 2 || p->reorth < 0 || p->reorth > 2) return TRL_HIP_EINVAL;
   try {
     const Context c = Context::borrow(ctx);
     const int rc = dispatch_b(p->b, [&](auto Bc) {
@@ -358,7 +360,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       if (mp->refine == 1) {
         // ---- FP64 phase: block refinement of the FP32 Ritz blocks -----------
         const bool asc = (p->order == 1);
-        BlockRefiner<B> rf(c, n, *op64, mass64, nref, asc);
+        BlockRefiner<B> rf(c, n, *op64, mass64, nref, asc, p->reorth == 2 ? 0.70710678118654752 : 0.0);
         if (mp->phase1_fp64)
           throw_on(trl_hip_memcpy_d2d(c.get(), rf.X, X1.data(), std::size_t(nref) * n * B * sizeof(double)), "copy X1");
         else

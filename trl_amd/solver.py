@@ -18,7 +18,7 @@ from . import _lib as L
 from .problems import CSR
 
 ORDER = {"largest_magnitude": 0, "ascending": 1, "descending": 2}
-REORTH = {"mgs": 0, "cgs2": 1}
+REORTH = {"mgs": 0, "cgs2": 1, "dgks": 2}
 
 
 @dataclass
