@@ -71,6 +71,16 @@ public:
     throw_on(trl_hip_spmm_axpby(ctx_.get(), dtype, n_, b, &op_, X, Y, Z, alpha, beta, gamma), "apply_axpby");
   }
 
+  // The same on npanels consecutive panels (panel j at j*n*b).
+  void apply_panels(int b, int npanels, const S* X, S* Y) const
+  {
+    throw_on(trl_hip_spmm_panels(ctx_.get(), dtype, n_, b, &op_, npanels, X, Y), "apply_panels");
+  }
+  void axpby_panels(int b, int npanels, const S* X, S* Y, const S* Z, double alpha, double beta, double gamma) const
+  {
+    throw_on(trl_hip_spmm_axpby_panels(ctx_.get(), dtype, n_, b, &op_, npanels, X, Y, Z, alpha, beta, gamma), "apply_axpby_panels");
+  }
+
   const trl_hip_spmat& descriptor() const { return op_; }
   std::size_t size() const { return n_; }
   int tuned_block_size() const { return tuned_b_; }

@@ -170,6 +170,14 @@ int trl_hip_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_h
 int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, const void* X, void* Y, const void* Z, double alpha,
                        double beta, double gamma);
 
+/* The same two products on npanels panels at once (panel j of X, Y, Z at
+ * j*n*b elements): the matrix entries of a row range are fetched once for
+ * all panels (the workgroups of one XCD take every panel of a row range in
+ * turn).  For the Chebyshev filter of a block of Ritz vectors. */
+int trl_hip_spmm_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y);
+int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y, const void* Z,
+                              double alpha, double beta, double gamma);
+
 /* CSR convenience forms of the two calls above (variant as in trl_hip_spmat). */
 int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
                      const void* dscale, int variant);

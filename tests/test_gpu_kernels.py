@@ -134,6 +134,43 @@ def test_spmm_ell_variants(ctx, rng, p, b, variant):
 
 
 @pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", [1, 4, 8])
+@pytest.mark.parametrize("fmt,variant", [(0, 0), (0, 1), (0, 4), (1, 1), (1, -2)])
+def test_spmm_panels(ctx, rng, p, b, fmt, variant):
+    # npanels panels in one launch == one launch per panel (bitwise), and
+    # the fused form with Z == Y (in-place older iterate)
+    if variant < 0 and not split_ok(p, b, -variant):
+        pytest.skip("column split needs 8 bytes per lane")
+    A = P.laplacian_3d(9, 7, 6, rng.uniform(0, 1, 378))
+    dt = DT[p]
+    npan = 3
+    X = rng.standard_normal((npan, A.n, b)).astype(dt)
+    Z = rng.standard_normal((npan, A.n, b)).astype(dt)
+    d = rng.uniform(0.5, 2, A.n).astype(dt)
+    rp, ci, va = dev(ctx, A.rowptr, np.int32), dev(ctx, A.col, np.int32), dev(ctx, A.val, dt)
+    width, hcol, hval = csr_to_ell(A)
+    ec, ev, dd = dev(ctx, hcol, np.int32), dev(ctx, hval, dt), dev(ctx, d, dt)
+    M = spmat(0, rp, 0, ci, va, dd, variant) if fmt == 0 else spmat(1, None, width, ec, ev, dd, variant)
+    Xd = dev(ctx, X, dt)
+    Y = ctx.empty((npan, A.n, b), dt)
+    L.check(H().trl_hip_spmm_panels(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), npan, Xd.ptr, Y.ptr), "panels")
+    Y1 = ctx.empty((A.n, b), dt)
+    got = Y.download()
+    for j in range(npan):
+        L.check(H().trl_hip_spmm(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), Xd.offset(j * A.n * b), Y1.ptr), "one")
+        assert np.array_equal(got[j], Y1.download())
+        assert rel(got[j], oracle.spmm(A, X[j].astype(np.float64), d.astype(np.float64))) < TOL[p]
+    Zd = dev(ctx, Z, dt)
+    L.check(H().trl_hip_spmm_axpby_panels(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), npan, Xd.ptr, Zd.ptr, Zd.ptr, 0.5, -1.0, 0.25), "axpby panels")
+    got = Zd.download()
+    for j in range(npan):
+        ref = 0.5 * oracle.spmm(A, X[j].astype(np.float64), d.astype(np.float64)) - X[j] + 0.25 * Z[j]
+        assert rel(got[j], ref) < TOL[p]
+    # X overlapping Y is refused
+    assert H().trl_hip_spmm_axpby_panels(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), npan, Xd.ptr, Xd.offset(A.n * b), None, 1.0, 0.0, 0.0) == -1
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
 @pytest.mark.parametrize("b", [1, 8])
 def test_spmm_tune(ctx, rng, p, b):
     # the tuner returns a descriptor that computes the same product

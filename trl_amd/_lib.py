@@ -60,6 +60,9 @@ HIP_API = [
     ("trl_hip_rotate", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("trl_hip_spmm", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_void_p, c_void_p]),
     ("trl_hip_spmm_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_void_p, c_void_p, c_void_p, C.c_double, C.c_double, C.c_double]),
+    ("trl_hip_spmm_panels", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_int, c_void_p, c_void_p]),
+    ("trl_hip_spmm_axpby_panels", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_int, c_void_p, c_void_p, c_void_p, C.c_double, C.c_double,
+                                          C.c_double]),
     ("trl_hip_csr_max_row", c_int, [c_void_p, c_size, c_void_p, c_int_p]),
     ("trl_hip_csr_to_ell", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_spmm_tune", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_spmat_p, c_spmat_p, c_double_p]),

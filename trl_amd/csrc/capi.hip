@@ -407,6 +407,29 @@ int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const
   return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant);
 }
 
+int trl_hip_spmm_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y)
+{
+  if (!ctx || !valid_b(b) || npanels < 1 || check_spmat(n, A) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + 2.0 * npanels * panel_bytes(t, n, b) : 0.0);
+  return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, nullptr, false, 0, 0, 0, A->variant,
+                     npanels);
+}
+
+int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y, const void* Z,
+                              double alpha, double beta, double gamma)
+{
+  if (!ctx || !valid_b(b) || npanels < 1 || check_spmat(n, A) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
+  const size_t es = t == TRL_HIP_F32 ? 4 : 8;
+  const char *x = static_cast<const char*>(X), *y = static_cast<const char*>(Y);
+  const size_t span = size_t(npanels) * n * b * es;
+  if (x < y + span && y < x + span) return TRL_HIP_EINVAL; // X and Y must not overlap
+  if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + (Z ? 4.0 : 3.0) * npanels * panel_bytes(t, n, b) : 0.0);
+  return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant,
+                     npanels);
+}
+
 int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
                      const void* dscale, int variant)
 {

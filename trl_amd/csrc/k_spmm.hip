@@ -38,7 +38,17 @@ struct Epi {
   const void* dscale;
   const void* Z;
   double alpha, beta, gamma;
+  unsigned npan;  // panels of X, Y, Z processed by one launch
+  size_t pstride; // elements between consecutive panels
 };
+
+// Workgroup -> (row range, panel): consecutive workgroups in an XCD's
+// contiguous range take every panel of one row range before moving on, so
+// the matrix entries of those rows are read from HBM once and then hit L2.
+struct PanelMap {
+  unsigned pan, wrow;
+};
+__device__ inline PanelMap panel_map(unsigned blk, unsigned npan) { return PanelMap{blk % npan, blk / npan}; }
 
 // AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z
 template <typename S, int BC, bool AXPBY>
@@ -72,8 +82,11 @@ __global__ __launch_bounds__(kThreads) void k_spmm_rows(const int32_t* __restric
                                                         const S* __restrict__ val, const S* __restrict__ X, S* Y, Epi e, int n)
 {
   constexpr int BC = B / CS;
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  const int gid = int(blk * blockDim.x + threadIdx.x);
+  const PanelMap pm = panel_map(xcd_block(blockIdx.x, gridDim.x), e.npan);
+  const int gid = int(pm.wrow * blockDim.x + threadIdx.x);
+  const size_t po = size_t(pm.pan) * e.pstride;
+  X += po;
+  Y += po;
   const int row = gid / CS;
   const int off = (gid % CS) * BC;
   if (row >= n) return;
@@ -100,15 +113,20 @@ __global__ __launch_bounds__(kThreads) void k_spmm_rows(const int32_t* __restric
 #pragma unroll
       for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
   }
-  epilogue<S, BC, AXPBY>(acc, row, B, off, X, Y, e);
+  Epi ee = e;
+  if (ee.Z) ee.Z = static_cast<const S*>(ee.Z) + po;
+  epilogue<S, BC, AXPBY>(acc, row, B, off, X, Y, ee);
 }
 
 template <typename S, int B, int L, bool AXPBY>
 __global__ __launch_bounds__(kThreads) void k_spmm_lanes(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
                                                          const S* __restrict__ X, S* Y, Epi e, int n)
 {
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  const int gid = int(blk * blockDim.x + threadIdx.x);
+  const PanelMap pm = panel_map(xcd_block(blockIdx.x, gridDim.x), e.npan);
+  const int gid = int(pm.wrow * blockDim.x + threadIdx.x);
+  const size_t po = size_t(pm.pan) * e.pstride;
+  X += po;
+  Y += po;
   const int row = gid / L;
   const int sub = int(threadIdx.x) % L;
   if (row >= n) return; // whole lane groups leave together
@@ -129,7 +147,9 @@ __global__ __launch_bounds__(kThreads) void k_spmm_lanes(const int32_t* __restri
 #pragma unroll
     for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
   if (sub != 0) return;
-  epilogue<S, B, AXPBY>(acc, row, B, 0, X, Y, e);
+  Epi ee = e;
+  if (ee.Z) ee.Z = static_cast<const S*>(ee.Z) + po;
+  epilogue<S, B, AXPBY>(acc, row, B, 0, X, Y, ee);
 }
 
 inline unsigned xcd_grid(size_t threads)
@@ -147,7 +167,7 @@ int launch_rows(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, 
   const bool ell = rowptr == nullptr;
   const int cs = variant < 0 ? -variant : 1;
   const size_t lanes = cs > 1 ? size_t(cs) : size_t(variant > 1 ? variant : 1);
-  const dim3 g{xcd_grid(n * lanes)}, blk{unsigned(kThreads)};
+  const dim3 g{xcd_grid(n * lanes) * e.npan}, blk{unsigned(kThreads)};
   if (variant > 1) {
     if (ell) return TRL_HIP_EINVAL;
     switch (variant) {
@@ -260,9 +280,11 @@ int launch_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t
 bool valid_spmm_variant(int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == -2 || v == -4; }
 
 int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, int width, const int32_t* col, const void* val, const void* dscale,
-                const void* X, void* Y, const void* Z, bool axpby, double alpha, double beta, double gamma, int variant)
+                const void* X, void* Y, const void* Z, bool axpby, double alpha, double beta, double gamma, int variant, int npanels)
 {
-  const Epi e{dscale, Z, alpha, beta, gamma};
+  if (npanels < 1 || size_t(npanels) * ((n * size_t(variant < 0 ? -variant : (variant > 1 ? variant : 1)) + 255) / 256 + 8) > size_t(UINT32_MAX))
+    return TRL_HIP_EINVAL;
+  const Epi e{dscale, Z, alpha, beta, gamma, unsigned(npanels), n * size_t(b)};
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
       const int v = variant == 0 ? default_variant<S, B>() : variant;

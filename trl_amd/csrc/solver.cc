@@ -179,8 +179,7 @@ struct BlockRefiner {
   void rayleigh_ritz(std::vector<double>& theta)
   {
     const unsigned nt = nb * B;
-    for (unsigned j = 0; j < nb; ++j)
-      op.apply(int(B), panel(X, j), panel(AX.data(), j));
+    for (unsigned j = 0; j < nb; ++j) op.apply(int(B), panel(X, j), panel(AX.data(), j));
     double* C = coef.data();
     for (unsigned j = 0; j < nb; ++j)
       throw_on(trl_hip_gram(c.get(), TRL_HIP_F64, n, B, int(nb), X, panel(AX.data(), j), mass, C + std::size_t(j) * nt * B), "refine projection");
@@ -218,6 +217,9 @@ struct BlockRefiner {
     if (degree == 0) return;
     const double e = 0.5 * (hi - lo), cc = 0.5 * (hi + lo);
     const double s1 = e / (a0 - cc);
+    // panel by panel: each panel's X stays cache-resident across the stencil
+    // neighbours of a row range (all panels at once would multiply that
+    // working set by nb)
     for (unsigned j = 0; j < nb; ++j) {
       double* prev = panel(X, j);
       double* cur = panel(spare, j);
