@@ -403,7 +403,8 @@ int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const
 {
   if (!ctx || !valid_b(b) || check_spmat(n, A) || (n && (!X || !Y)) || X == Y) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, Z ? 4 : 3) : 0.0);
+  // algorithmic panels: X (gathered rows and the beta term read it once), Z, Y
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, Z ? 3 : 2) : 0.0);
   return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant);
 }
 
@@ -425,7 +426,7 @@ int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b
   const size_t span = size_t(npanels) * n * b * es;
   if (x < y + span && y < x + span) return TRL_HIP_EINVAL; // X and Y must not overlap
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + (Z ? 4.0 : 3.0) * npanels * panel_bytes(t, n, b) : 0.0);
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + (Z ? 3.0 : 2.0) * npanels * panel_bytes(t, n, b) : 0.0);
   return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant,
                      npanels);
 }

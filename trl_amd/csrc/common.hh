@@ -134,8 +134,28 @@ inline int hip_status(hipError_t e) { return e == hipSuccess ? 0 : static_cast<i
 // device helpers
 namespace trl_hip_dev {
 
+// Plain or non-temporal (streaming: read or written once, kept out of the
+// way of reused data in the caches) element access.
+template <bool NT, typename T>
+__device__ inline T ld(const T* p)
+{
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+template <bool NT, typename T>
+__device__ inline void st(T* p, const T& v)
+{
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // Loads/stores of one panel row (B contiguous scalars) with the widest
-// aligned vector the row allows (rows start at r*B*sizeof(S)).
+// aligned vector the row allows (rows start at r*B*sizeof(S)); NT selects
+// non-temporal accesses.
 template <typename S, int B>
 struct RowIO {
   static constexpr int bytes = B * int(sizeof(S));
@@ -144,6 +164,7 @@ struct RowIO {
   static constexpr int nv = B / per;
   using V16 = __attribute__((ext_vector_type(16 / sizeof(S)))) S;
 
+  template <bool NT = false>
   __device__ static inline void load(const S* __restrict__ p, S (&v)[B])
   {
     if constexpr (vec == 16) {
@@ -151,7 +172,7 @@ struct RowIO {
       const VT* q = reinterpret_cast<const VT*>(p);
 #pragma unroll
       for (int i = 0; i < nv; ++i) {
-        VT t = q[i];
+        VT t = ld<NT>(q + i);
 #pragma unroll
         for (int e = 0; e < per; ++e) v[i * per + e] = t[e];
       }
@@ -161,17 +182,18 @@ struct RowIO {
       const VT* q = reinterpret_cast<const VT*>(p);
 #pragma unroll
       for (int i = 0; i < nv; ++i) {
-        VT t = q[i];
+        VT t = ld<NT>(q + i);
         v[2 * i] = t[0];
         v[2 * i + 1] = t[1];
       }
     }
     else {
 #pragma unroll
-      for (int i = 0; i < B; ++i) v[i] = p[i];
+      for (int i = 0; i < B; ++i) v[i] = ld<NT>(p + i);
     }
   }
 
+  template <bool NT = false>
   __device__ static inline void store(S* __restrict__ p, const S (&v)[B])
   {
     if constexpr (vec == 16) {
@@ -182,7 +204,7 @@ struct RowIO {
         VT t;
 #pragma unroll
         for (int e = 0; e < per; ++e) t[e] = v[i * per + e];
-        q[i] = t;
+        st<NT>(q + i, t);
       }
     }
     else if constexpr (vec == 8 && per == 2) {
@@ -193,12 +215,12 @@ struct RowIO {
         VT t;
         t[0] = v[2 * i];
         t[1] = v[2 * i + 1];
-        q[i] = t;
+        st<NT>(q + i, t);
       }
     }
     else {
 #pragma unroll
-      for (int i = 0; i < B; ++i) p[i] = v[i];
+      for (int i = 0; i < B; ++i) st<NT>(p + i, v[i]);
     }
   }
 };

@@ -16,6 +16,10 @@
 //
 // Workgroups are renumbered so that each XCD walks a contiguous range of rows:
 // the stencil neighbours of a row (r +- 1, r +- nx, ...) then share that XCD's L2.
+// The ELL matrix stream (indices, values; every cache line used by one load),
+// the row scaling, Z and Y are touched once and use non-temporal accesses,
+// which leaves the L2 to the gathered rows of X.  CSR index and value lines
+// are shared by consecutive loads of neighbouring lanes and stay cached.
 #include <algorithm>
 
 #include "common.hh"
@@ -57,7 +61,7 @@ __device__ inline void epilogue(const S (&acc)[BC], int row, int B, int off, con
   const S* ds = static_cast<const S*>(e.dscale);
   S out[BC];
   if constexpr (!AXPBY) {
-    const S d = ds ? ds[row] : S(1);
+    const S d = ds ? ld<true>(ds + row) : S(1);
 #pragma unroll
     for (int q = 0; q < BC; ++q) out[q] = ds ? acc[q] * d : acc[q];
   }
@@ -65,8 +69,8 @@ __device__ inline void epilogue(const S (&acc)[BC], int row, int B, int off, con
     const S* Z = static_cast<const S*>(e.Z);
     S xr[BC], zr[BC];
     RowIO<S, BC>::load(X + size_t(row) * B + off, xr);
-    if (Z) RowIO<S, BC>::load(Z + size_t(row) * B + off, zr);
-    const double a = ds ? e.alpha * double(ds[row]) : e.alpha;
+    if (Z) RowIO<S, BC>::template load<true>(Z + size_t(row) * B + off, zr);
+    const double a = ds ? e.alpha * double(ld<true>(ds + row)) : e.alpha;
 #pragma unroll
     for (int q = 0; q < BC; ++q) {
       double v = a * double(acc[q]) + e.beta * double(xr[q]);
@@ -74,7 +78,7 @@ __device__ inline void epilogue(const S (&acc)[BC], int row, int B, int off, con
       out[q] = S(v);
     }
   }
-  RowIO<S, BC>::store(Y + size_t(row) * B + off, out);
+  RowIO<S, BC>::template store<true>(Y + size_t(row) * B + off, out);
 }
 
 template <typename S, int B, int CS, bool ELL, bool AXPBY>
@@ -102,8 +106,8 @@ __global__ __launch_bounds__(kThreads) void k_spmm_rows(const int32_t* __restric
     for (int q = 0; q < kChunk; ++q) {
       const bool ok = p0 + q < end;
       const size_t p = ELL ? size_t(p0 + q) * n + row : size_t(p0 + q);
-      ci[q] = ok ? col[p] : row;
-      vi[q] = ok ? val[p] : S(0);
+      ci[q] = ok ? ld<ELL>(col + p) : row; // slot-major ELL lines are used once: stream them
+      vi[q] = ok ? ld<ELL>(val + p) : S(0);
     }
     S x[kChunk][BC];
 #pragma unroll

@@ -1,0 +1,117 @@
+"""Kernel driver for profiling: runs single hot kernels of the 3D-Laplacian
+workload a fixed number of times, so that a profiler (rocprofv3 kernel trace
+or hardware counters) sees clean, repeated dispatches of one kernel.
+
+    python -m trl_amd.kernel_probe --side 160 --b 8 --reps 20 --what spmm_axpby64,copy
+
+Kernels (--what, comma separated):
+  spmm32, spmm64        Y = A X with the autotuned operator (FP32 / FP64)
+  spmm_axpby64          Y = alpha A X + beta X + gamma Z (the FP64 Chebyshev step)
+  gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
+  cholqr32              CholQR2 of one FP32 panel
+  copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
+Prints, per kernel, the mean time per launch (HIP events) and the algorithmic
+bytes per launch.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import sys
+
+import numpy as np
+
+from . import _lib as L
+from . import problems as P
+from .solver import DeviceCSR
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--side", type=int, default=160)
+    ap.add_argument("--b", type=int, default=8)
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby64,gram32,update32,cholqr32,copy")
+    a = ap.parse_args(argv)
+    what = a.what.split(",")
+    cfg = P.config_c3(a.side)
+    A = cfg.csr
+    n, b = A.n, a.b
+    ctx = L.Context(0)
+    H = L.hip()
+    dA = DeviceCSR(ctx, A)
+    rng = np.random.default_rng(0)
+
+    def timed(name, fn, bytes_per_launch, family):
+        for _ in range(2):
+            fn()
+        ctx.sync()
+        ctx.prof_enable(True)
+        for _ in range(a.reps):
+            fn()
+        st = ctx.prof_read()[family]
+        ctx.prof_enable(False)
+        avg = st["seconds"] / st["calls"]
+        print(f"{name:14s} launches {a.reps:4d}  mean {avg * 1e6:9.1f} us  algorithmic {bytes_per_launch / 1e6:9.1f} MB  "
+              f"{bytes_per_launch / avg / 1e9:7.0f} GB/s", flush=True)
+
+    for w in what:
+        if w in ("spmm32", "spmm64", "spmm_axpby64"):
+            p = "f32" if w == "spmm32" else "f64"
+            dt = np.float32 if p == "f32" else np.float64
+            op = dA.operator(b, p, tune=True)
+            s = np.dtype(dt).itemsize
+            X = ctx.to_device(rng.standard_normal((n, b)).astype(dt))
+            Y = ctx.empty((n, b), dt)
+            mat = (op.desc.width * n * (4 + s)) if op.desc.format == 1 else (A.nnz * (4 + s) + (n + 1) * 4)
+            if w == "spmm_axpby64":
+                Z = ctx.to_device(rng.standard_normal((n, b)))
+
+                def fn(op=op, X=X, Y=Y, Z=Z):
+                    L.check(H.trl_hip_spmm_axpby(ctx.h, L.F64, n, b, C.byref(op.desc), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125), "axpby")
+
+                timed(f"{w}[{op.name}]", fn, mat + 3 * n * b * s, "spmm")
+            else:
+                timed(f"{w}[{op.name}]", lambda op=op, X=X, Y=Y: op.apply(X, Y), mat + 2 * n * b * s, "spmm")
+        elif w in ("gram32", "update32"):
+            k = 16
+            V = ctx.to_device(rng.standard_normal((k + 1, n, b)).astype(np.float32))
+            Cd = ctx.empty((k * b * b,))
+            W = V.offset(k * n * b)
+            P_ = n * b * 4
+            L.check(H.trl_hip_gram(ctx.h, L.F32, n, b, k, V.ptr, W, None, Cd.ptr), "gram")
+            if w == "gram32":
+                timed(w, lambda: L.check(H.trl_hip_gram(ctx.h, L.F32, n, b, k, V.ptr, W, None, Cd.ptr), "gram"), (k + 1) * P_, "gram")
+            else:
+                Cs = ctx.to_device(np.zeros(k * b * b))  # W unchanged: W -= V 0
+                timed(w, lambda: L.check(H.trl_hip_update(ctx.h, L.F32, n, b, k, V.ptr, Cs.ptr, W), "update"), (k + 2) * P_, "update")
+        elif w == "cholqr32":
+            Wd = ctx.to_device(rng.standard_normal((n, b)).astype(np.float32))
+            R = ctx.empty((b * b,))
+            timed(w, lambda: L.check(H.trl_hip_cholqr(ctx.h, L.F32, n, b, Wd.ptr, None, R.ptr, 2), "cholqr"), 5 * n * b * 4, "cholqr")
+        elif w == "copy":
+            cnt = 64 << 20  # 64 Mi doubles = 512 MiB each way
+            src = ctx.to_device(np.ones(cnt))
+            dst = ctx.empty((cnt,))
+            for _ in range(2):
+                L.check(H.trl_hip_convert(ctx.h, L.F64, src.ptr, L.F64, dst.ptr, cnt), "copy")
+            ctx.sync()
+            import time
+
+            t = time.perf_counter()
+            for _ in range(a.reps):
+                L.check(H.trl_hip_convert(ctx.h, L.F64, src.ptr, L.F64, dst.ptr, cnt), "copy")
+            ctx.sync()
+            avg = (time.perf_counter() - t) / a.reps
+            print(f"{'copy':14s} launches {a.reps:4d}  mean {avg * 1e6:9.1f} us  algorithmic {2 * cnt * 8 / 1e6:9.1f} MB  "
+                  f"{2 * cnt * 8 / avg / 1e9:7.0f} GB/s (wall clock, includes launch gaps)", flush=True)
+        else:
+            print(f"unknown kernel {w}", file=sys.stderr)
+            return 2
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
