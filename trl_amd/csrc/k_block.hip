@@ -10,6 +10,9 @@
 //   FP32: i = 4*q + e   (four consecutive rows per lane)
 //   FP64: i = 4*e + q   (rows strided by four)
 // (the f32/f64 output groupings of the 16x16x4 instructions differ).
+#include <cstdlib>
+#include <string>
+
 #include "common.hh"
 
 using namespace trl_hip_dev;
@@ -137,8 +140,9 @@ __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size
     }
   }
 
-  const size_t P = size_t(k) * B * B;
-  double* out = partials + (size_t(blockIdx.x) * RS + rs) * P;
+  // entry-major partials: entry idx of (workgroup, row subset) w at idx*NW + w
+  const size_t NW = size_t(gridDim.x) * RS;
+  double* out = partials + size_t(blockIdx.x) * RS + rs;
   if (!wval) return;
 #pragma unroll
   for (int q = 0; q < TPW; ++q) {
@@ -151,37 +155,110 @@ __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size
       if (blk < k && ci < B) {
         double v = static_cast<double>(acc[q][e]);
         if constexpr (sizeof(S) == 4) v += static_cast<double>(lo[q][e]);
-        out[(size_t(blk) * B + ci) * B + i16] = v;
+        out[((size_t(blk) * B + ci) * B + i16) * NW] = v;
       }
     }
   }
 }
 
-// out[p] = sum_{w < nw} partials[w*P + p], summed in fixed order (deterministic):
-// 16 entries per workgroup, 16 lanes striding over w, combined in fixed order.
+// out[p] = sum_{w < nw} partials[p*nw + w] (entry-major), summed in a fixed
+// order (deterministic): one workgroup per entry, each lane strides over w
+// with four accumulators, then a butterfly per wave and the waves in order.
 __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ partials, int nw, int P, double* __restrict__ out,
                                                      const int* __restrict__ gate)
 {
   if (gate && *gate == 0) return;
-  __shared__ double red[16][17];
-  const int pl = threadIdx.x & 15, s = threadIdx.x >> 4;
-  const int p = blockIdx.x * 16 + pl;
-  double acc = 0.0, acc1 = 0.0;
-  if (p < P) {
-    int w = s;
-    for (; w + 16 < nw; w += 32) {
-      acc += partials[size_t(w) * P + p];
-      acc1 += partials[size_t(w + 16) * P + p];
-    }
-    if (w < nw) acc += partials[size_t(w) * P + p];
+  __shared__ double red[kThreads / 64];
+  const int p = blockIdx.x;
+  if (p >= P) return;
+  const double* src = partials + size_t(p) * nw;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int w = threadIdx.x;
+  for (; w + 3 * kThreads < nw; w += 4 * kThreads) {
+    a0 += src[w];
+    a1 += src[w + kThreads];
+    a2 += src[w + 2 * kThreads];
+    a3 += src[w + 3 * kThreads];
   }
-  red[s][pl] = acc + acc1;
+  for (; w < nw; w += kThreads) a0 += src[w];
+  double v = (a0 + a1) + (a2 + a3);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (s == 0 && p < P) {
+  if (threadIdx.x == 0) {
     double t = 0.0;
-    for (int q = 0; q < 16; ++q) t += red[q][pl];
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q];
     out[p] = t;
   }
+}
+
+// The same factorisation for a compile-time B with every array in
+// registers (fully unrolled): the one-thread runtime-b version below is
+// bound by LDS latency (tens of microseconds at b = 8).
+template <int B>
+__device__ void chol_reg(const double* G, const double* Rprev, double* Rout, double* Rinv, int* flags)
+{
+  double L[B][B], X[B][B], g[B][B];
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      L[i][j] = X[i][j] = 0.0;
+      g[i][j] = G[i * B + j];
+    }
+  int fail = 0;
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    double d = g[j][j];
+#pragma unroll
+    for (int p = 0; p < j; ++p) d -= L[p][j] * L[p][j];
+    if (!(d > 0.0) && !fail) fail = j + 1;
+    const double rjj = sqrt(d > 0.0 ? d : 1.0);
+    L[j][j] = rjj;
+#pragma unroll
+    for (int i = j + 1; i < B; ++i) {
+      double t = g[i][j];
+#pragma unroll
+      for (int p = 0; p < j; ++p) t -= L[p][j] * L[p][i];
+      L[j][i] = t / rjj;
+    }
+  }
+  if (fail) {
+    if (flags[0] == 0) flags[0] = fail;
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) Rout[i * B + j] = Rinv[i * B + j] = (i == j) ? 1.0 : 0.0;
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    X[j][j] = 1.0 / L[j][j];
+#pragma unroll
+    for (int i = j - 1; i >= 0; --i) {
+      double t = 0.0;
+#pragma unroll
+      for (int p = i + 1; p <= j; ++p) t += L[i][p] * X[p][j];
+      X[i][j] = -t / L[i][i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      Rinv[i * B + j] = X[i][j];
+      double v = 0.0;
+      if (j >= i) {
+        if (Rprev) {
+#pragma unroll
+          for (int p = i; p <= j; ++p) v += L[i][p] * Rprev[p * B + j];
+        }
+        else
+          v = L[i][j];
+      }
+      Rout[i * B + j] = v;
+    }
 }
 
 // Factor G = R^T R (G symmetric b x b in LDS, lower triangle read as an LLT
@@ -336,35 +413,41 @@ __global__ __launch_bounds__(kThreads) void k_cholqr_pass(S* __restrict__ W, con
   __syncthreads();
   if (ticket != G - 1) return;
   __threadfence();
-  // last workgroup: one wave per entry, lanes stride over the workgroups,
-  // then a butterfly (fixed order: deterministic for a given grid)
-  for (int i = wave; i < NP; i += kThreads / 64) {
-    const double* src = partials + size_t(i) * G;
-    double t0 = 0.0, t1 = 0.0;
-    unsigned w = lane;
-    for (; w + 64 < G; w += 128) {
-      t0 += src[w];
-      t1 += src[w + 64];
-    }
-    if (w < G) t0 += src[w];
-    double t = t0 + t1;
+  // last workgroup: every thread sums the workgroups w = t, t + 256, ... of
+  // all entries (independent loads), then the same butterfly and wave order
+  // as above (fixed order: deterministic for a given grid)
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-    if (lane == 0) {
-      // unpack the upper-triangle index i into (r, c) and mirror
-      int r = 0, rem = i;
-      while (rem >= B - r) {
-        rem -= B - r;
-        ++r;
-      }
-      const int c = r + rem;
-      Gs[r * B + c] = t;
-      Gs[c * B + r] = t;
+  for (int i = 0; i < NP; ++i) g[i] = 0.0;
+  for (unsigned w = threadIdx.x; w < G; w += kThreads)
+#pragma unroll
+    for (int i = 0; i < NP; ++i) g[i] += partials[size_t(i) * G + w];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    double v = g[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
+    double t = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q][i];
+    // unpack the upper-triangle index i into (r, c) and mirror
+    int r = 0, rem = i;
+    while (rem >= B - r) {
+      rem -= B - r;
+      ++r;
     }
+    const int c = r + rem;
+    Gs[r * B + c] = t;
+    Gs[c * B + r] = t;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    chol_bxb(Gs, B, Rprev, Rout, Rinv_out, flags, Rl, Xl);
+    if constexpr (B <= 8)
+      chol_reg<B>(Gs, Rprev, Rout, Rinv_out, flags);
+    else
+      chol_bxb(Gs, B, Rprev, Rout, Rinv_out, flags, Rl, Xl);
     *counter = 0u;
   }
 }
@@ -626,7 +709,7 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
       const S* w = static_cast<const S*>(W);
       const S* m = static_cast<const S*>(md);
       if (n == 0) {
-        hipLaunchKernelGGL(k_reduce, dim3(unsigned((size_t(k) * B * B + 15) / 16)), dim3(kThreads), 0, ctx->stream, ctx->partials, 0, k * B * B, C, gate);
+        hipLaunchKernelGGL(k_reduce, dim3(unsigned(size_t(k) * B * B)), dim3(kThreads), 0, ctx->stream, ctx->partials, 0, k * B * B, C, gate);
         TRL_HIP_LAUNCH_CHECK();
         return 0;
       }
@@ -637,7 +720,7 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
         int rc = queue_gram_partials<S, B>(ctx, n, kc, v + size_t(j0) * n * B, w, m, &nw, gate);
         if (rc) return rc;
         const int P = kc * B * B;
-        hipLaunchKernelGGL(k_reduce, dim3(unsigned((P + 15) / 16)), dim3(kThreads), 0, ctx->stream, ctx->partials, nw, P, C + size_t(j0) * B * B, gate);
+        hipLaunchKernelGGL(k_reduce, dim3(unsigned(P)), dim3(kThreads), 0, ctx->stream, ctx->partials, nw, P, C + size_t(j0) * B * B, gate);
         TRL_HIP_LAUNCH_CHECK();
       }
     });
@@ -665,8 +748,61 @@ int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, con
   return 0;
 }
 
+// Cholesky of a reduced Gram matrix G (B x B, global) by one thread.
+template <int B>
+__global__ void k_chol_small(const double* __restrict__ G, const double* __restrict__ Rprev, double* __restrict__ Rout, double* __restrict__ Rinv,
+                             int* __restrict__ flags)
+{
+  __shared__ double Gs[B * B];
+  __shared__ double Rl[16][16], Xl[16][16];
+  for (int i = threadIdx.x; i < B * B; i += blockDim.x) Gs[i] = G[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if constexpr (B <= 8)
+      chol_reg<B>(Gs, Rprev, Rout, Rinv, flags);
+    else
+      chol_bxb(Gs, B, Rprev, Rout, Rinv, flags, Rl, Xl);
+  }
+}
+
+// Cholesky-QR with the Gram matrix on MFMA (the block-product kernel on W
+// against itself), a one-thread FP64 Cholesky and the FP64 scale; one panel
+// read for the Gram and one read + write for the scale per pass.
+int launch_cholqr_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes)
+{
+  double* R1 = ctx->small;
+  double* Rinv = ctx->small + 256;
+  double* G = ctx->small + 768;
+  for (int pass = 0; pass < passes; ++pass) {
+    int rc = launch_gram(ctx, t, n, b, 1, W, W, md, G);
+    if (rc) return rc;
+    const bool last = pass == passes - 1;
+    if (n == 0 && pass > 0) continue;
+    TRL_DISPATCH_T(t, {
+      TRL_DISPATCH_B(b, {
+        hipLaunchKernelGGL((k_chol_small<B>), dim3(1), dim3(64), 0, ctx->stream, G, pass == 0 ? nullptr : R1, last ? R : R1, Rinv, ctx->flags);
+        TRL_HIP_LAUNCH_CHECK();
+        if (n == 0) return 0;
+        size_t gs = (n + kThreads - 1) / kThreads;
+        const size_t gsmax = size_t(8) * ctx->num_cus;
+        if (gs > gsmax) gs = gsmax;
+        hipLaunchKernelGGL((k_scale<S, B>), dim3(unsigned(gs)), dim3(kThreads), 0, ctx->stream, static_cast<S*>(W), Rinv, n);
+        TRL_HIP_LAUNCH_CHECK();
+      });
+    });
+  }
+  return 0;
+}
+
 int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes)
 {
+  // default: the Gram on MFMA (launch_cholqr_gram), measured faster than the
+  // fused thread-per-row passes below; TRL_HIP_CHOLQR=fused selects those
+  static const bool fused = [] {
+    const char* e = std::getenv("TRL_HIP_CHOLQR");
+    return e && std::string(e) == "fused";
+  }();
+  if (!fused) return launch_cholqr_gram(ctx, t, n, b, W, md, R, passes);
   // scratch: [0,256) R1, [256,512) Rinv of pass 1, [512,768) Rinv of pass 2
   double* R1 = ctx->small;
   double* Rinv1 = ctx->small + 256;

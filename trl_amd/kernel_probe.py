@@ -8,7 +8,7 @@ Kernels (--what, comma separated):
   spmm32, spmm64        Y = A X with the autotuned operator (FP32 / FP64)
   spmm_axpby64          Y = alpha A X + beta X + gamma Z (the FP64 Chebyshev step)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
-  cholqr32              CholQR2 of one FP32 panel
+  cholqr32, cholqr64    CholQR2 of one FP32 panel, CholQR of one FP64 panel
   copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
 Prints, per kernel, the mean time per launch (HIP events) and the algorithmic
 bytes per launch.
@@ -86,10 +86,14 @@ def main(argv=None) -> int:
             else:
                 Cs = ctx.to_device(np.zeros(k * b * b))  # W unchanged: W -= V 0
                 timed(w, lambda: L.check(H.trl_hip_update(ctx.h, L.F32, n, b, k, V.ptr, Cs.ptr, W), "update"), (k + 2) * P_, "update")
-        elif w == "cholqr32":
-            Wd = ctx.to_device(rng.standard_normal((n, b)).astype(np.float32))
+        elif w in ("cholqr32", "cholqr64"):
+            f32 = w == "cholqr32"
+            Wd = ctx.to_device(rng.standard_normal((n, b)).astype(np.float32 if f32 else np.float64))
             R = ctx.empty((b * b,))
-            timed(w, lambda: L.check(H.trl_hip_cholqr(ctx.h, L.F32, n, b, Wd.ptr, None, R.ptr, 2), "cholqr"), 5 * n * b * 4, "cholqr")
+            passes = 2 if f32 else 1
+            s_ = 4 if f32 else 8
+            timed(w, lambda: L.check(H.trl_hip_cholqr(ctx.h, L.F32 if f32 else L.F64, n, b, Wd.ptr, None, R.ptr, passes), "cholqr"),
+                  (5 if passes == 2 else 3) * n * b * s_, "cholqr")
         elif w == "copy":
             cnt = 64 << 20  # 64 Mi doubles = 512 MiB each way
             src = ctx.to_device(np.ones(cnt))
