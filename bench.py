@@ -186,9 +186,10 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--side", type=int, default=160, help="grid side of the 3D Laplacian (160 = the named workload)")
-    ap.add_argument("--tol-fp32", type=float, default=1e-3)
+    ap.add_argument("--tol-fp32", type=float, default=1e-2)
     ap.add_argument("--cheb-degree", type=int, default=80)
-    ap.add_argument("--refine-blocks", type=int, default=0)
+    ap.add_argument("--refine-blocks", type=int, default=9, help="Ritz blocks refined together in FP64 (0: nev/b + 7)")
+    ap.add_argument("--max-refine", type=int, default=14, help="FP64 refinement sweeps before the FP64 Lanczos fallback")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -222,7 +223,8 @@ def main(argv=None) -> int:
     log(f"spmm f32 {sp32['kernel']} {sp32['GBps']:.0f} GB/s ({sp32['avg_us']:.1f} us), f64 {sp64['kernel']} {sp64['GBps']:.0f} GB/s ({sp64['avg_us']:.1f} us)")
 
     # ---- warmup + profiled pass -------------------------------------------------
-    mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks, spmm_tune=tune, reorth=a.reorth)
+    mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks, max_refine=a.max_refine, spmm_tune=tune,
+                    reorth=a.reorth)
     for w in range(a.warmup):
         solve_mixed(dA, cfg.b, params, V0, **mixed_kw)
         log(f"warmup {w} done")
@@ -321,7 +323,7 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"{cfg.name}: 3D 7-point Laplacian {a.side}^3 + diag U[0,1]", "n": cfg.csr.n, "nnz": cfg.csr.nnz, "b": cfg.b, "nev": cfg.nev,
                    "ncv": cfg.ncv, "tol": cfg.tol, "order": "smallest", "tol_fp32": a.tol_fp32, "cheb_degree": a.cheb_degree,
-                   "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "spmm_tune": tune, "reorth": a.reorth},
+                   "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "max_refine": a.max_refine, "spmm_tune": tune, "reorth": a.reorth},
         "converged": bool(last.result["converged"]),
         "max_true_residual": float(last.resid.max()),
         "fp32_phase": last.result32,

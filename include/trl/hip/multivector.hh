@@ -73,6 +73,14 @@ public:
     throw_on(trl_hip_panel_mult(ctx_.get(), dtype, n_, block_size, data_, R.data_, other.data_, TRL_HIP_SUBTRACT_PRODUCT), "subtract_product");
   }
 
+  // this -= other * R^T in one pass (no counterpart in the concepts: the
+  // driver's mult_transpose into a scratch block followed by -=)
+  void subtract_product_transpose(const BlockView& other, const MatrixBlockView& R)
+  {
+    throw_on(trl_hip_panel_mult(ctx_.get(), dtype, n_, block_size, data_, R.data_, other.data_, TRL_HIP_SUBTRACT_PRODUCT_TRANSPOSE),
+             "subtract_product_transpose");
+  }
+
   const Context& context() const { return ctx_; }
 
   ScalarT* data_;

@@ -13,9 +13,11 @@
 // The reference's driver itself also runs unchanged on trl::hip (the
 // conformance build compiles it against these backend types); this copy
 // exists because the GPU runtime does not carry the reference headers, and
-// it adds two backend hooks when the EVP provides them:
+// it adds backend hooks when the EVP provides them:
 //   - evp.rotate(V, m, Y, k_r, W): the Ritz rotation as one contraction
 //     instead of k_r*m mult_add calls (lanczos.hh:105-114);
+//   - the three-term recurrence's "mult into scratch, then -=" pairs as one
+//     pass each (subtract_product, subtract_product_transpose);
 //   - ritz_vectors(): X = V Y[:, 0:nev], which the reference never forms.
 
 #include <memory>
@@ -138,12 +140,8 @@ public:
       ++res.n_op_apply;
       auto Tkk = T.block_view(k, k);
       evp->dot(Vk, Vk1, Tkk);
-      Vk.mult(Tkk, W0);
-      Vk1 -= W0;
-      if (k > 0) {
-        V.block_view(k - 1).mult_transpose(T.block_view(k, k - 1), W0);
-        Vk1 -= W0;
-      }
+      sub_product(Vk1, Vk, Tkk, W0);
+      if (k > 0) sub_product_transpose(Vk1, V.block_view(k - 1), T.block_view(k, k - 1), W0);
       reorth_(*evp, V, k + 1, Vk1, Z0);
       evp->orthonormalize(Vk1, beta);
       if (k + 1 < T.block_rows()) {
@@ -172,14 +170,10 @@ public:
       auto Vn = V.block_view(i + 1);
       evp->apply(Vc, Vn); // V_{i+1} = A V_i
       ++napp;
-      if (i > 0) { // V_{i+1} -= V_{i-1} beta^T
-        V.block_view(i - 1).mult_transpose(beta, W0);
-        Vn -= W0;
-      }
-      auto Tii = T.block_view(i, i); // T_ii = <V_i, V_{i+1}>
+      if (i > 0) sub_product_transpose(Vn, V.block_view(i - 1), beta, W0); // V_{i+1} -= V_{i-1} beta^T
+      auto Tii = T.block_view(i, i);                                        // T_ii = <V_i, V_{i+1}>
       evp->dot(Vc, Vn, Tii);
-      Vc.mult(Tii, W0); // V_{i+1} -= V_i T_ii
-      Vn -= W0;
+      sub_product(Vn, Vc, Tii, W0); // V_{i+1} -= V_i T_ii
       reorth_(*evp, V, i + 1, Vn, Z0);
       evp->orthonormalize(Vn, beta); // V_{i+1} beta = old V_{i+1}
       if (i + 1 < T.block_rows()) {
@@ -229,6 +223,31 @@ public:
   unsigned int get_nconv() const { return last_nconv; }
 
 private:
+  // Y -= X B as mult into scratch + -= (lanczos.hh:162-170, 259-265), or in
+  // one pass when the view offers subtract_product (every backend of the
+  // concepts does, blockview.hh:189-201)
+  template <class View, class MView>
+  static void sub_product(View& Y, const View& X, const MView& Bm, View& scratch)
+  {
+    if constexpr (requires { Y.subtract_product(X, Bm); })
+      Y.subtract_product(X, Bm);
+    else {
+      X.mult(Bm, scratch);
+      Y -= scratch;
+    }
+  }
+  // Y -= X B^T (lanczos.hh:166-169, 255-258): one pass when the backend has it
+  template <class View, class MView>
+  static void sub_product_transpose(View& Y, const View& X, const MView& Bm, View& scratch)
+  {
+    if constexpr (requires { Y.subtract_product_transpose(X, Bm); })
+      Y.subtract_product_transpose(X, Bm);
+    else {
+      X.mult_transpose(Bm, scratch);
+      Y -= scratch;
+    }
+  }
+
   std::shared_ptr<EVP> evp;
   unsigned int nev;
   unsigned int ncv;

@@ -90,6 +90,7 @@ int trl_hip_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* 
 #define TRL_HIP_MULT_ADD 1       /* Y += X * W   : BlockView::mult_add (blockview.hh:81-97)          */
 #define TRL_HIP_MULT_TRANSPOSE 2 /* Y  = X * W^T : BlockView::mult_transpose (blockview.hh:126-141)  */
 #define TRL_HIP_SUBTRACT_PRODUCT 3 /* X -= Y * W : BlockView::subtract_product (blockview.hh:189-201) */
+#define TRL_HIP_SUBTRACT_PRODUCT_TRANSPOSE 4 /* X -= Y * W^T : mult_transpose then -= (lanczos.hh:255-258) in one pass */
 /* W: b x b FP64 device block. */
 int trl_hip_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const double* W, void* Y, int mode);
 

@@ -229,6 +229,13 @@ def test_panel_ops(ctx, rng, p, b):
             assert rel(Xd.download(), Xr) < TOL[p]
         else:
             assert rel(Yd.download(), Yr) < TOL[p]
+    # mode 4: X -= Y W^T (mult_transpose into a scratch block, then -=), one pass
+    Xd, Yd = dev(ctx, X, dt), dev(ctx, Y0, dt)
+    L.check(H().trl_hip_panel_mult(ctx.h, L.dtype_code(dt), n, b, Xd.ptr, Wd.ptr, Yd.ptr, 4), "panel_mult")
+    Wt, Xr = np.zeros_like(Y0), X.copy()
+    oracle.panel_mult(Y0.copy(), W, Wt, 2)
+    assert rel(Xd.download(), Xr - Wt) < TOL[p]
+    assert H().trl_hip_panel_mult(ctx.h, L.dtype_code(dt), n, b, Xd.ptr, Wd.ptr, Yd.ptr, 5) == -1
     Xd, Yd = dev(ctx, X, dt), dev(ctx, Y0, dt)
     L.check(H().trl_hip_panel_sub(ctx.h, L.dtype_code(dt), n, b, Xd.ptr, Yd.ptr), "sub")
     assert rel(Xd.download(), X - Y0) < TOL[p]

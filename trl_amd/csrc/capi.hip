@@ -273,9 +273,9 @@ int trl_hip_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* 
 
 int trl_hip_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const double* W, void* Y, int mode)
 {
-  if (!ctx || !valid_b(b) || mode < 0 || mode > 3 || (n && (!X || !W || !Y))) return TRL_HIP_EINVAL;
+  if (!ctx || !valid_b(b) || mode < 0 || mode > 4 || (n && (!X || !W || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_PANEL, ((mode == 1 || mode == 3) ? 3 : 2) * panel_bytes(t, n, b));
+  Prof pr(ctx, TRL_HIP_PROF_PANEL, ((mode == 1 || mode >= 3) ? 3 : 2) * panel_bytes(t, n, b));
   return launch_panel_mult(ctx, t, n, b, X, W, Y, mode);
 }
 

@@ -70,13 +70,13 @@ __global__ __launch_bounds__(kThreads) void k_panel_mult(S* __restrict__ X, cons
   __shared__ double Ws[B * B];
   for (int i = threadIdx.x; i < B * B; i += blockDim.x) {
     const int r = i / B, c = i % B;
-    Ws[i] = (MODE == 2) ? Wg[c * B + r] : Wg[i];
+    Ws[i] = (MODE == 2 || MODE == 4) ? Wg[c * B + r] : Wg[i];
   }
   __syncthreads();
   const size_t stride = size_t(gridDim.x) * blockDim.x;
   for (size_t row = size_t(blockIdx.x) * blockDim.x + threadIdx.x; row < n; row += stride) {
     S in[B];
-    const S* src = (MODE == 3) ? Y : X;
+    const S* src = (MODE >= 3) ? Y : X;
     RowIO<S, B>::load(src + row * B, in);
     double acc[B];
 #pragma unroll
@@ -247,8 +247,11 @@ int launch_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* 
       case 2:
         hipLaunchKernelGGL((k_panel_mult<S, B, 2>), g, blk, 0, ctx->stream, x, W, y, n);
         break;
-      default:
+      case 3:
         hipLaunchKernelGGL((k_panel_mult<S, B, 3>), g, blk, 0, ctx->stream, x, W, y, n);
+        break;
+      default:
+        hipLaunchKernelGGL((k_panel_mult<S, B, 4>), g, blk, 0, ctx->stream, x, W, y, n);
         break;
       }
     });
