@@ -928,12 +928,16 @@ int trl_hip_batch_extend(trl_hip_batch* bt, int k, int m_end, int quasi, const d
   }
   else if ((rc = enqueue_cycle(bt, k, m_end, quasi)))
     return rc;
+  // the copies land in host memory owned by this frame and the caller:
+  // always wait for the stream before returning, also on an error
   int err = 0;
-  TRL_HIP_TRY(hipMemcpyAsync(&err, bt->err, sizeof(int), hipMemcpyDeviceToHost, s));
-  if (alpha) TRL_HIP_TRY(hipMemcpyAsync(alpha, bt->alpha, size_t(bt->nprob) * bt->m * BB * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (beta) TRL_HIP_TRY(hipMemcpyAsync(beta, bt->beta, size_t(bt->nprob) * bt->m * BB * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (pflag) TRL_HIP_TRY(hipMemcpyAsync(pflag, bt->pflag, bt->nprob * sizeof(int), hipMemcpyDeviceToHost, s));
-  TRL_HIP_TRY(hipStreamSynchronize(s));
+  hipError_t e = hipMemcpyAsync(&err, bt->err, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && alpha) e = hipMemcpyAsync(alpha, bt->alpha, size_t(bt->nprob) * bt->m * BB * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && beta) e = hipMemcpyAsync(beta, bt->beta, size_t(bt->nprob) * bt->m * BB * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && pflag) e = hipMemcpyAsync(pflag, bt->pflag, bt->nprob * sizeof(int), hipMemcpyDeviceToHost, s);
+  const hipError_t es = hipStreamSynchronize(s);
+  if (e != hipSuccess) return int(e);
+  if (es != hipSuccess) return int(es);
   return err ? TRL_HIP_EBARRIER : 0;
 }
 
@@ -1011,8 +1015,10 @@ int trl_hip_batch_residuals(trl_hip_batch* bt, int nblk, const double* lambda, d
 #undef TRL_BRES
   TRL_HIP_LAUNCH_CHECK();
   std::vector<double> sums(ns);
-  TRL_HIP_TRY(hipMemcpyAsync(sums.data(), bt->rsum, ns * sizeof(double), hipMemcpyDeviceToHost, s));
-  TRL_HIP_TRY(hipStreamSynchronize(s));
+  const hipError_t e = hipMemcpyAsync(sums.data(), bt->rsum, ns * sizeof(double), hipMemcpyDeviceToHost, s);
+  const hipError_t es = hipStreamSynchronize(s);
+  if (e != hipSuccess) return int(e);
+  if (es != hipSuccess) return int(es);
   std::vector<double> acc(size_t(bt->nprob) * nblk * 2 * b, 0.0);
   for (int t = 0; t < bt->ntiles; ++t) {
     const int p = bt->tile_prob_h[t];
