@@ -96,24 +96,25 @@ def cpu_baseline(cfg, fp64_cycles, budget_s):
     }
 
 
-def dominant_kernel_rate(ctx, dA, b, reps=50) -> dict:
-    """The hottest kernel of the mixed solve, the FP64 Chebyshev step
-    Y = alpha A X + beta X + gamma Z with the autotuned operator, timed alone
-    with HIP events on the library stream; algorithmic bytes = the stored
-    matrix entries (values + int32 indices) + X + Z + Y."""
+def dominant_kernel_rate(ctx, dA, b, dtype, reps=50) -> dict:
+    """One Chebyshev step Y = alpha A X + beta X + gamma Z with the autotuned
+    operator of the given panel dtype (the hottest kernel of the mixed solve),
+    timed alone with HIP events on the library stream; algorithmic bytes =
+    the stored matrix entries (values + int32 indices) + X + Z + Y."""
     import ctypes as C
 
     from trl_amd import _lib as L
 
-    op = dA.operator(b, "f64", tune=True)
+    op = dA.operator(b, dtype, tune=True)
+    dt = op.dtype
     rng = np.random.default_rng(1)
-    X = ctx.to_device(rng.standard_normal((dA.n, b)))
-    Z = ctx.to_device(rng.standard_normal((dA.n, b)))
-    Y = ctx.empty((dA.n, b))
+    X = ctx.to_device(rng.standard_normal((dA.n, b)).astype(dt))
+    Z = ctx.to_device(rng.standard_normal((dA.n, b)).astype(dt))
+    Y = ctx.empty((dA.n, b), dt)
     H = L.hip()
 
     def step():
-        L.check(H.trl_hip_spmm_axpby(ctx.h, L.F64, dA.n, b, C.byref(op.desc), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125), "axpby")
+        L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), dA.n, b, C.byref(op.desc), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125), "axpby")
 
     for _ in range(3):
         step()
@@ -124,8 +125,9 @@ def dominant_kernel_rate(ctx, dA, b, reps=50) -> dict:
     ctx.prof_enable(False)
     avg = st["seconds"] / st["calls"]
     per = st["bytes"] / st["calls"]
-    name = f"k_spmm_rows<double, {b}, {max(1, -op.desc.variant)}, {'true' if op.desc.format else 'false'}, true>" if op.desc.variant <= 1 else \
-        f"k_spmm_lanes<double, {b}, {op.desc.variant}, true>"
+    S = "double" if dtype == "f64" else "float"
+    name = f"k_spmm_rows<{S}, {b}, {max(1, -op.desc.variant)}, {'true' if op.desc.format else 'false'}, true>" if op.desc.variant <= 1 else \
+        f"k_spmm_lanes<{S}, {b}, {op.desc.variant}, true>"
     return {"kernel": name, "operator": op.name, "launches": st["calls"], "avg_launch_us": avg * 1e6, "algorithmic_bytes_per_launch": per,
             "achieved": per / avg / 1e9}
 
@@ -186,10 +188,12 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--side", type=int, default=160, help="grid side of the 3D Laplacian (160 = the named workload)")
-    ap.add_argument("--tol-fp32", type=float, default=1e-2)
-    ap.add_argument("--cheb-degree", type=int, default=80)
+    ap.add_argument("--tol-fp32", type=float, default=5e-2)
+    ap.add_argument("--cheb-degree", type=int, default=100)
     ap.add_argument("--refine-blocks", type=int, default=9, help="Ritz blocks refined together in FP64 (0: nev/b + 7)")
     ap.add_argument("--max-refine", type=int, default=14, help="FP64 refinement sweeps before the FP64 Lanczos fallback")
+    ap.add_argument("--fp32-sweeps", type=int, default=8, help="FP32 refinement sweeps (at most) before the FP64 ones")
+    ap.add_argument("--tol-fp32-refine", type=float, default=3e-5, help="FP32 residual at which the FP32 sweeps stop")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -224,13 +228,14 @@ def main(argv=None) -> int:
 
     # ---- warmup + profiled pass -------------------------------------------------
     mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks, max_refine=a.max_refine, spmm_tune=tune,
-                    reorth=a.reorth)
+                    reorth=a.reorth, fp32_sweeps=a.fp32_sweeps, tol_fp32_refine=a.tol_fp32_refine)
     for w in range(a.warmup):
         solve_mixed(dA, cfg.b, params, V0, **mixed_kw)
         log(f"warmup {w} done")
     ctx.prof_enable(True)
     prof_out = solve_mixed(dA, cfg.b, params, V0, **mixed_kw)
     fam = ctx.prof_read()
+    fam_t = {"f32": ctx.prof_read_dtype(L.F32), "f64": ctx.prof_read_dtype(L.F64)}
     reorth_stats = ctx.prof_reorth()
     ctx.prof_enable(False)
     log(f"profiled mixed solve: {prof_out.result['seconds']:.2f} s, fp32 cycles {prof_out.result32['iterations']}, fp64 cycles {prof_out.result['iterations']}")
@@ -277,13 +282,17 @@ def main(argv=None) -> int:
     dom = max(busy, key=lambda k: busy[k]["seconds"])
     d = busy[dom]
     if dom == "spmm":
-        k = dominant_kernel_rate(ctx, dA, cfg.b)
+        # the operator kernel of the precision that spent more time in it
+        kdt = max(("f32", "f64"), key=lambda t: fam_t[t]["spmm"]["seconds"])
+        k = dominant_kernel_rate(ctx, dA, cfg.b, kdt)
         traffic, tsrc = traffic_of(k["kernel"])
         roof = {"kernel": k["kernel"], "bound": "hbm", "achieved": k["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": k["achieved"] / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": tsrc, "launches": k["launches"],
                 "avg_launch_us": k["avg_launch_us"], "algorithmic_bytes_per_launch": k["algorithmic_bytes_per_launch"],
                 "operator": k["operator"], "family_in_solve": {"name": dom, "calls": d["calls"], "seconds": d["seconds"],
-                                                                "GBps": d["bytes"] / d["seconds"] / 1e9}}
+                                                                "GBps": d["bytes"] / d["seconds"] / 1e9,
+                                                                "by_dtype": {t: {"calls": fam_t[t][dom]["calls"], "seconds": fam_t[t][dom]["seconds"]}
+                                                                             for t in fam_t}}}
         log(f"dominant kernel {k['kernel']}: {k['avg_launch_us']:.1f} us, {k['achieved']:.0f} GB/s")
     else:
         achieved = d["bytes"] / d["seconds"] / 1e9
@@ -292,6 +301,11 @@ def main(argv=None) -> int:
                 "algorithmic_bytes_per_launch": d["bytes"] / d["calls"]}
     families = {k: {"calls": v["calls"], "seconds": round(v["seconds"], 6), "GBps": (v["bytes"] / v["seconds"] / 1e9) if v["seconds"] and v["bytes"] else None}
                 for k, v in fam.items() if v["calls"]}
+    for t in fam_t:
+        for k, v in fam_t[t].items():
+            if v["calls"] and k in families:
+                families[k][t] = {"calls": v["calls"], "seconds": round(v["seconds"], 6),
+                                  "GBps": (v["bytes"] / v["seconds"] / 1e9) if v["seconds"] and v["bytes"] else None}
 
     batched = None
     if not a.no_batch:
@@ -323,7 +337,8 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"{cfg.name}: 3D 7-point Laplacian {a.side}^3 + diag U[0,1]", "n": cfg.csr.n, "nnz": cfg.csr.nnz, "b": cfg.b, "nev": cfg.nev,
                    "ncv": cfg.ncv, "tol": cfg.tol, "order": "smallest", "tol_fp32": a.tol_fp32, "cheb_degree": a.cheb_degree,
-                   "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "max_refine": a.max_refine, "spmm_tune": tune, "reorth": a.reorth},
+                   "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "max_refine": a.max_refine, "fp32_sweeps": a.fp32_sweeps,
+                   "tol_fp32_refine": a.tol_fp32_refine, "spmm_tune": tune, "reorth": a.reorth},
         "converged": bool(last.result["converged"]),
         "max_true_residual": float(last.resid.max()),
         "fp32_phase": last.result32,

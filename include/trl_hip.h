@@ -310,6 +310,8 @@ int trl_hip_prof_enable(trl_hip_ctx* ctx, int on);
 /* Synchronous: folds finished calls in and returns the totals of one family
  * since the last enable: calls, seconds between the event pairs, bytes. */
 int trl_hip_prof_read(trl_hip_ctx* ctx, int family, long long* calls, double* seconds, double* bytes);
+/* The same totals restricted to calls on panels of dtype t. */
+int trl_hip_prof_read_dtype(trl_hip_ctx* ctx, int family, trl_hip_dtype t, long long* calls, double* seconds, double* bytes);
 /* Profiling mode: trl_hip_reorth calls with eta > 0 since the last enable,
  * and how many of them took the second pass. */
 int trl_hip_prof_reorth(trl_hip_ctx* ctx, long long* calls, long long* second_passes);

@@ -72,13 +72,17 @@ typedef struct {
   unsigned refine_blocks;    /* refine = 1: Ritz blocks refined together (0: nev/b + 7);
                                 the blocks beyond nev/b + 1 are a buffer that moves
                                 the filter cutoff away from the wanted values         */
+  unsigned fp32_sweeps;      /* refine = 1: at most this many filter sweeps in FP32
+                                panels first (0: none) ...                            */
+  double tol_fp32_refine;    /* ... stopping once the FP32 residuals are below this   */
 } trl_hip_mixed_params;
 
 /* Mixed precision.  Phase 1: an FP32 thick-restart solve (FP32 basis, SpMM
  * and Gram products with compensated sums, FP64 coefficients, Cholesky and
  * Rayleigh-Ritz) to tol_fp32, keeping its leading Ritz blocks X.
  * Phase 2 (FP64), by mp->refine:
- *   1: block refinement of all of X together: FP64 block CGS2 + CholQR2,
+ *   1: (optionally first the same sweeps in FP32 panels, mp->fp32_sweeps)
+ *      block refinement of all of X together: FP64 block CGS + CholQR2,
  *      Rayleigh-Ritz, and while a true residual exceeds p->tol, a scaled
  *      Chebyshev filter of degree cheb_degree (Zhou & Saad, J. Comput. Phys.
  *      2007) that damps the spectrum beyond the kept Ritz values, then

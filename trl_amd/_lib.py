@@ -76,6 +76,7 @@ HIP_API = [
     ("trl_hip_small_mult", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_residual_sq", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_reorth", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, C.c_double]),
+    ("trl_hip_prof_read_dtype", c_int, [c_void_p, c_int, c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("trl_hip_prof_reorth", c_int, [c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     ("trl_hip_prof_enable", c_int, [c_void_p, c_int]),
     ("trl_hip_prof_read", c_int, [c_void_p, c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -225,6 +226,15 @@ class Context:
             out[name] = {"calls": n.value, "seconds": s.value, "bytes": b.value}
         return out
 
+    def prof_read_dtype(self, dtype: int) -> dict:
+        """prof_read restricted to calls on panels of one dtype (F32 / F64)."""
+        out = {}
+        for f, name in enumerate(self.PROF_FAMILIES):
+            n, s, b = C.c_longlong(), C.c_double(), C.c_double()
+            check(hip().trl_hip_prof_read_dtype(self.h, f, dtype, C.byref(n), C.byref(s), C.byref(b)), "prof_read_dtype")
+            out[name] = {"calls": n.value, "seconds": s.value, "bytes": b.value}
+        return out
+
     def prof_reorth(self) -> dict:
         """trl_hip_reorth calls with the second-pass test since prof_enable, and
         how many took the second pass."""
@@ -264,7 +274,7 @@ class SolverResult(C.Structure):
 
 class MixedParams(C.Structure):
     _fields_ = [("tol_fp32", C.c_double), ("max_restarts_fp32", C.c_uint), ("seed", C.c_ulonglong), ("refine", c_int), ("cheb_degree", C.c_uint),
-                ("max_refine", C.c_uint), ("phase1_fp64", c_int), ("refine_blocks", C.c_uint)]
+                ("max_refine", C.c_uint), ("phase1_fp64", c_int), ("refine_blocks", C.c_uint), ("fp32_sweeps", C.c_uint), ("tol_fp32_refine", C.c_double)]
 
 
 class BatchParams(C.Structure):

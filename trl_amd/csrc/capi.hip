@@ -30,8 +30,9 @@ struct Prof {
   trl_hip_ctx* ctx;
   int family;
   double bytes;
+  int dtype; // trl_hip_dtype of the call, -1 if none
   hipEvent_t start = nullptr;
-  Prof(trl_hip_ctx* c, int fam, double b) : ctx(c), family(fam), bytes(b)
+  Prof(trl_hip_ctx* c, int fam, double b, int t = -1) : ctx(c), family(fam), bytes(b), dtype(t)
   {
     if (!ctx->prof_on) return;
     start = take();
@@ -54,7 +55,7 @@ struct Prof {
     hipEvent_t stop = take();
     if (!stop) return;
     (void)hipEventRecord(stop, ctx->stream);
-    ctx->prof_pending.push_back({start, stop, family, bytes});
+    ctx->prof_pending.push_back({start, stop, family, bytes, dtype});
   }
 };
 
@@ -68,6 +69,11 @@ int prof_fold(trl_hip_ctx* ctx)
       ctx->prof_seconds[r.family] += 1e-3 * ms;
       ctx->prof_bytes[r.family] += r.bytes;
       ctx->prof_count[r.family] += 1;
+      if (r.dtype == TRL_HIP_F32 || r.dtype == TRL_HIP_F64) {
+        ctx->prof_t_seconds[r.family][r.dtype] += 1e-3 * ms;
+        ctx->prof_t_bytes[r.family][r.dtype] += r.bytes;
+        ctx->prof_t_count[r.family][r.dtype] += 1;
+      }
     }
     ctx->prof_pool.push_back(r.start);
     ctx->prof_pool.push_back(r.stop);
@@ -267,7 +273,7 @@ int trl_hip_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* 
 {
   if (!ctx || !valid_b(b) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_PANEL, 3 * panel_bytes(t, n, b));
+  Prof pr(ctx, TRL_HIP_PROF_PANEL, 3 * panel_bytes(t, n, b), t);
   return launch_panel_sub(ctx, t, n, b, X, Y);
 }
 
@@ -275,14 +281,14 @@ int trl_hip_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void*
 {
   if (!ctx || !valid_b(b) || mode < 0 || mode > 4 || (n && (!X || !W || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_PANEL, ((mode == 1 || mode >= 3) ? 3 : 2) * panel_bytes(t, n, b));
+  Prof pr(ctx, TRL_HIP_PROF_PANEL, ((mode == 1 || mode >= 3) ? 3 : 2) * panel_bytes(t, n, b), t);
   return launch_panel_mult(ctx, t, n, b, X, W, Y, mode);
 }
 
 int trl_hip_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* mdiag, double* C)
 {
   if (!ctx || !valid_b(b) || k < 1 || !C || (n && (!V || !W))) return TRL_HIP_EINVAL;
-  Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0));
+  Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0), t);
   return launch_gram(ctx, t, n, b, k, V, W, mdiag, C);
 }
 
@@ -290,7 +296,7 @@ int trl_hip_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
 {
   if (!ctx || !valid_b(b) || k < 1 || !C || (n && (!V || !W))) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * panel_bytes(t, n, b));
+  Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * panel_bytes(t, n, b), t);
   return launch_update(ctx, t, n, b, k, V, C, W);
 }
 
@@ -306,10 +312,10 @@ int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
   if (eta <= 0.0) { // two full passes
     for (double* Cp : {C1, C2}) {
       {
-        Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm);
+        Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm, t);
         if ((rc = launch_gram(ctx, t, n, b, k, V, W, mdiag, Cp))) return rc;
       }
-      Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P);
+      Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P, t);
       if ((rc = launch_update(ctx, t, n, b, k, V, Cp, W))) return rc;
     }
     return 0;
@@ -318,7 +324,7 @@ int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
   const bool contiguous = static_cast<const char*>(V) + size_t(k) * n * b * es == static_cast<const char*>(W);
   {
     // pass 1 with W^T M W in the same sweep when W follows the k blocks
-    Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm + (contiguous ? 0.0 : 2 * P + Pm));
+    Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm + (contiguous ? 0.0 : 2 * P + Pm), t);
     if (contiguous)
       rc = launch_gram(ctx, t, n, b, k + 1, V, W, mdiag, C1);
     else {
@@ -330,7 +336,7 @@ int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
   int* gate = ctx->flags + 24;
   if ((rc = launch_dgks(ctx, b, k, C1, eta * eta, gate))) return rc;
   {
-    Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P);
+    Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P, t);
     if ((rc = launch_update(ctx, t, n, b, k, V, C1, W))) return rc;
   }
   if (ctx->prof_on) {
@@ -342,10 +348,10 @@ int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
     if (!g) return 0;
     ctx->reorth_second += 1;
     {
-      Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm);
+      Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * P + Pm, t);
       if ((rc = launch_gram(ctx, t, n, b, k, V, W, mdiag, C2))) return rc;
     }
-    Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P);
+    Prof pr(ctx, TRL_HIP_PROF_UPDATE, (k + 2) * P, t);
     return launch_update(ctx, t, n, b, k, V, C2, W);
   }
   if ((rc = launch_gram(ctx, t, n, b, k, V, W, mdiag, C2, gate))) return rc;
@@ -355,7 +361,7 @@ int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
 int trl_hip_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* mdiag, double* R, int passes)
 {
   if (!ctx || !valid_b(b) || !R || passes < 1 || passes > 2 || (n && !W)) return TRL_HIP_EINVAL;
-  Prof pr(ctx, TRL_HIP_PROF_CHOLQR, passes * (3 * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0)));
+  Prof pr(ctx, TRL_HIP_PROF_CHOLQR, passes * (3 * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0)), t);
   return launch_cholqr(ctx, t, n, b, W, mdiag, R, passes);
 }
 
@@ -363,7 +369,7 @@ int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, co
 {
   if (!ctx || !valid_b(b) || m < 1 || kc < 1 || y_block_cols < kc || !Y || (n && (!V || !Out))) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_ROTATE, (m + kc) * panel_bytes(t, n, b));
+  Prof pr(ctx, TRL_HIP_PROF_ROTATE, (m + kc) * panel_bytes(t, n, b), t);
   return launch_rotate(ctx, t, n, b, m, V, Y, y_block_cols, kc, Out);
 }
 
@@ -394,7 +400,7 @@ int trl_hip_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_h
 {
   if (!ctx || !valid_b(b) || check_spmat(n, A) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 2) : 0.0);
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 2) : 0.0, t);
   return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, nullptr, false, 0, 0, 0, A->variant);
 }
 
@@ -404,7 +410,7 @@ int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const
   if (!ctx || !valid_b(b) || check_spmat(n, A) || (n && (!X || !Y)) || X == Y) return TRL_HIP_EINVAL;
   if (!n) return 0;
   // algorithmic panels: X (gathered rows and the beta term read it once), Z, Y
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, Z ? 3 : 2) : 0.0);
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, Z ? 3 : 2) : 0.0, t);
   return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant);
 }
 
@@ -412,7 +418,7 @@ int trl_hip_spmm_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, cons
 {
   if (!ctx || !valid_b(b) || npanels < 1 || check_spmat(n, A) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + 2.0 * npanels * panel_bytes(t, n, b) : 0.0);
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + 2.0 * npanels * panel_bytes(t, n, b) : 0.0, t);
   return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, nullptr, false, 0, 0, 0, A->variant,
                      npanels);
 }
@@ -426,7 +432,7 @@ int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b
   const size_t span = size_t(npanels) * n * b * es;
   if (x < y + span && y < x + span) return TRL_HIP_EINVAL; // X and Y must not overlap
   if (!n) return 0;
-  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + (Z ? 3.0 : 2.0) * npanels * panel_bytes(t, n, b) : 0.0);
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + (Z ? 3.0 : 2.0) * npanels * panel_bytes(t, n, b) : 0.0, t);
   return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant,
                      npanels);
 }
@@ -576,6 +582,11 @@ int trl_hip_prof_enable(trl_hip_ctx* ctx, int on)
       ctx->prof_count[f] = 0;
       ctx->prof_seconds[f] = 0.0;
       ctx->prof_bytes[f] = 0.0;
+      for (int d = 0; d < 2; ++d) {
+        ctx->prof_t_count[f][d] = 0;
+        ctx->prof_t_seconds[f][d] = 0.0;
+        ctx->prof_t_bytes[f][d] = 0.0;
+      }
     }
     ctx->reorth_calls = ctx->reorth_second = 0;
   }
@@ -591,6 +602,17 @@ int trl_hip_prof_read(trl_hip_ctx* ctx, int family, long long* calls, double* se
   if (calls) *calls = ctx->prof_count[family];
   if (seconds) *seconds = ctx->prof_seconds[family];
   if (bytes) *bytes = ctx->prof_bytes[family];
+  return 0;
+}
+
+int trl_hip_prof_read_dtype(trl_hip_ctx* ctx, int family, trl_hip_dtype t, long long* calls, double* seconds, double* bytes)
+{
+  if (!ctx || family < 0 || family >= TRL_HIP_PROF_FAMILIES || (t != TRL_HIP_F32 && t != TRL_HIP_F64)) return TRL_HIP_EINVAL;
+  int rc = prof_fold(ctx);
+  if (rc) return rc;
+  if (calls) *calls = ctx->prof_t_count[family][t];
+  if (seconds) *seconds = ctx->prof_t_seconds[family][t];
+  if (bytes) *bytes = ctx->prof_t_bytes[family][t];
   return 0;
 }
 

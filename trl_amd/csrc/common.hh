@@ -33,6 +33,7 @@ struct trl_hip_ctx {
     hipEvent_t start, stop;
     int family;
     double bytes;
+    int dtype;
   };
   int prof_on = 0;
   std::vector<ProfRec> prof_pending;
@@ -40,6 +41,10 @@ struct trl_hip_ctx {
   long long prof_count[TRL_HIP_PROF_FAMILIES] = {};
   double prof_seconds[TRL_HIP_PROF_FAMILIES] = {};
   double prof_bytes[TRL_HIP_PROF_FAMILIES] = {};
+  // the same split by panel dtype (index trl_hip_dtype)
+  long long prof_t_count[TRL_HIP_PROF_FAMILIES][2] = {};
+  double prof_t_seconds[TRL_HIP_PROF_FAMILIES][2] = {};
+  double prof_t_bytes[TRL_HIP_PROF_FAMILIES][2] = {};
   // profiling mode: trl_hip_reorth calls and how many took the second pass
   long long reorth_calls = 0, reorth_second = 0;
   std::unordered_map<const void*, long long> nnz_cache;

@@ -135,26 +135,29 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
   }
 }
 
-// FP64 block refinement of nb Ritz panels toward one end of the spectrum:
+// Block refinement of nb Ritz panels (panel type S, FP64 coefficients)
+// toward one end of the spectrum:
 // block CGS2 + CholQR2 orthonormalisation, Rayleigh-Ritz over the nb*B
 // vectors, and a scaled Chebyshev filter (Zhou & Saad, "Self-consistent-field
 // calculations using Chebyshev-filtered subspace iteration", J. Comput. Phys.
 // 226 (2007), Algorithm 3.2) that keeps the value 1 at the far wanted end a0
 // and damps [lo, hi].
-template <unsigned B>
+template <class S, unsigned B>
 struct BlockRefiner {
+  static constexpr trl_hip_dtype t = trl::hip::dtype_of<S>();
   const Context& c;
   std::size_t n;
-  const SparseOperator<double>& op;
-  const double* mass;
+  const SparseOperator<S>& op;
+  const S* mass;
   unsigned nb;
   bool ascending;
   double eta; // second-pass test of the block Gram-Schmidt (<= 0: always two passes)
-  trl::hip::DeviceBuffer<double> Xa, Xb, AX, coef;
-  double* X; // current Ritz panels (Xa or Xb)
-  double* spare;
+  trl::hip::DeviceBuffer<S> Xa, Xb, AX;
+  trl::hip::DeviceBuffer<double> coef;
+  S* X; // current Ritz panels (Xa or Xb)
+  S* spare;
 
-  BlockRefiner(const Context& ctx, std::size_t n_, const SparseOperator<double>& o, const double* m, unsigned nb_, bool asc, double eta_)
+  BlockRefiner(const Context& ctx, std::size_t n_, const SparseOperator<S>& o, const S* m, unsigned nb_, bool asc, double eta_)
       : c(ctx), n(n_), op(o), mass(m), nb(nb_), ascending(asc), eta(eta_), Xa(ctx, std::size_t(nb_) * n_ * B),
         Xb(ctx, std::size_t(nb_) * n_ * B), AX(ctx, std::size_t(nb_) * n_ * B), coef(ctx, std::size_t(nb_) * nb_ * B * B + 2 * B * B)
   {
@@ -162,15 +165,15 @@ struct BlockRefiner {
     spare = Xb.data();
   }
 
-  double* panel(double* base, unsigned j) const { return base + std::size_t(j) * n * B; }
+  S* panel(S* base, unsigned j) const { return base + std::size_t(j) * n * B; }
 
   void orthonormalize()
   {
     double* C = coef.data();
     double* R = coef.data() + std::size_t(nb) * nb * B * B;
     for (unsigned j = 0; j < nb; ++j) {
-      if (j > 0) throw_on(trl_hip_reorth(c.get(), TRL_HIP_F64, n, B, int(j), X, panel(X, j), mass, C, eta), "refine reorth");
-      throw_on(trl_hip_cholqr(c.get(), TRL_HIP_F64, n, B, panel(X, j), mass, R, 2), "refine cholqr");
+      if (j > 0) throw_on(trl_hip_reorth(c.get(), t, n, B, int(j), X, panel(X, j), mass, C, eta), "refine reorth");
+      throw_on(trl_hip_cholqr(c.get(), t, n, B, panel(X, j), mass, R, 2), "refine cholqr");
     }
     c.sync(); // raises a breakdown
   }
@@ -182,7 +185,7 @@ struct BlockRefiner {
     for (unsigned j = 0; j < nb; ++j) op.apply(int(B), panel(X, j), panel(AX.data(), j));
     double* C = coef.data();
     for (unsigned j = 0; j < nb; ++j)
-      throw_on(trl_hip_gram(c.get(), TRL_HIP_F64, n, B, int(nb), X, panel(AX.data(), j), mass, C + std::size_t(j) * nt * B), "refine projection");
+      throw_on(trl_hip_gram(c.get(), t, n, B, int(nb), X, panel(AX.data(), j), mass, C + std::size_t(j) * nt * B), "refine projection");
     std::vector<double> Cj(std::size_t(nt) * nt);
     throw_on(trl_hip_memcpy_d2h(c.get(), Cj.data(), C, Cj.size() * sizeof(double)), "refine download");
     // C + j*nt*B holds rows r = 0..nt-1 (panel i = r/B, column r%B) of H[:, j*B:(j+1)*B]
@@ -206,7 +209,7 @@ struct BlockRefiner {
         for (unsigned ci = 0; ci < B; ++ci)
           for (unsigned cj = 0; cj < B; ++cj) Y[((std::size_t(bi) * nb + bj) * B + ci) * B + cj] = Z[std::size_t(bi * B + ci) * nt + idx[bj * B + cj]];
     throw_on(trl_hip_memcpy_h2d(c.get(), C, Y.data(), Y.size() * sizeof(double)), "refine upload");
-    throw_on(trl_hip_rotate(c.get(), TRL_HIP_F64, n, B, int(nb), X, C, int(nb), int(nb), spare), "refine rotate");
+    throw_on(trl_hip_rotate(c.get(), t, n, B, int(nb), X, C, int(nb), int(nb), spare), "refine rotate");
     std::swap(X, spare);
   }
 
@@ -221,8 +224,8 @@ struct BlockRefiner {
     // neighbours of a row range (all panels at once would multiply that
     // working set by nb)
     for (unsigned j = 0; j < nb; ++j) {
-      double* prev = panel(X, j);
-      double* cur = panel(spare, j);
+      S* prev = panel(X, j);
+      S* cur = panel(spare, j);
       op.axpby(int(B), prev, cur, nullptr, s1 / e, -s1 * cc / e, 0.0);
       double sg = s1;
       for (unsigned i = 2; i <= degree; ++i) {
@@ -235,6 +238,41 @@ struct BlockRefiner {
     if (degree % 2 == 1) std::swap(X, spare); // odd degree leaves the result in the spare buffer
   }
 };
+
+// Refinement sweeps: orthonormalise, Rayleigh-Ritz, true residuals of the
+// first nev pairs; stop when all are <= tol or after max_sweeps filters.
+// Returns true on convergence; theta and rs hold the last values.
+template <class S, unsigned B>
+bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, const SparseOperator<S>& op, const S* mass, unsigned nev, double tol,
+                   unsigned max_sweeps, unsigned degree, double lo_all, double hi_all, std::vector<double>& theta, std::vector<double>& rs,
+                   unsigned& sweeps, const char* tag)
+{
+  const double margin = 0.01 * (hi_all - lo_all);
+  rs.assign(nev, 0.0);
+  sweeps = 0;
+  for (;;) {
+    rf.orthonormalize();
+    rf.rayleigh_ritz(theta);
+    true_residuals<S, B>(c, n, op, mass, rf.X, nev, theta.data(), rs.data());
+    double worst = 0.0;
+    for (double r : rs) worst = std::max(worst, r);
+    if (std::getenv("TRL_HIP_VERBOSE")) {
+      std::vector<double> srt(rs);
+      std::sort(srt.begin(), srt.end());
+      std::fprintf(stderr, "[refine %s] sweep %u: resid max %.3e median %.3e min %.3e theta0 %.9g theta_kept %.9g\n", tag, sweeps, worst,
+                   srt[srt.size() / 2], srt.front(), theta.front(), theta.back());
+    }
+    if (worst <= tol) return true;
+    if (sweeps >= max_sweeps) return false;
+    ++sweeps;
+    const double far = theta.back(); // last kept Ritz value
+    if (rf.ascending)
+      rf.filter(degree, far, hi_all + margin, theta.front());
+    else
+      rf.filter(degree, lo_all - margin, far, theta.front());
+  }
+}
+
 
 template <class F>
 int dispatch_b(int b, F&& f)
@@ -352,54 +390,45 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         p32.cholqr_passes = p->cholqr_passes;
         solve_typed<double, B>(c, n, rowptr, col, val64, mass64, minv64, op64, p32, V0, ev32.data(), X1.data(), nref, nullptr, r32, &all32, &got);
       }
-      else {
-        const auto op32 = make_operator<float, B>(c, n, rowptr, col, val32, minv32, *p);
+      std::shared_ptr<SparseOperator<float>> op32;
+      if (!mp->phase1_fp64) {
+        op32 = make_operator<float, B>(c, n, rowptr, col, val32, minv32, *p);
         solve_typed<float, B>(c, n, rowptr, col, val32, mass32, minv32, op32, p32, V0f.data(), ev32.data(), X32.data(), nref, nullptr, r32, &all32, &got);
       }
       nref = got;
+      // spectrum ends seen by the phase-1 Krylov space
+      double lo_all = all32.front(), hi_all = all32.front();
+      for (double v : all32) {
+        lo_all = std::min(lo_all, v);
+        hi_all = std::max(hi_all, v);
+      }
+      const bool asc = (p->order == 1);
+      const double eta = p->reorth == 2 ? 0.70710678118654752 : 0.0;
+      if (mp->refine == 1 && op32 && mp->fp32_sweeps > 0) {
+        // ---- FP32 refinement sweeps: the same filter in FP32 panels (half
+        // the bytes) until the FP32 residuals reach tol_fp32_refine ----------
+        BlockRefiner<float, B> rf32(c, n, *op32, mass32, nref, asc, eta);
+        throw_on(trl_hip_memcpy_d2d(c.get(), rf32.X, X32.data(), std::size_t(nref) * n * B * sizeof(float)), "copy X32");
+        std::vector<double> th32, rs32;
+        unsigned sw32 = 0;
+        refine_sweeps<float, B>(rf32, c, n, *op32, mass32, p->nev, mp->tol_fp32_refine, mp->fp32_sweeps, mp->cheb_degree, lo_all, hi_all, th32, rs32, sw32,
+                                "fp32");
+        throw_on(trl_hip_memcpy_d2d(c.get(), X32.data(), rf32.X, std::size_t(nref) * n * B * sizeof(float)), "copy X32");
+        r32.n_op_apply += (sw32 + 1) * nref * 2 + sw32 * nref * mp->cheb_degree;
+        r32.iterations += sw32;
+      }
 
       bool done = false;
       if (mp->refine == 1) {
         // ---- FP64 phase: block refinement of the FP32 Ritz blocks -----------
-        const bool asc = (p->order == 1);
-        BlockRefiner<B> rf(c, n, *op64, mass64, nref, asc, p->reorth == 2 ? 0.70710678118654752 : 0.0);
+        BlockRefiner<double, B> rf(c, n, *op64, mass64, nref, asc, eta);
         if (mp->phase1_fp64)
           throw_on(trl_hip_memcpy_d2d(c.get(), rf.X, X1.data(), std::size_t(nref) * n * B * sizeof(double)), "copy X1");
         else
           throw_on(trl_hip_convert(c.get(), TRL_HIP_F32, X32.data(), TRL_HIP_F64, rf.X, std::size_t(nref) * n * B), "convert X32");
-        // spectrum ends seen by the FP32 Krylov space
-        double lo_all = all32.front(), hi_all = all32.front();
-        for (double v : all32) {
-          lo_all = std::min(lo_all, v);
-          hi_all = std::max(hi_all, v);
-        }
-        const double margin = 0.01 * (hi_all - lo_all);
-        std::vector<double> theta, rs(p->nev);
+        std::vector<double> theta, rs;
         unsigned sweep = 0;
-        for (;;) {
-          rf.orthonormalize();
-          rf.rayleigh_ritz(theta);
-          true_residuals<double, B>(c, n, *op64, mass64, rf.X, p->nev, theta.data(), rs.data());
-          double worst = 0.0;
-          for (double r : rs) worst = std::max(worst, r);
-          if (std::getenv("TRL_HIP_VERBOSE")) {
-            std::vector<double> srt(rs);
-            std::sort(srt.begin(), srt.end());
-            std::fprintf(stderr, "[refine] sweep %u: resid max %.3e median %.3e min %.3e theta0 %.9g theta_kept %.9g\n", sweep, worst,
-                         srt[srt.size() / 2], srt.front(), theta.front(), theta.back());
-          }
-          if (worst <= p->tol) {
-            done = true;
-            break;
-          }
-          if (sweep >= mp->max_refine) break;
-          ++sweep;
-          const double far = theta.back(); // last kept Ritz value
-          if (asc)
-            rf.filter(mp->cheb_degree, far, hi_all + margin, theta.front());
-          else
-            rf.filter(mp->cheb_degree, lo_all - margin, far, theta.front());
-        }
+        done = refine_sweeps<double, B>(rf, c, n, *op64, mass64, p->nev, p->tol, mp->max_refine, mp->cheb_degree, lo_all, hi_all, theta, rs, sweep, "fp64");
         if (done) {
           for (unsigned i = 0; i < p->nev; ++i) evals[i] = theta[i];
           if (X) throw_on(trl_hip_memcpy_d2d(c.get(), X, rf.X, std::size_t(nblk) * n * B * sizeof(double)), "copy X");

@@ -6,7 +6,7 @@ or hardware counters) sees clean, repeated dispatches of one kernel.
 
 Kernels (--what, comma separated):
   spmm32, spmm64        Y = A X with the autotuned operator (FP32 / FP64)
-  spmm_axpby64          Y = alpha A X + beta X + gamma Z (the FP64 Chebyshev step)
+  spmm_axpby32/64       Y = alpha A X + beta X + gamma Z (the Chebyshev step, FP32 / FP64)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
   cholqr32, cholqr64    CholQR2 of one FP32 panel, CholQR of one FP64 panel
   copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
@@ -32,7 +32,7 @@ def main(argv=None) -> int:
     ap.add_argument("--side", type=int, default=160)
     ap.add_argument("--b", type=int, default=8)
     ap.add_argument("--reps", type=int, default=20)
-    ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby64,gram32,update32,cholqr32,copy")
+    ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,gram32,update32,cholqr32,cholqr64,copy")
     a = ap.parse_args(argv)
     what = a.what.split(",")
     cfg = P.config_c3(a.side)
@@ -57,19 +57,19 @@ def main(argv=None) -> int:
               f"{bytes_per_launch / avg / 1e9:7.0f} GB/s", flush=True)
 
     for w in what:
-        if w in ("spmm32", "spmm64", "spmm_axpby64"):
-            p = "f32" if w == "spmm32" else "f64"
+        if w in ("spmm32", "spmm64", "spmm_axpby32", "spmm_axpby64"):
+            p = "f32" if w in ("spmm32", "spmm_axpby32") else "f64"
             dt = np.float32 if p == "f32" else np.float64
             op = dA.operator(b, p, tune=True)
             s = np.dtype(dt).itemsize
             X = ctx.to_device(rng.standard_normal((n, b)).astype(dt))
             Y = ctx.empty((n, b), dt)
             mat = (op.desc.width * n * (4 + s)) if op.desc.format == 1 else (A.nnz * (4 + s) + (n + 1) * 4)
-            if w == "spmm_axpby64":
-                Z = ctx.to_device(rng.standard_normal((n, b)))
+            if w.startswith("spmm_axpby"):
+                Z = ctx.to_device(rng.standard_normal((n, b)).astype(dt))
 
-                def fn(op=op, X=X, Y=Y, Z=Z):
-                    L.check(H.trl_hip_spmm_axpby(ctx.h, L.F64, n, b, C.byref(op.desc), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125), "axpby")
+                def fn(op=op, X=X, Y=Y, Z=Z, dt=dt):
+                    L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), n, b, C.byref(op.desc), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125), "axpby")
 
                 timed(f"{w}[{op.name}]", fn, mat + 3 * n * b * s, "spmm")
             else:
