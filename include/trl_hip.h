@@ -257,11 +257,13 @@ int trl_hip_batch_create(trl_hip_ctx* ctx, int nprob, const size_t* n, const int
                          int b, int ncv, trl_hip_batch** out);
 int trl_hip_batch_destroy(trl_hip_batch* batch);
 /* Kernel variant: format 0 CSR / 1 ELL, rows_per_thread (tile = 256 x that
- * many rows); TRL_HIP_ERANGE if the grid would not fit on the device at once. */
+ * many rows); tiles cover the active problems only; TRL_HIP_ERANGE if the
+ * grid would not fit on the device at once. */
 int trl_hip_batch_set_variant(trl_hip_batch* batch, int format, int rows_per_thread);
-/* Autotuner: times one full-width step of every variant that fits and keeps
- * the fastest (uses block m of the basis and the coefficient arrays as
- * scratch: call it before the first extend). */
+/* Autotuner: times one full-width step of the active problems for every
+ * variant that fits and keeps the fastest (uses block m of the basis and the
+ * coefficient arrays as scratch: call it before an extend that starts at
+ * block 0 or with the quasi-Lanczos step). */
 int trl_hip_batch_tune(trl_hip_batch* batch, double* best_us);
 int trl_hip_batch_info(const trl_hip_batch* batch, int* ntiles, int* format, int* rows_per_thread, double* tuned_us);
 /* cholqr_passes 1 (as the reference's orthonormalize) or 2; use_graph 0
@@ -283,6 +285,10 @@ int trl_hip_batch_extend(trl_hip_batch* batch, int k, int m_end, int quasi, cons
  * sum_{q<m} V_q Y_p(q, j), block kr becomes V_m (lanczos.hh:105-118).  Y:
  * nprob row-major (m b) x (kr b) matrices. */
 int trl_hip_batch_rotate(trl_hip_batch* batch, const double* Y, int kr);
+/* Copies blocks 0..nblk-1 of problem p out of the basis into the batch's
+ * result array (done when p finishes: later steps and restarts only cover
+ * active problems); trl_hip_batch_get_block then reads from there. */
+int trl_hip_batch_save_blocks(trl_hip_batch* batch, int p, int nblk);
 /* ||A x - lambda x|| / ||A x|| of the pairs (block q column c of every
  * problem, lambda[(p*nblk + q)*b + c]) into out (same indexing). */
 int trl_hip_batch_residuals(trl_hip_batch* batch, int nblk, const double* lambda, double* out);

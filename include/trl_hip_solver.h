@@ -119,6 +119,9 @@ typedef struct {
   unsigned graph_launches;
   int ntiles, format, rows_per_thread;
   double tuned_us;       /* the chosen variant's time for one full-width step  */
+  double extend_seconds; /* of the loop: the replayed step graphs (device)     */
+  double rotate_seconds; /*              the restart rotations                 */
+  unsigned retiles;      /* tilings rebuilt as problems finished               */
 } trl_hip_batch_stats;
 
 /* nprob standard FP64 problems A_p x = lambda x (host CSR arrays, host start

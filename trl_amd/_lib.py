@@ -91,6 +91,7 @@ HIP_API = [
     ("trl_hip_batch_set_active", c_int, [c_void_p, c_void_p]),
     ("trl_hip_batch_extend", c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_batch_rotate", c_int, [c_void_p, c_void_p, c_int]),
+    ("trl_hip_batch_save_blocks", c_int, [c_void_p, c_int, c_int]),
     ("trl_hip_batch_residuals", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_batch_reset_flags", c_int, [c_void_p]),
     ("trl_hip_sddmm_adjoint", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -284,7 +285,8 @@ class BatchParams(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("host_seconds", C.c_double), ("setup_seconds", C.c_double), ("cycles", C.c_uint),
-                ("graph_launches", C.c_uint), ("ntiles", c_int), ("format", c_int), ("rows_per_thread", c_int), ("tuned_us", C.c_double)]
+                ("graph_launches", C.c_uint), ("ntiles", c_int), ("format", c_int), ("rows_per_thread", c_int), ("tuned_us", C.c_double),
+                ("extend_seconds", C.c_double), ("rotate_seconds", C.c_double), ("retiles", C.c_uint)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -100,9 +100,13 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
   std::vector<int> pflag(nprob), active(nprob, 1);
   const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   unsigned cycles = 0, launches = 0;
-  double host_s = 0.0;
+  double host_s = 0.0, extend_s = 0.0, rotate_s = 0.0;
   unsigned k = 0;
   int quasi = 0;
+  const unsigned nblk = (nev + unsigned(b) - 1) / unsigned(b);
+  size_t rows_tiled = 0;
+  for (int p = 0; p < nprob; ++p) rows_tiled += n[p];
+  unsigned retiles = 0;
   const double t0 = now();
   const double setup_s = t0 - t_setup;
   auto Tblk = [&](Problem& q, unsigned i, unsigned j) { return q.T.data() + (size_t(i) * m + j) * BB; };
@@ -113,7 +117,9 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
       if (P[p].active) live.push_back(p);
     if (live.empty()) break;
     ++cycles;
+    const double te = now();
     if ((rc = trl_hip_batch_extend(g.b, int(k), int(m), quasi, quasi ? arrow.data() : nullptr, alpha.data(), beta.data(), pflag.data()))) return rc;
+    extend_s += now() - te;
     launches += 1;
     const double th = now();
     parallel_for(live, nthreads, [&](int p) {
@@ -161,15 +167,22 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
     // every problem still in the batch this cycle is rotated: the converged
     // ones get their Ritz vectors in blocks 0..kr-1, the others restart
     if ((rc = trl_hip_batch_set_active(g.b, active.data()))) return rc;
+    const double tr = now();
     if ((rc = trl_hip_batch_rotate(g.b, Y.data(), int(kr)))) return rc;
+    rotate_s += now() - tr;
     const double tdone = now();
     for (int p : live) {
       Problem& q = P[p];
-      if (q.error) continue;
+      if (q.error) {
+        if ((rc = trl_hip_batch_save_blocks(g.b, p, int(nblk)))) return rc;
+        continue;
+      }
       if (q.converged || !restart_possible || cycles >= prm->max_restarts) {
         q.active = false;
         active[p] = 0;
         q.t_done = tdone - t0;
+        // its Ritz vectors leave the basis: later cycles only cover active problems
+        if ((rc = trl_hip_batch_save_blocks(g.b, p, int(nblk)))) return rc;
         continue;
       }
       // T <- [diag(theta) arrow; arrow^T ...] (lanczos.hh:123-143)
@@ -192,13 +205,29 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
       std::memcpy(arrow.data() + size_t(p) * BB, Tblk(q, kr, kr - 1), BB * sizeof(double));
     }
     if ((rc = trl_hip_batch_set_active(g.b, active.data()))) return rc;
+    // fewer rows left: new tiles over the active problems only (re-tuned:
+    // with a few large problems left, smaller tiles give more workgroups)
+    size_t rows_active = 0;
+    for (int p = 0; p < nprob; ++p)
+      if (P[p].active) rows_active += n[p];
+    if (rows_active > 0 && 2 * rows_active < rows_tiled) {
+      if (prm->tune)
+        rc = trl_hip_batch_tune(g.b, nullptr);
+      else {
+        int fmt = 0, rpt = 0;
+        trl_hip_batch_info(g.b, nullptr, &fmt, &rpt, nullptr);
+        rc = trl_hip_batch_set_variant(g.b, fmt, rpt);
+      }
+      if (rc) return rc;
+      rows_tiled = rows_active;
+      ++retiles;
+    }
     k = kr;
     quasi = 1;
   }
   const double total = now() - t0;
 
   // results: Ritz values, vectors (blocks 0..nblk-1) and true residuals
-  const unsigned nblk = (nev + unsigned(b) - 1) / unsigned(b);
   std::vector<double> lam(size_t(nprob) * nblk * b, 0.0), rr(size_t(nprob) * nblk * b, 0.0);
   int first_error = 0;
   for (int p = 0; p < nprob; ++p) {
@@ -232,6 +261,9 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
     stats->setup_seconds = setup_s;
     stats->cycles = cycles;
     stats->graph_launches = launches;
+    stats->extend_seconds = extend_s;
+    stats->retiles = retiles;
+    stats->rotate_seconds = rotate_s;
     trl_hip_batch_info(g.b, &stats->ntiles, &stats->format, &stats->rows_per_thread, &stats->tuned_us);
   }
   return first_error;

@@ -473,14 +473,14 @@ __global__ __launch_bounds__(kT) void k_batch_rotate(BatchDev d, const double* _
 // problems): per tile and block, ||A x_c - lambda_c x_c||^2 and ||A x_c||^2
 // for every column c, stored at out[(t*nblk + q)*2B ...].
 template <int B, bool ELL>
-__global__ __launch_bounds__(kT) void k_batch_residual(BatchDev d, int nblk, const double* __restrict__ lam, double* __restrict__ out)
+__global__ __launch_bounds__(kT) void k_batch_residual(BatchDev d, const double* __restrict__ V, int nblk, const double* __restrict__ lam,
+                                                        double* __restrict__ out)
 {
   __shared__ double red[4 * 2 * B];
   const int t = blockIdx.x;
   if (t >= d.ntiles) return;
   const int p = d.tile_prob[t];
   const size_t NB = d.N * B;
-  const double* V = *d.vsel ? d.V1 : d.V0;
   for (int q = 0; q < nblk; ++q) {
     double acc[2 * B];
 #pragma unroll
@@ -542,6 +542,11 @@ struct trl_hip_batch {
   int cq_passes = 1;
   bool use_graph = true;
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+  // leading blocks of finished problems (nblk_saved blocks of N x b), kept
+  // out of the basis buffers so that the tiles need only cover active problems
+  double* Xres = nullptr;
+  int nblk_saved = 0;
+  std::vector<int> saved_h;
 };
 
 namespace {
@@ -634,14 +639,16 @@ bool valid_rpt(int rpt) { return rpt == 1 || rpt == 2 || rpt == 4 || rpt == 8 ||
 
 // Builds the tiles for rows-per-thread rpt and checks that the whole grid
 // fits on the device at once (the barriers need every tile resident).
-int build_tiles(trl_hip_batch* bt, int rpt)
+int build_tiles(trl_hip_batch* bt, int rpt, bool all = false)
 {
   const int rows = kT * rpt;
   std::vector<int> tp, r0, r1, loc, pn(bt->nprob), pt0(bt->nprob);
   for (int p = 0; p < bt->nprob; ++p) {
     pt0[p] = int(tp.size());
     const size_t np = bt->n[p];
-    const int cnt = int((np + rows - 1) / rows);
+    // active problems only (a step or rotation never touches the others);
+    // every problem for the residual pass
+    const int cnt = (all || bt->active_h[p]) ? int((np + rows - 1) / rows) : 0;
     pn[p] = cnt;
     for (int k = 0; k < cnt; ++k) {
       tp.push_back(p);
@@ -657,7 +664,8 @@ int build_tiles(trl_hip_batch* bt, int rpt)
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, kT, shm) == hipSuccess ? 0 : TRL_HIP_EINVAL;
   });
   if (rc) return rc;
-  if (occ < 1 || size_t(nt) > size_t(occ) * size_t(bt->ctx->num_cus)) return TRL_HIP_ERANGE;
+  // the step kernel's barriers need every tile resident at once
+  if (!all && (occ < 1 || size_t(nt) > size_t(occ) * size_t(bt->ctx->num_cus))) return TRL_HIP_ERANGE;
   dfree(bt->tile_prob);
   dfree(bt->tile_row0);
   dfree(bt->tile_row1);
@@ -694,6 +702,7 @@ int build_tiles(trl_hip_batch* bt, int rpt)
 // first; quasi: the first step is the quasi-Lanczos step).
 int enqueue_cycle(trl_hip_batch* bt, int k, int m_end, int quasi)
 {
+  if (bt->ntiles == 0) return 0;
   const BatchDev d = make_dev(bt);
   const size_t nc = size_t(bt->nprob) * (bt->m + 1);
   hipLaunchKernelGGL(k_zero_u32, dim3(unsigned(std::min<size_t>((nc + 255) / 256, 64))), dim3(256), 0, bt->ctx->stream, bt->counters, nc);
@@ -790,6 +799,7 @@ int trl_hip_batch_create(trl_hip_ctx* ctx, int nprob, const size_t* n, const int
       hipMemset(bt->beta_prev, 0, size_t(nprob) * BB * sizeof(double)) != hipSuccess)
     return fail(TRL_HIP_EINVAL);
   bt->active_h.assign(nprob, 1);
+  bt->saved_h.assign(nprob, 0);
   if (hipMemcpy(bt->active, bt->active_h.data(), nprob * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return fail(TRL_HIP_EINVAL);
   // default variant: CSR, the largest rows-per-thread whose grid fits
   bt->format = 0;
@@ -834,6 +844,7 @@ int trl_hip_batch_destroy(trl_hip_batch* bt)
   dfree(bt->Ybuf);
   dfree(bt->lam);
   dfree(bt->rsum);
+  dfree(bt->Xres);
   dfree(bt->pflag);
   dfree(bt->err);
   delete bt;
@@ -884,7 +895,9 @@ int trl_hip_batch_set_block(trl_hip_batch* bt, int p, int q, const double* host)
 int trl_hip_batch_get_block(trl_hip_batch* bt, int p, int q, double* host)
 {
   if (!bt || p < 0 || p >= bt->nprob || q < 0 || q > bt->m || !host) return TRL_HIP_EINVAL;
-  const double* src = bt->V[bt->vsel_h] + (size_t(q) * bt->N + bt->roff[p]) * bt->b;
+  const bool saved = bt->saved_h[p] && q < bt->nblk_saved;
+  const double* base = saved ? bt->Xres : bt->V[bt->vsel_h];
+  const double* src = base + (size_t(q) * bt->N + bt->roff[p]) * bt->b;
   TRL_HIP_TRY(hipMemcpyAsync(host, src, bt->n[p] * bt->b * sizeof(double), hipMemcpyDeviceToHost, bt->ctx->stream));
   TRL_HIP_TRY(hipStreamSynchronize(bt->ctx->stream));
   return 0;
@@ -893,10 +906,16 @@ int trl_hip_batch_get_block(trl_hip_batch* bt, int p, int q, double* host)
 int trl_hip_batch_set_active(trl_hip_batch* bt, const int* active)
 {
   if (!bt || !active) return TRL_HIP_EINVAL;
-  for (int p = 0; p < bt->nprob; ++p) bt->active_h[p] = active[p] ? 1 : 0;
+  bool need_tiles = false;
+  for (int p = 0; p < bt->nprob; ++p) {
+    bt->active_h[p] = active[p] ? 1 : 0;
+    if (bt->active_h[p] && bt->prob_ntiles_h[p] == 0) need_tiles = true;
+  }
   TRL_HIP_TRY(hipMemcpyAsync(bt->active, bt->active_h.data(), bt->nprob * sizeof(int), hipMemcpyHostToDevice, bt->ctx->stream));
   TRL_HIP_TRY(hipStreamSynchronize(bt->ctx->stream));
-  return 0;
+  // a problem that (re)joins needs tiles; finished ones keep theirs until the
+  // next retiling (trl_hip_batch_set_variant / trl_hip_batch_tune)
+  return need_tiles ? build_tiles(bt, bt->rpt) : 0;
 }
 
 int trl_hip_batch_extend(trl_hip_batch* bt, int k, int m_end, int quasi, const double* arrow, double* alpha, double* beta, int* pflag)
@@ -944,6 +963,7 @@ int trl_hip_batch_extend(trl_hip_batch* bt, int k, int m_end, int quasi, const d
 int trl_hip_batch_rotate(trl_hip_batch* bt, const double* Y, int kr)
 {
   if (!bt || !Y || kr < 1 || kr >= bt->m) return TRL_HIP_EINVAL;
+  if (bt->ntiles == 0) return 0;
   const int b = bt->b;
   const size_t ysz = size_t(bt->m) * b * kr * b;
   const size_t shm = ysz * sizeof(double);
@@ -977,25 +997,52 @@ int trl_hip_batch_rotate(trl_hip_batch* bt, const double* Y, int kr)
   return 0;
 }
 
+int trl_hip_batch_save_blocks(trl_hip_batch* bt, int p, int nblk)
+{
+  if (!bt || p < 0 || p >= bt->nprob || nblk < 1 || nblk > bt->m + 1) return TRL_HIP_EINVAL;
+  int rc = 0;
+  if (!bt->Xres) {
+    if ((rc = dalloc(&bt->Xres, size_t(nblk) * bt->N * bt->b))) return rc;
+    TRL_HIP_TRY(hipMemsetAsync(bt->Xres, 0, size_t(nblk) * bt->N * bt->b * sizeof(double), bt->ctx->stream));
+    bt->nblk_saved = nblk;
+  }
+  if (nblk > bt->nblk_saved) return TRL_HIP_EINVAL;
+  const double* V = bt->V[bt->vsel_h];
+  for (int q = 0; q < nblk; ++q) {
+    const size_t off = (size_t(q) * bt->N + bt->roff[p]) * bt->b;
+    TRL_HIP_TRY(hipMemcpyAsync(bt->Xres + off, V + off, bt->n[p] * bt->b * sizeof(double), hipMemcpyDeviceToDevice, bt->ctx->stream));
+  }
+  TRL_HIP_TRY(hipStreamSynchronize(bt->ctx->stream));
+  bt->saved_h[p] = 1;
+  return 0;
+}
+
 int trl_hip_batch_residuals(trl_hip_batch* bt, int nblk, const double* lambda, double* out)
 {
   if (!bt || nblk < 1 || nblk > bt->m + 1 || !lambda || !out) return TRL_HIP_EINVAL;
+  int rc = 0;
+  // every problem's leading blocks in the saved array (those still in the basis now)
+  for (int p = 0; p < bt->nprob; ++p)
+    if (!bt->saved_h[p] && (rc = trl_hip_batch_save_blocks(bt, p, nblk))) return rc;
+  if (nblk > bt->nblk_saved) return TRL_HIP_EINVAL;
+  // tiles over every problem for this pass (no barriers: no residency limit)
+  const int step_rpt = bt->rpt;
+  if ((rc = build_tiles(bt, step_rpt, true))) return rc;
   const int b = bt->b;
   hipStream_t s = bt->ctx->stream;
   const size_t nl = size_t(bt->nprob) * nblk * b;
   const size_t ns = size_t(bt->ntiles) * nblk * 2 * b;
   dfree(bt->lam);
   dfree(bt->rsum);
-  int rc = 0;
   if ((rc = dalloc(&bt->lam, nl)) || (rc = dalloc(&bt->rsum, ns))) return rc;
   TRL_HIP_TRY(hipMemcpyAsync(bt->lam, lambda, nl * sizeof(double), hipMemcpyHostToDevice, s));
   const BatchDev d = make_dev(bt);
   const bool ell = bt->format == 1;
 #define TRL_BRES(B_)                                                                                                                                        \
   if (ell)                                                                                                                                                  \
-    hipLaunchKernelGGL((k_batch_residual<B_, true>), dim3(unsigned(bt->ntiles)), dim3(kT), 0, s, d, nblk, bt->lam, bt->rsum);                             \
+    hipLaunchKernelGGL((k_batch_residual<B_, true>), dim3(unsigned(bt->ntiles)), dim3(kT), 0, s, d, bt->Xres, nblk, bt->lam, bt->rsum);                  \
   else                                                                                                                                                      \
-    hipLaunchKernelGGL((k_batch_residual<B_, false>), dim3(unsigned(bt->ntiles)), dim3(kT), 0, s, d, nblk, bt->lam, bt->rsum);
+    hipLaunchKernelGGL((k_batch_residual<B_, false>), dim3(unsigned(bt->ntiles)), dim3(kT), 0, s, d, bt->Xres, nblk, bt->lam, bt->rsum);
   switch (b) {
   case 1:
     TRL_BRES(1);
@@ -1030,6 +1077,7 @@ int trl_hip_batch_residuals(trl_hip_batch* bt, int nblk, const double* lambda, d
         const double* a = acc.data() + (size_t(p) * nblk + q) * 2 * b;
         out[(size_t(p) * nblk + q) * b + c] = std::sqrt(a[c]) / std::max(std::sqrt(a[b + c]), 1e-300);
       }
+  (void)build_tiles(bt, step_rpt); // back to the active problems' tiles
   return 0;
 }
 
