@@ -97,8 +97,8 @@ def cpu_baseline(cfg, fp64_cycles, budget_s):
 
 
 def dominant_kernel_rate(ctx, dA, b, dtype, reps=50) -> dict:
-    """One Chebyshev step Y = alpha A X + beta X + gamma Z with the autotuned
-    operator of the given panel dtype (the hottest kernel of the mixed solve),
+    """One Chebyshev step Y = alpha A X + beta X + gamma Z with the operator
+    variant autotuned for that fused form, of the given panel dtype (the hottest kernel of the mixed solve),
     timed alone with HIP events on the library stream; algorithmic bytes =
     the stored matrix entries (values + int32 indices) + X + Z + Y."""
     import ctypes as C
@@ -114,7 +114,7 @@ def dominant_kernel_rate(ctx, dA, b, dtype, reps=50) -> dict:
     H = L.hip()
 
     def step():
-        L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), dA.n, b, C.byref(op.desc), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125), "axpby")
+        L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), dA.n, b, C.byref(op.desc_axpby), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125), "axpby")
 
     for _ in range(3):
         step()
@@ -126,9 +126,10 @@ def dominant_kernel_rate(ctx, dA, b, dtype, reps=50) -> dict:
     avg = st["seconds"] / st["calls"]
     per = st["bytes"] / st["calls"]
     S = "double" if dtype == "f64" else "float"
-    name = f"k_spmm_rows<{S}, {b}, {max(1, -op.desc.variant)}, {'true' if op.desc.format else 'false'}, true>" if op.desc.variant <= 1 else \
-        f"k_spmm_lanes<{S}, {b}, {op.desc.variant}, true>"
-    return {"kernel": name, "operator": op.name, "launches": st["calls"], "avg_launch_us": avg * 1e6, "algorithmic_bytes_per_launch": per,
+    d = op.desc_axpby
+    name = f"k_spmm_rows<{S}, {b}, {max(1, -d.variant)}, {'true' if d.format else 'false'}, true>" if d.variant <= 1 else \
+        f"k_spmm_lanes<{S}, {b}, {d.variant}, true>"
+    return {"kernel": name, "operator": op.name_axpby, "launches": st["calls"], "avg_launch_us": avg * 1e6, "algorithmic_bytes_per_launch": per,
             "achieved": per / avg / 1e9}
 
 

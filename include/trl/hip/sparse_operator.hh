@@ -24,7 +24,7 @@ public:
   // Borrows rowptr[n+1], col[nnz], val[nnz] (device) and the optional row
   // scaling dscale[n]; variant as in trl_hip_spmat (0 = untuned default).
   SparseOperator(const Context& ctx, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* dscale = nullptr, int variant = 0)
-      : ctx_(ctx), n_(n), csr_{0, rowptr, 0, col, val, dscale, variant}, op_(csr_)
+      : ctx_(ctx), n_(n), csr_{0, rowptr, 0, col, val, dscale, variant}, op_(csr_), op_axpby_(csr_)
   {
   }
 
@@ -49,9 +49,10 @@ public:
         ell = &ell_desc_;
       }
     }
-    double us = 0.0;
+    double us = 0.0, us_axpby = 0.0;
     throw_on(trl_hip_spmm_tune(ctx_.get(), dtype, n_, b, &csr_, ell, &op_, &us), "spmm_tune");
-    if (op_.format == 0) ell_.reset(); // the ELL copy is only kept while it is used
+    throw_on(trl_hip_spmm_tune_axpby(ctx_.get(), dtype, n_, b, &csr_, ell, &op_axpby_, &us_axpby), "spmm_tune_axpby");
+    if (op_.format == 0 && op_axpby_.format == 0) ell_.reset(); // the ELL copy is only kept while it is used
     tuned_b_ = b;
     tuned_us_ = us;
     return us;
@@ -65,10 +66,10 @@ public:
     throw_on(trl_hip_spmm(ctx_.get(), dtype, n_, b, &d, X, Y), "apply");
   }
 
-  // Y = alpha diag(dscale) A X + beta X + gamma Z.
+  // Y = alpha diag(dscale) A X + beta X + gamma Z (with the variant tuned for this form).
   void axpby(int b, const S* X, S* Y, const S* Z, double alpha, double beta, double gamma) const
   {
-    throw_on(trl_hip_spmm_axpby(ctx_.get(), dtype, n_, b, &op_, X, Y, Z, alpha, beta, gamma), "apply_axpby");
+    throw_on(trl_hip_spmm_axpby(ctx_.get(), dtype, n_, b, &op_axpby_, X, Y, Z, alpha, beta, gamma), "apply_axpby");
   }
 
   // The same on npanels consecutive panels (panel j at j*n*b).
@@ -78,10 +79,11 @@ public:
   }
   void axpby_panels(int b, int npanels, const S* X, S* Y, const S* Z, double alpha, double beta, double gamma) const
   {
-    throw_on(trl_hip_spmm_axpby_panels(ctx_.get(), dtype, n_, b, &op_, npanels, X, Y, Z, alpha, beta, gamma), "apply_axpby_panels");
+    throw_on(trl_hip_spmm_axpby_panels(ctx_.get(), dtype, n_, b, &op_axpby_, npanels, X, Y, Z, alpha, beta, gamma), "apply_axpby_panels");
   }
 
   const trl_hip_spmat& descriptor() const { return op_; }
+  const trl_hip_spmat& axpby_descriptor() const { return op_axpby_; }
   std::size_t size() const { return n_; }
   int tuned_block_size() const { return tuned_b_; }
   double tuned_us() const { return tuned_us_; }
@@ -98,6 +100,7 @@ private:
   std::size_t n_;
   trl_hip_spmat csr_;
   trl_hip_spmat op_;
+  trl_hip_spmat op_axpby_;
   trl_hip_spmat ell_desc_{};
   std::shared_ptr<Ell> ell_;
   int tuned_b_ = 0;

@@ -201,6 +201,10 @@ int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_
  * mean time per call to *best_us (may be NULL). */
 int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
                       double* best_us);
+/* The same for the fused axpby form (trl_hip_spmm_axpby with Z), whose best
+ * variant can differ (it reads and writes more panel bytes per row). */
+int trl_hip_spmm_tune_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
+                            double* best_us);
 
 /* ---- small b x b blocks (MatrixBlockViewConcept, concepts.hh:26-39) ----- */
 

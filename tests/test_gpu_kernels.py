@@ -189,6 +189,16 @@ def test_spmm_tune(ctx, rng, p, b):
         Xd, Y = dev(ctx, X, dt), ctx.empty((A.n, b), dt)
         L.check(H().trl_hip_spmm(ctx.h, L.dtype_code(dt), A.n, b, C.byref(best), Xd.ptr, Y.ptr), "spmm")
         assert rel(Y.download(), oracle.spmm(A, X)) < TOL[p]
+        # the fused axpby form has its own tuner; its pick computes the same step
+        Z = rng.standard_normal((A.n, b)).astype(dt).astype(np.float64)
+        bax, usax = L.SpMat(), C.c_double(0)
+        L.check(H().trl_hip_spmm_tune_axpby(ctx.h, L.dtype_code(dt), A.n, b, C.byref(csr), C.byref(e) if e else None, C.byref(bax), C.byref(usax)),
+                "tune_axpby")
+        print("tuned axpby", p, b, "ell" if bax.format else "csr", bax.variant, usax.value)
+        assert usax.value > 0 and bax.format in ((0,) if e is None else (0, 1)) and bax.variant != 0
+        Zd, Y2 = dev(ctx, Z, dt), ctx.empty((A.n, b), dt)
+        L.check(H().trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), A.n, b, C.byref(bax), Xd.ptr, Y2.ptr, Zd.ptr, 0.5, -0.25, 0.125), "axpby")
+        assert rel(Y2.download(), 0.5 * oracle.spmm(A, X) - 0.25 * X + 0.125 * Z) < TOL[p]
 
 
 def test_spmm_empty_rows(ctx, rng):

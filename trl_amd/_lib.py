@@ -66,6 +66,7 @@ HIP_API = [
     ("trl_hip_csr_max_row", c_int, [c_void_p, c_size, c_void_p, c_int_p]),
     ("trl_hip_csr_to_ell", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_spmm_tune", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_spmat_p, c_spmat_p, c_double_p]),
+    ("trl_hip_spmm_tune_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_spmat_p, c_spmat_p, c_double_p]),
     ("trl_hip_spmm_csr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     ("trl_hip_spmm_csr_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.c_double,
                                        C.c_double, C.c_double]),

@@ -140,10 +140,12 @@ int trl_hip_ctx_create(int device, trl_hip_ctx** out)
   }
   // default-priority, non-blocking in-order stream
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  // cleared on the context stream itself, so that its first kernels see zeros
   if (e == hipSuccess) e = hipMalloc(&ctx->flags, 64 * sizeof(int));
-  if (e == hipSuccess) e = hipMemset(ctx->flags, 0, 64 * sizeof(int));
+  if (e == hipSuccess) e = hipMemsetAsync(ctx->flags, 0, 64 * sizeof(int), ctx->stream);
   if (e == hipSuccess) e = hipMalloc(&ctx->small, kSmallScratch * sizeof(double));
-  if (e == hipSuccess) e = hipMemset(ctx->small, 0, kSmallScratch * sizeof(double));
+  if (e == hipSuccess) e = hipMemsetAsync(ctx->small, 0, kSmallScratch * sizeof(double), ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     trl_hip_ctx_destroy(ctx);
     return static_cast<int>(e);
@@ -480,31 +482,39 @@ int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_
   return launch_csr_to_ell(ctx, t, n, rowptr, col, val, width, ell_col, ell_val);
 }
 
-int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
-                      double* best_us)
+} // extern "C"
+
+namespace {
+
+// Times every applicable variant of csr (and ell) for the plain product or
+// the fused axpby form on scratch panels; keeps the fastest.
+int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, bool axpby,
+                   trl_hip_spmat* best, double* best_us)
 {
   if (!ctx || !valid_b(b) || !best || check_spmat(n, csr) || csr->format != 0 || (ell && (check_spmat(n, ell) || ell->format != 1))) return TRL_HIP_EINVAL;
   *best = *csr;
   if (best_us) *best_us = 0.0;
   if (!n) return 0;
   const size_t bytes = n * size_t(b) * (t == TRL_HIP_F32 ? 4 : 8);
-  void *X = nullptr, *Y = nullptr;
+  void *X = nullptr, *Y = nullptr, *Z = nullptr;
   int rc = trl_hip_malloc(ctx, bytes, &X);
   if (!rc) rc = trl_hip_malloc(ctx, bytes, &Y);
+  if (!rc && axpby) rc = trl_hip_malloc(ctx, bytes, &Z);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = trl_hip_fill_zero(ctx, X, bytes);
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = TRL_HIP_ENOMEM;
   double best_t = 1e300;
   const int csr_variants[] = {1, -2, -4, 2, 4, 8};
   const int ell_variants[] = {1, -2, -4};
+  auto launch = [&](const trl_hip_spmat& A) {
+    return launch_spmm(ctx, t, n, b, A.format == 0 ? A.rowptr : nullptr, A.width, A.col, A.val, A.dscale, X, Y, Z, axpby, 0.5, -0.25, 0.125, A.variant);
+  };
   auto time_one = [&](const trl_hip_spmat& A) -> double {
     for (int w = 0; w < 2; ++w)
-      if (launch_spmm(ctx, t, n, b, A.format == 0 ? A.rowptr : nullptr, A.width, A.col, A.val, A.dscale, X, Y, nullptr, false, 0, 0, 0, A.variant))
-        return -1.0; // variant not applicable to this b / dtype
-    const int reps = 5;
+      if (launch(A)) return -1.0; // variant not applicable to this b / dtype
+    const int reps = 8;
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return -1.0;
-    for (int r = 0; r < reps; ++r)
-      launch_spmm(ctx, t, n, b, A.format == 0 ? A.rowptr : nullptr, A.width, A.col, A.val, A.dscale, X, Y, nullptr, false, 0, 0, 0, A.variant);
+    for (int r = 0; r < reps; ++r) launch(A);
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
@@ -533,7 +543,24 @@ int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const 
   if (e1) (void)hipEventDestroy(e1);
   if (X) trl_hip_free(ctx, X);
   if (Y) trl_hip_free(ctx, Y);
+  if (Z) trl_hip_free(ctx, Z);
   return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
+                      double* best_us)
+{
+  return spmm_tune_impl(ctx, t, n, b, csr, ell, false, best, best_us);
+}
+
+int trl_hip_spmm_tune_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
+                            double* best_us)
+{
+  return spmm_tune_impl(ctx, t, n, b, csr, ell, true, best, best_us);
 }
 
 int trl_hip_small_copy(trl_hip_ctx* ctx, int b, const double* src, double* dst, int transpose)

@@ -69,8 +69,9 @@ std::shared_ptr<SparseOperator<S>> make_operator(const Context& c, std::size_t n
   auto op = std::make_shared<SparseOperator<S>>(c, n, rowptr, col, val, minv, p.spmm_tune ? 0 : p.lanes_per_row);
   if (p.spmm_tune) op->tune(int(B));
   if (std::getenv("TRL_HIP_VERBOSE"))
-    std::fprintf(stderr, "[operator] %s b=%u format=%s variant=%d (%.1f us)\n", sizeof(S) == 4 ? "f32" : "f64", B, op->descriptor().format ? "ell" : "csr",
-                 op->descriptor().variant, op->tuned_us());
+    std::fprintf(stderr, "[operator] %s b=%u format=%s variant=%d (%.1f us), axpby format=%s variant=%d\n", sizeof(S) == 4 ? "f32" : "f64", B,
+                 op->descriptor().format ? "ell" : "csr", op->descriptor().variant, op->tuned_us(), op->axpby_descriptor().format ? "ell" : "csr",
+                 op->axpby_descriptor().variant);
   return op;
 }
 

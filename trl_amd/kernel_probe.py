@@ -64,14 +64,18 @@ def main(argv=None) -> int:
             s = np.dtype(dt).itemsize
             X = ctx.to_device(rng.standard_normal((n, b)).astype(dt))
             Y = ctx.empty((n, b), dt)
-            mat = (op.desc.width * n * (4 + s)) if op.desc.format == 1 else (A.nnz * (4 + s) + (n + 1) * 4)
+            def mat_bytes(d):
+                return (d.width * n * (4 + s)) if d.format == 1 else (A.nnz * (4 + s) + (n + 1) * 4)
+
+            mat = mat_bytes(op.desc)
             if w.startswith("spmm_axpby"):
                 Z = ctx.to_device(rng.standard_normal((n, b)).astype(dt))
 
                 def fn(op=op, X=X, Y=Y, Z=Z, dt=dt):
-                    L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), n, b, C.byref(op.desc), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125), "axpby")
+                    L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), n, b, C.byref(op.desc_axpby), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125),
+                            "axpby")
 
-                timed(f"{w}[{op.name}]", fn, mat + 3 * n * b * s, "spmm")
+                timed(f"{w}[{op.name_axpby}]", fn, mat_bytes(op.desc_axpby) + 3 * n * b * s, "spmm")
             else:
                 timed(f"{w}[{op.name}]", lambda op=op, X=X, Y=Y: op.apply(X, Y), mat + 2 * n * b * s, "spmm")
         elif w in ("gram32", "update32"):

@@ -70,6 +70,8 @@ class DeviceOperator:
         self.ell = None
         self.us = None
         self.desc = csr
+        self.desc_axpby = csr
+        self.us_axpby = None
         if tune:
             w = C.c_int()
             L.check(H.trl_hip_csr_max_row(ctx.h, A.n, A.rowptr.ptr, C.byref(w)), "csr_max_row")
@@ -81,12 +83,20 @@ class DeviceOperator:
             best, us = L.SpMat(), C.c_double()
             L.check(H.trl_hip_spmm_tune(ctx.h, L.dtype_code(dt), A.n, b, C.byref(csr), C.byref(ell) if ell else None, C.byref(best), C.byref(us)), "spmm_tune")
             self.desc, self.us = best, us.value
-            if best.format == 0:
+            best_ax, us_ax = L.SpMat(), C.c_double()
+            L.check(H.trl_hip_spmm_tune_axpby(ctx.h, L.dtype_code(dt), A.n, b, C.byref(csr), C.byref(ell) if ell else None, C.byref(best_ax),
+                                              C.byref(us_ax)), "spmm_tune_axpby")
+            self.desc_axpby, self.us_axpby = best_ax, us_ax.value
+            if best.format == 0 and best_ax.format == 0:
                 self.ell = None
 
     @property
     def name(self) -> str:
         return f"{'ell' if self.desc.format else 'csr'}/v{self.desc.variant}"
+
+    @property
+    def name_axpby(self) -> str:
+        return f"{'ell' if self.desc_axpby.format else 'csr'}/v{self.desc_axpby.variant}"
 
     def bytes_per_apply(self) -> float:
         """Algorithmic HBM bytes of one apply: matrix arrays once, X read once, Y written."""
