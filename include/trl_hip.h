@@ -201,8 +201,9 @@ int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_
  * mean time per call to *best_us (may be NULL). */
 int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
                       double* best_us);
-/* The same for the fused axpby form (trl_hip_spmm_axpby with Z), whose best
- * variant can differ (it reads and writes more panel bytes per row). */
+/* The same for the fused axpby form (trl_hip_spmm_axpby with Z = Y, the
+ * Chebyshev filter's in-place step), whose best variant can differ (it reads
+ * and writes more panel bytes per row). */
 int trl_hip_spmm_tune_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
                             double* best_us);
 
@@ -325,6 +326,11 @@ int trl_hip_prof_read_dtype(trl_hip_ctx* ctx, int family, trl_hip_dtype t, long 
 /* Profiling mode: trl_hip_reorth calls with eta > 0 since the last enable,
  * and how many of them took the second pass. */
 int trl_hip_prof_reorth(trl_hip_ctx* ctx, long long* calls, long long* second_passes);
+/* Region timer: begin records a HIP event on the context stream, end records
+ * a second one, waits for it and returns the seconds between the two (the
+ * device time of everything queued in between, without per-call events). */
+int trl_hip_region_begin(trl_hip_ctx* ctx);
+int trl_hip_region_end(trl_hip_ctx* ctx, double* seconds);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,9 @@ typedef struct {
   unsigned fp32_sweeps;      /* refine = 1: at most this many filter sweeps in FP32
                                 panels first (0: none) ...                            */
   double tol_fp32_refine;    /* ... stopping once the FP32 residuals are below this   */
+  int degree_opt;            /* refine = 1: 1 gives each Ritz block its own filter
+                                degree from the measured convergence rate of its
+                                vectors (0 when converged), 0: cheb_degree on all  */
 } trl_hip_mixed_params;
 
 /* Mixed precision.  Phase 1: an FP32 thick-restart solve (FP32 basis, SpMM

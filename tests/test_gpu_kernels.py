@@ -464,6 +464,18 @@ def test_convert_and_fill(ctx, rng):
     assert not d.download().any()
 
 
+def test_region_timer(ctx):
+    # one event pair around queued work: positive, and ten times the work
+    # takes longer than one
+    d = ctx.empty((64 << 20,), np.float64)
+    t1 = ctx.time_region(lambda: L.check(H().trl_hip_fill_zero(ctx.h, d.ptr, d.nbytes), "fill"), 1)
+    t10 = ctx.time_region(lambda: L.check(H().trl_hip_fill_zero(ctx.h, d.ptr, d.nbytes), "fill"), 10) * 10
+    assert 0 < t1 < t10
+    s = C.c_double()
+    assert H().trl_hip_region_end(None, C.byref(s)) == -1
+    assert H().trl_hip_region_end(ctx.h, None) == -1
+
+
 def test_invalid_arguments(ctx):
     assert H().trl_hip_gram(ctx.h, L.F64, 10, 3, 1, None, None, None, None) == -1  # b = 3 unsupported
     assert H().trl_hip_spmm_csr(ctx.h, L.F64, 10, 4, None, None, None, None, None, None, 3) == -1

@@ -163,17 +163,20 @@ def test_mixed_refine_laplace3d(ctx, phase1_fp64, reorth):
     assert max_angle(Xh, ref["X"]) < 1e-6
 
 
-def test_mixed_fp32_refinement_sweeps(ctx):
+@pytest.mark.parametrize("degree_opt", [False, True])
+def test_mixed_fp32_refinement_sweeps(ctx, degree_opt):
     # FP32 Lanczos to a loose 5e-2, FP32 Chebyshev/Rayleigh-Ritz sweeps down to
-    # the FP32 floor, then FP64 sweeps: the same eigenpairs as the plain FP64 solve
+    # the FP32 floor, then FP64 sweeps (every block at the full degree, or each
+    # block at the degree its convergence rate asks for): the same eigenpairs
+    # as the plain FP64 solve
     c = P.config_c3(24)
     dA = DeviceCSR(ctx, c.csr)
     V0 = v0(ctx, c.csr.n, c.b)
     prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
     o64 = solve(dA, c.b, prm, V0, reorth="dgks")
     om = solve_mixed(dA, c.b, prm, V0, tol_fp32=5e-2, cheb_degree=60, refine_blocks=9, max_refine=14, fp32_sweeps=8, tol_fp32_refine=3e-5,
-                     reorth="dgks", spmm_tune=True)
-    print("mixed with fp32 sweeps", om.result, om.result32)
+                     reorth="dgks", spmm_tune=True, degree_opt=degree_opt)
+    print("mixed with fp32 sweeps", degree_opt, om.result, om.result32)
     assert om.result["converged"] == 1
     assert om.resid.max() <= 1e-8
     assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))

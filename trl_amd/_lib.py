@@ -80,6 +80,8 @@ HIP_API = [
     ("trl_hip_prof_read_dtype", c_int, [c_void_p, c_int, c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("trl_hip_prof_reorth", c_int, [c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     ("trl_hip_prof_enable", c_int, [c_void_p, c_int]),
+    ("trl_hip_region_begin", c_int, [c_void_p]),
+    ("trl_hip_region_end", c_int, [c_void_p, C.POINTER(C.c_double)]),
     ("trl_hip_prof_read", c_int, [c_void_p, c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("trl_hip_batch_create", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, C.POINTER(c_void_p)]),
     ("trl_hip_batch_destroy", c_int, [c_void_p]),
@@ -237,6 +239,16 @@ class Context:
             out[name] = {"calls": n.value, "seconds": s.value, "bytes": b.value}
         return out
 
+    def time_region(self, fn, reps: int) -> float:
+        """Mean device seconds per call of fn over reps calls, from one HIP
+        event pair on the context stream around all of them."""
+        check(hip().trl_hip_region_begin(self.h), "region_begin")
+        for _ in range(reps):
+            fn()
+        s = C.c_double()
+        check(hip().trl_hip_region_end(self.h, C.byref(s)), "region_end")
+        return s.value / reps
+
     def prof_reorth(self) -> dict:
         """trl_hip_reorth calls with the second-pass test since prof_enable, and
         how many took the second pass."""
@@ -276,7 +288,8 @@ class SolverResult(C.Structure):
 
 class MixedParams(C.Structure):
     _fields_ = [("tol_fp32", C.c_double), ("max_restarts_fp32", C.c_uint), ("seed", C.c_ulonglong), ("refine", c_int), ("cheb_degree", C.c_uint),
-                ("max_refine", C.c_uint), ("phase1_fp64", c_int), ("refine_blocks", C.c_uint), ("fp32_sweeps", C.c_uint), ("tol_fp32_refine", C.c_double)]
+                ("max_refine", C.c_uint), ("phase1_fp64", c_int), ("refine_blocks", C.c_uint), ("fp32_sweeps", C.c_uint), ("tol_fp32_refine", C.c_double),
+                ("degree_opt", c_int)]
 
 
 class BatchParams(C.Structure):

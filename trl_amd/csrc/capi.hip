@@ -173,6 +173,8 @@ int trl_hip_ctx_destroy(trl_hip_ctx* ctx)
     (void)hipEventDestroy(r.stop);
   }
   for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
+  for (auto e : ctx->region)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
@@ -487,7 +489,7 @@ int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_
 namespace {
 
 // Times every applicable variant of csr (and ell) for the plain product or
-// the fused axpby form on scratch panels; keeps the fastest.
+// the fused axpby form (with Z = Y) on scratch panels; keeps the fastest.
 int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, bool axpby,
                    trl_hip_spmat* best, double* best_us)
 {
@@ -499,7 +501,7 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
   void *X = nullptr, *Y = nullptr, *Z = nullptr;
   int rc = trl_hip_malloc(ctx, bytes, &X);
   if (!rc) rc = trl_hip_malloc(ctx, bytes, &Y);
-  if (!rc && axpby) rc = trl_hip_malloc(ctx, bytes, &Z);
+  Z = axpby ? Y : nullptr; // in place, as the Chebyshev filter calls it
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = trl_hip_fill_zero(ctx, X, bytes);
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = TRL_HIP_ENOMEM;
@@ -543,7 +545,6 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
   if (e1) (void)hipEventDestroy(e1);
   if (X) trl_hip_free(ctx, X);
   if (Y) trl_hip_free(ctx, Y);
-  if (Z) trl_hip_free(ctx, Z);
   return rc;
 }
 
@@ -596,6 +597,26 @@ int trl_hip_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, cons
 {
   if (!ctx || !valid_b(b) || !out || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   return launch_residual_sq(ctx, t, n, b, X, Y, mdiag, lambda, out);
+}
+
+int trl_hip_region_begin(trl_hip_ctx* ctx)
+{
+  if (!ctx) return TRL_HIP_EINVAL;
+  for (auto& e : ctx->region)
+    if (!e) TRL_HIP_TRY(hipEventCreate(&e));
+  TRL_HIP_TRY(hipEventRecord(ctx->region[0], ctx->stream));
+  return 0;
+}
+
+int trl_hip_region_end(trl_hip_ctx* ctx, double* seconds)
+{
+  if (!ctx || !seconds || !ctx->region[0]) return TRL_HIP_EINVAL;
+  TRL_HIP_TRY(hipEventRecord(ctx->region[1], ctx->stream));
+  TRL_HIP_TRY(hipEventSynchronize(ctx->region[1]));
+  float ms = 0.f;
+  TRL_HIP_TRY(hipEventElapsedTime(&ms, ctx->region[0], ctx->region[1]));
+  *seconds = double(ms) * 1e-3;
+  return 0;
 }
 
 int trl_hip_prof_enable(trl_hip_ctx* ctx, int on)

@@ -47,6 +47,8 @@ struct trl_hip_ctx {
   double prof_t_bytes[TRL_HIP_PROF_FAMILIES][2] = {};
   // profiling mode: trl_hip_reorth calls and how many took the second pass
   long long reorth_calls = 0, reorth_second = 0;
+  // region timer events (created on first use)
+  hipEvent_t region[2] = {nullptr, nullptr};
   std::unordered_map<const void*, long long> nnz_cache;
 };
 

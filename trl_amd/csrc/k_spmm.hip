@@ -21,6 +21,7 @@
 // which leaves the L2 to the gathered rows of X.  CSR index and value lines
 // are shared by consecutive loads of neighbouring lanes and stay cached.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hh"
 
@@ -29,6 +30,18 @@ using namespace trl_hip_dev;
 namespace {
 
 constexpr int kThreads = 256;
+// Workgroup size of the product kernels.  FP32 lanes holding at most 4
+// values of a row need under 64 registers, so two 1024-lane workgroups fit a
+// CU; so few workgroup launches (millions of rows, one or a few lanes each)
+// keep dispatch from limiting the memory stream (measured: 85 us against
+// 95 us with 256 lanes for the FP32 ELL product of a 4M-row 7-point stencil,
+// b = 8).  Wider lanes take 256: with one 1024-lane workgroup per CU the
+// whole CU waits for its slowest wave.
+template <typename S, int BC>
+constexpr int spmm_threads()
+{
+  return (sizeof(S) == 4 && BC <= 4) ? 1024 : 256;
+}
 constexpr unsigned kXcds = 8;
 constexpr int kChunk = 8;
 
@@ -54,23 +67,43 @@ struct PanelMap {
 };
 __device__ inline PanelMap panel_map(unsigned blk, unsigned npan) { return PanelMap{blk % npan, blk / npan}; }
 
-// AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z
-template <typename S, int BC, bool AXPBY>
-__device__ inline void epilogue(const S (&acc)[BC], int row, int B, int off, const S* __restrict__ X, S* Y, const Epi& e)
+// Address of element i of an array.  O32: i * sizeof(T) fits in 32 bits
+// (checked by the launcher), so the address is a 64-bit workgroup-uniform
+// base plus a 32-bit lane offset, which the global loads take as an SGPR
+// base and a VGPR offset: one VGPR per address instead of two, and a whole
+// ELL chunk of slot loads shares one offset register.
+template <bool O32>
+using Idx = std::conditional_t<O32, unsigned, size_t>;
+
+template <bool O32, typename T>
+__device__ inline T* elem(T* base, Idx<O32> i)
+{
+  if constexpr (O32) {
+    using C = std::conditional_t<std::is_const_v<T>, const char, char>;
+    return reinterpret_cast<T*>(reinterpret_cast<C*>(base) + i * unsigned(sizeof(T)));
+  }
+  else
+    return base + i;
+}
+
+// AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z.
+// ro = row * B + off, the first panel element of this lane.
+template <typename S, int BC, bool AXPBY, bool O32>
+__device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S* __restrict__ X, S* Y, const Epi& e)
 {
   const S* ds = static_cast<const S*>(e.dscale);
   S out[BC];
   if constexpr (!AXPBY) {
-    const S d = ds ? ld<true>(ds + row) : S(1);
+    const S d = ds ? ld<true>(elem<O32>(ds, row)) : S(1);
 #pragma unroll
     for (int q = 0; q < BC; ++q) out[q] = ds ? acc[q] * d : acc[q];
   }
   else {
     const S* Z = static_cast<const S*>(e.Z);
     S xr[BC], zr[BC];
-    RowIO<S, BC>::load(X + size_t(row) * B + off, xr);
-    if (Z) RowIO<S, BC>::template load<true>(Z + size_t(row) * B + off, zr);
-    const double a = ds ? e.alpha * double(ld<true>(ds + row)) : e.alpha;
+    RowIO<S, BC>::load(elem<O32>(X, ro), xr);
+    if (Z) RowIO<S, BC>::template load<true>(elem<O32>(Z, ro), zr);
+    const double a = ds ? e.alpha * double(ld<true>(elem<O32>(ds, row))) : e.alpha;
 #pragma unroll
     for (int q = 0; q < BC; ++q) {
       double v = a * double(acc[q]) + e.beta * double(xr[q]);
@@ -78,13 +111,14 @@ __device__ inline void epilogue(const S (&acc)[BC], int row, int B, int off, con
       out[q] = S(v);
     }
   }
-  RowIO<S, BC>::template store<true>(Y + size_t(row) * B + off, out);
+  RowIO<S, BC>::template store<true>(elem<O32>(Y, ro), out);
 }
 
-template <typename S, int B, int CS, bool ELL, bool AXPBY>
-__global__ __launch_bounds__(kThreads) void k_spmm_rows(const int32_t* __restrict__ rowptr, int width, const int32_t* __restrict__ col,
+template <typename S, int B, int CS, bool ELL, bool AXPBY, bool O32>
+__global__ __launch_bounds__((spmm_threads<S, B / CS>())) void k_spmm_rows(const int32_t* __restrict__ rowptr, int width, const int32_t* __restrict__ col,
                                                         const S* __restrict__ val, const S* __restrict__ X, S* Y, Epi e, int n)
 {
+  using I = Idx<O32>;
   constexpr int BC = B / CS;
   const PanelMap pm = panel_map(xcd_block(blockIdx.x, gridDim.x), e.npan);
   const int gid = int(pm.wrow * blockDim.x + threadIdx.x);
@@ -97,35 +131,61 @@ __global__ __launch_bounds__(kThreads) void k_spmm_rows(const int32_t* __restric
   S acc[BC];
 #pragma unroll
   for (int c = 0; c < BC; ++c) acc[c] = S(0);
-  const int beg = ELL ? 0 : rowptr[row];
-  const int end = ELL ? width : rowptr[row + 1];
-  for (int p0 = beg; p0 < end; p0 += kChunk) {
-    int ci[kChunk];
-    S vi[kChunk];
+  if constexpr (ELL) {
+    // slot-major ELL: slot s of every row at s*n + row.  The slot count is
+    // uniform, so padding slots of the last chunk are skipped by every lane.
+    for (int s0 = 0; s0 < width; s0 += kChunk) {
+      int ci[kChunk];
+      S vi[kChunk];
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q) {
-      const bool ok = p0 + q < end;
-      const size_t p = ELL ? size_t(p0 + q) * n + row : size_t(p0 + q);
-      ci[q] = ok ? ld<ELL>(col + p) : row; // slot-major ELL lines are used once: stream them
-      vi[q] = ok ? ld<ELL>(val + p) : S(0);
+      for (int q = 0; q < kChunk; ++q)
+        if (s0 + q < width) {
+          const size_t base = size_t(s0 + q) * size_t(n);
+          ci[q] = ld<true>(elem<O32>(col + base, I(row))); // every line used by one load: stream it
+          vi[q] = ld<true>(elem<O32>(val + base, I(row)));
+        }
+      S x[kChunk][BC];
+#pragma unroll
+      for (int q = 0; q < kChunk; ++q)
+        if (s0 + q < width) RowIO<S, BC>::load(elem<O32>(X, I(ci[q]) * I(B) + I(off)), x[q]);
+#pragma unroll
+      for (int q = 0; q < kChunk; ++q)
+        if (s0 + q < width)
+#pragma unroll
+          for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
     }
-    S x[kChunk][BC];
+  }
+  else {
+    const int beg = rowptr[row];
+    const int end = rowptr[row + 1];
+    for (int p0 = beg; p0 < end; p0 += kChunk) {
+      int ci[kChunk];
+      S vi[kChunk];
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q) RowIO<S, BC>::load(X + size_t(ci[q]) * B + off, x[q]);
+      for (int q = 0; q < kChunk; ++q) {
+        const bool ok = p0 + q < end;
+        ci[q] = ok ? col[p0 + q] : row; // padding gathers the row's own X row with value 0
+        vi[q] = ok ? val[p0 + q] : S(0);
+      }
+      S x[kChunk][BC];
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q)
+      for (int q = 0; q < kChunk; ++q) RowIO<S, BC>::load(elem<O32>(X, I(ci[q]) * I(B) + I(off)), x[q]);
 #pragma unroll
-      for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
+      for (int q = 0; q < kChunk; ++q)
+#pragma unroll
+        for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
+    }
   }
   Epi ee = e;
   if (ee.Z) ee.Z = static_cast<const S*>(ee.Z) + po;
-  epilogue<S, BC, AXPBY>(acc, row, B, off, X, Y, ee);
+  epilogue<S, BC, AXPBY, O32>(acc, I(row), I(row) * I(B) + I(off), X, Y, ee);
 }
 
-template <typename S, int B, int L, bool AXPBY>
-__global__ __launch_bounds__(kThreads) void k_spmm_lanes(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
+template <typename S, int B, int L, bool AXPBY, bool O32>
+__global__ __launch_bounds__((spmm_threads<S, B>())) void k_spmm_lanes(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
                                                          const S* __restrict__ X, S* Y, Epi e, int n)
 {
+  using I = Idx<O32>;
   const PanelMap pm = panel_map(xcd_block(blockIdx.x, gridDim.x), e.npan);
   const int gid = int(pm.wrow * blockDim.x + threadIdx.x);
   const size_t po = size_t(pm.pan) * e.pstride;
@@ -142,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_lanes(const int32_t* __restri
     const int c = col[p];
     const S v = val[p];
     S x[B];
-    RowIO<S, B>::load(X + size_t(c) * B, x);
+    RowIO<S, B>::load(elem<O32>(X, I(c) * I(B)), x);
 #pragma unroll
     for (int q = 0; q < B; ++q) acc[q] = fma(v, x[q], acc[q]);
   }
@@ -153,39 +213,41 @@ __global__ __launch_bounds__(kThreads) void k_spmm_lanes(const int32_t* __restri
   if (sub != 0) return;
   Epi ee = e;
   if (ee.Z) ee.Z = static_cast<const S*>(ee.Z) + po;
-  epilogue<S, B, AXPBY>(acc, row, B, 0, X, Y, ee);
+  epilogue<S, B, AXPBY, O32>(acc, I(row), I(row) * I(B), X, Y, ee);
 }
 
-inline unsigned xcd_grid(size_t threads)
+inline unsigned xcd_grid(size_t threads, int per_block)
 {
-  size_t g = (threads + kThreads - 1) / kThreads;
+  size_t g = (threads + size_t(per_block) - 1) / size_t(per_block);
   return unsigned((g + kXcds - 1) / kXcds * kXcds); // a multiple of the XCD count so the renumbering applies
 }
 
 // variant: 1 = one row per lane; -2 / -4 = 2 / 4 lanes per row splitting the
 // columns; 2, 4, 8, 16 = lane groups splitting the nonzeros (CSR only).
-// rowptr == NULL selects ELL storage (width slots).
-template <typename S, int B, bool AXPBY>
+// rowptr == NULL selects ELL storage (width slots).  O32: byte offsets into
+// one panel and into each ELL array fit in 32 bits.
+template <typename S, int B, bool AXPBY, bool O32>
 int launch_rows(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, int width, const int32_t* col, const S* val, const S* X, S* Y, const Epi& e)
 {
   const bool ell = rowptr == nullptr;
   const int cs = variant < 0 ? -variant : 1;
   const size_t lanes = cs > 1 ? size_t(cs) : size_t(variant > 1 ? variant : 1);
-  const dim3 g{xcd_grid(n * lanes) * e.npan}, blk{unsigned(kThreads)};
   if (variant > 1) {
     if (ell) return TRL_HIP_EINVAL;
+    constexpr int T = spmm_threads<S, B>();
+    const dim3 g{xcd_grid(n * lanes, T) * e.npan}, blk{unsigned(T)};
     switch (variant) {
     case 2:
-      hipLaunchKernelGGL((k_spmm_lanes<S, B, 2, AXPBY>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
+      hipLaunchKernelGGL((k_spmm_lanes<S, B, 2, AXPBY, O32>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
       break;
     case 4:
-      hipLaunchKernelGGL((k_spmm_lanes<S, B, 4, AXPBY>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
+      hipLaunchKernelGGL((k_spmm_lanes<S, B, 4, AXPBY, O32>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
       break;
     case 8:
-      hipLaunchKernelGGL((k_spmm_lanes<S, B, 8, AXPBY>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
+      hipLaunchKernelGGL((k_spmm_lanes<S, B, 8, AXPBY, O32>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
       break;
     case 16:
-      hipLaunchKernelGGL((k_spmm_lanes<S, B, 16, AXPBY>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
+      hipLaunchKernelGGL((k_spmm_lanes<S, B, 16, AXPBY, O32>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
       break;
     default:
       return TRL_HIP_EINVAL;
@@ -193,7 +255,12 @@ int launch_rows(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, 
     TRL_HIP_LAUNCH_CHECK();
     return 0;
   }
-#define TRL_ROWS(CS_, ELL_) hipLaunchKernelGGL((k_spmm_rows<S, B, CS_, ELL_, AXPBY>), g, blk, 0, ctx->stream, rowptr, width, col, val, X, Y, e, int(n))
+#define TRL_ROWS(CS_, ELL_)                                                                                                                        \
+  do {                                                                                                                                             \
+    constexpr int T = spmm_threads<S, B / CS_>();                                                                                                  \
+    const dim3 g{xcd_grid(n * lanes, T) * e.npan}, blk{unsigned(T)};                                                                              \
+    hipLaunchKernelGGL((k_spmm_rows<S, B, CS_, ELL_, AXPBY, O32>), g, blk, 0, ctx->stream, rowptr, width, col, val, X, Y, e, int(n));            \
+  } while (0)
   if (cs == 1) {
     if (ell) TRL_ROWS(1, true);
     else TRL_ROWS(1, false);
@@ -220,6 +287,16 @@ int launch_rows(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, 
 #undef TRL_ROWS
   TRL_HIP_LAUNCH_CHECK();
   return 0;
+}
+
+template <typename S, int B, bool AXPBY>
+int launch_rows_any(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, int width, const int32_t* col, const S* val, const S* X, S* Y,
+                    const Epi& e)
+{
+  const size_t lim = size_t(UINT32_MAX);
+  const bool o32 = n * size_t(B) * sizeof(S) <= lim && (rowptr || size_t(width) * n * std::max<size_t>(sizeof(S), 4) <= lim);
+  if (o32) return launch_rows<S, B, AXPBY, true>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
+  return launch_rows<S, B, AXPBY, false>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
 }
 
 // untuned default: one row per lane; FP64 rows of 8+ columns split in two
@@ -295,8 +372,8 @@ int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_
       const S* vv = static_cast<const S*>(val);
       const S* x = static_cast<const S*>(X);
       S* y = static_cast<S*>(Y);
-      if (axpby) return launch_rows<S, B, true>(ctx, n, v, rowptr, width, col, vv, x, y, e);
-      return launch_rows<S, B, false>(ctx, n, v, rowptr, width, col, vv, x, y, e);
+      if (axpby) return launch_rows_any<S, B, true>(ctx, n, v, rowptr, width, col, vv, x, y, e);
+      return launch_rows_any<S, B, false>(ctx, n, v, rowptr, width, col, vv, x, y, e);
     });
   });
   return 0;

@@ -214,17 +214,21 @@ struct BlockRefiner {
     std::swap(X, spare);
   }
 
-  // Chebyshev filter of the given degree on every panel; damps [lo, hi],
-  // value 1 at a0.
-  void filter(unsigned degree, double lo, double hi, double a0)
+  // Chebyshev filter of degree deg[j] on panel j (0 leaves the panel as it
+  // is); damps [lo, hi], value 1 at a0.  Returns the operator applications.
+  std::size_t filter(const std::vector<unsigned>& deg, double lo, double hi, double a0)
   {
-    if (degree == 0) return;
     const double e = 0.5 * (hi - lo), cc = 0.5 * (hi + lo);
     const double s1 = e / (a0 - cc);
+    bool uniform = true;
+    for (unsigned d : deg) uniform = uniform && d == deg.front();
+    std::size_t applies = 0;
     // panel by panel: each panel's X stays cache-resident across the stencil
     // neighbours of a row range (all panels at once would multiply that
     // working set by nb)
     for (unsigned j = 0; j < nb; ++j) {
+      const unsigned degree = deg[j];
+      if (degree == 0) continue;
       S* prev = panel(X, j);
       S* cur = panel(spare, j);
       op.axpby(int(B), prev, cur, nullptr, s1 / e, -s1 * cc / e, 0.0);
@@ -235,22 +239,72 @@ struct BlockRefiner {
         std::swap(prev, cur);
         sg = sn;
       }
+      applies += degree;
+      // an odd degree leaves the result in the spare panel
+      if (degree % 2 == 1 && !uniform)
+        throw_on(trl_hip_memcpy_d2d(c.get(), panel(X, j), panel(spare, j), n * B * sizeof(S)), "filter copy");
     }
-    if (degree % 2 == 1) std::swap(X, spare); // odd degree leaves the result in the spare buffer
+    if (uniform && deg.front() % 2 == 1) std::swap(X, spare);
+    return applies;
   }
 };
 
+// Per-panel filter degrees for the next sweep (degree optimisation as in
+// ChASE: Winkelmann, Springer, Di Napoli, "ChASE: Chebyshev accelerated
+// subspace iteration eigensolver for sequences of Hermitian eigenvalue
+// problems", ACM TOMS 45(2), 2019, Sec. 4.2: each vector gets the degree
+// that brings its residual to the tolerance at its convergence rate).  The
+// rate of wanted vector i is measured: log(r_prev / r) over the degree its
+// panel had in the previous sweep.  A vector already at tol needs 0; one
+// without a measured rate needs the full degree.  Panels holding vectors
+// beyond nev (the buffer that keeps the filter cutoff away from the wanted
+// values) take the largest wanted degree.  The degrees aim at tol / 4, so
+// that a slightly optimistic rate still ends below tol.  Degrees are rounded
+// up to even.
+inline std::vector<unsigned> panel_degrees(unsigned nb, unsigned B, unsigned nev, double tol, unsigned degree, const std::vector<double>& rs,
+                                           const std::vector<double>& rs_prev, const std::vector<unsigned>& deg_prev)
+{
+  std::vector<unsigned> deg(nb, degree);
+  if (rs_prev.size() != rs.size() || deg_prev.size() != nb) return deg;
+  unsigned dmax = 0;
+  for (unsigned j = 0; j < nb; ++j) {
+    if ((j + 1) * B > nev) continue;
+    unsigned dj = 0;
+    for (unsigned i = j * B; i < (j + 1) * B; ++i) {
+      unsigned m = degree;
+      if (rs[i] <= tol)
+        m = 0;
+      else if (deg_prev[j] > 0 && rs_prev[i] > 1.5 * rs[i]) {
+        const double rate = std::log(rs_prev[i] / rs[i]) / double(deg_prev[j]);
+        const double need = std::log(4.0 * rs[i] / tol) / rate;
+        m = need < double(degree) ? unsigned(std::ceil(need)) : degree;
+      }
+      dj = std::max(dj, m);
+    }
+    deg[j] = std::min(degree, dj + (dj % 2));
+    dmax = std::max(dmax, deg[j]);
+  }
+  for (unsigned j = 0; j < nb; ++j)
+    if ((j + 1) * B > nev) deg[j] = dmax;
+  return deg;
+}
+
 // Refinement sweeps: orthonormalise, Rayleigh-Ritz, true residuals of the
 // first nev pairs; stop when all are <= tol or after max_sweeps filters.
-// Returns true on convergence; theta and rs hold the last values.
+// degree_opt: per-panel degrees (panel_degrees) instead of `degree` on every
+// panel from the second filter on.  Returns true on convergence; theta and
+// rs hold the last values, applies counts the filter's operator applications.
 template <class S, unsigned B>
 bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, const SparseOperator<S>& op, const S* mass, unsigned nev, double tol,
-                   unsigned max_sweeps, unsigned degree, double lo_all, double hi_all, std::vector<double>& theta, std::vector<double>& rs,
-                   unsigned& sweeps, const char* tag)
+                   unsigned max_sweeps, unsigned degree, bool degree_opt, double lo_all, double hi_all, std::vector<double>& theta,
+                   std::vector<double>& rs, unsigned& sweeps, std::size_t& applies, const char* tag)
 {
   const double margin = 0.01 * (hi_all - lo_all);
   rs.assign(nev, 0.0);
   sweeps = 0;
+  applies = 0;
+  std::vector<double> rs_prev;
+  std::vector<unsigned> deg_prev;
   for (;;) {
     rf.orthonormalize();
     rf.rayleigh_ritz(theta);
@@ -266,11 +320,19 @@ bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, cons
     if (worst <= tol) return true;
     if (sweeps >= max_sweeps) return false;
     ++sweeps;
+    std::vector<unsigned> deg = degree_opt ? panel_degrees(rf.nb, B, nev, tol, degree, rs, rs_prev, deg_prev) : std::vector<unsigned>(rf.nb, degree);
+    if (std::getenv("TRL_HIP_VERBOSE")) {
+      std::fprintf(stderr, "[refine %s] degrees", tag);
+      for (unsigned d : deg) std::fprintf(stderr, " %u", d);
+      std::fprintf(stderr, "\n");
+    }
     const double far = theta.back(); // last kept Ritz value
     if (rf.ascending)
-      rf.filter(degree, far, hi_all + margin, theta.front());
+      applies += rf.filter(deg, far, hi_all + margin, theta.front());
     else
-      rf.filter(degree, lo_all - margin, far, theta.front());
+      applies += rf.filter(deg, lo_all - margin, far, theta.front());
+    rs_prev = rs;
+    deg_prev = deg;
   }
 }
 
@@ -412,10 +474,11 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         throw_on(trl_hip_memcpy_d2d(c.get(), rf32.X, X32.data(), std::size_t(nref) * n * B * sizeof(float)), "copy X32");
         std::vector<double> th32, rs32;
         unsigned sw32 = 0;
-        refine_sweeps<float, B>(rf32, c, n, *op32, mass32, p->nev, mp->tol_fp32_refine, mp->fp32_sweeps, mp->cheb_degree, lo_all, hi_all, th32, rs32, sw32,
-                                "fp32");
+        std::size_t ap32 = 0;
+        refine_sweeps<float, B>(rf32, c, n, *op32, mass32, p->nev, mp->tol_fp32_refine, mp->fp32_sweeps, mp->cheb_degree, mp->degree_opt != 0, lo_all,
+                                hi_all, th32, rs32, sw32, ap32, "fp32");
         throw_on(trl_hip_memcpy_d2d(c.get(), X32.data(), rf32.X, std::size_t(nref) * n * B * sizeof(float)), "copy X32");
-        r32.n_op_apply += (sw32 + 1) * nref * 2 + sw32 * nref * mp->cheb_degree;
+        r32.n_op_apply += (sw32 + 1) * nref * 2 + ap32;
         r32.iterations += sw32;
       }
 
@@ -429,7 +492,9 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
           throw_on(trl_hip_convert(c.get(), TRL_HIP_F32, X32.data(), TRL_HIP_F64, rf.X, std::size_t(nref) * n * B), "convert X32");
         std::vector<double> theta, rs;
         unsigned sweep = 0;
-        done = refine_sweeps<double, B>(rf, c, n, *op64, mass64, p->nev, p->tol, mp->max_refine, mp->cheb_degree, lo_all, hi_all, theta, rs, sweep, "fp64");
+        std::size_t ap64 = 0;
+        done = refine_sweeps<double, B>(rf, c, n, *op64, mass64, p->nev, p->tol, mp->max_refine, mp->cheb_degree, mp->degree_opt != 0, lo_all, hi_all, theta,
+                                        rs, sweep, ap64, "fp64");
         if (done) {
           for (unsigned i = 0; i < p->nev; ++i) evals[i] = theta[i];
           if (X) throw_on(trl_hip_memcpy_d2d(c.get(), X, rf.X, std::size_t(nblk) * n * B * sizeof(double)), "copy X");
@@ -437,7 +502,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
             for (unsigned i = 0; i < p->nev; ++i) resid[i] = rs[i];
           res->converged = 1;
           res->iterations = sweep;
-          res->n_op_apply = (sweep + 1) * nref * 2 + sweep * nref * mp->cheb_degree;
+          res->n_op_apply = (sweep + 1) * nref * 2 + ap64;
         }
         else {
           // fall back to an FP64 Lanczos warm start from the refined block

@@ -6,12 +6,13 @@ or hardware counters) sees clean, repeated dispatches of one kernel.
 
 Kernels (--what, comma separated):
   spmm32, spmm64        Y = A X with the autotuned operator (FP32 / FP64)
-  spmm_axpby32/64       Y = alpha A X + beta X + gamma Z (the Chebyshev step, FP32 / FP64)
+  spmm_axpby32/64       Y = alpha A X + beta X + gamma Y (the Chebyshev step, FP32 / FP64)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
   cholqr32, cholqr64    CholQR2 of one FP32 panel, CholQR of one FP64 panel
   copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
-Prints, per kernel, the mean time per launch (HIP events) and the algorithmic
-bytes per launch.
+Prints, per kernel, the mean time per launch (one HIP event pair on the
+library stream around the timed launches, after --warmup untimed ones) and
+the algorithmic bytes per launch.
 """
 
 from __future__ import annotations
@@ -32,6 +33,7 @@ def main(argv=None) -> int:
     ap.add_argument("--side", type=int, default=160)
     ap.add_argument("--b", type=int, default=8)
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10, help="untimed launches before each kernel's timed ones")
     ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,gram32,update32,cholqr32,cholqr64,copy")
     a = ap.parse_args(argv)
     what = a.what.split(",")
@@ -43,16 +45,10 @@ def main(argv=None) -> int:
     dA = DeviceCSR(ctx, A)
     rng = np.random.default_rng(0)
 
-    def timed(name, fn, bytes_per_launch, family):
-        for _ in range(2):
+    def timed(name, fn, bytes_per_launch):
+        for _ in range(a.warmup):
             fn()
-        ctx.sync()
-        ctx.prof_enable(True)
-        for _ in range(a.reps):
-            fn()
-        st = ctx.prof_read()[family]
-        ctx.prof_enable(False)
-        avg = st["seconds"] / st["calls"]
+        avg = ctx.time_region(fn, a.reps)
         print(f"{name:14s} launches {a.reps:4d}  mean {avg * 1e6:9.1f} us  algorithmic {bytes_per_launch / 1e6:9.1f} MB  "
               f"{bytes_per_launch / avg / 1e9:7.0f} GB/s", flush=True)
 
@@ -69,15 +65,16 @@ def main(argv=None) -> int:
 
             mat = mat_bytes(op.desc)
             if w.startswith("spmm_axpby"):
-                Z = ctx.to_device(rng.standard_normal((n, b)).astype(dt))
+                Y = ctx.to_device(rng.standard_normal((n, b)).astype(dt))
 
-                def fn(op=op, X=X, Y=Y, Z=Z, dt=dt):
-                    L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), n, b, C.byref(op.desc_axpby), X.ptr, Y.ptr, Z.ptr, 0.5, -0.25, 0.125),
+                # Z = Y, as the Chebyshev filter calls it
+                def fn(op=op, X=X, Y=Y, dt=dt):
+                    L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), n, b, C.byref(op.desc_axpby), X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125),
                             "axpby")
 
-                timed(f"{w}[{op.name_axpby}]", fn, mat_bytes(op.desc_axpby) + 3 * n * b * s, "spmm")
+                timed(f"{w}[{op.name_axpby}]", fn, mat_bytes(op.desc_axpby) + 3 * n * b * s)
             else:
-                timed(f"{w}[{op.name}]", lambda op=op, X=X, Y=Y: op.apply(X, Y), mat + 2 * n * b * s, "spmm")
+                timed(f"{w}[{op.name}]", lambda op=op, X=X, Y=Y: op.apply(X, Y), mat + 2 * n * b * s)
         elif w in ("gram32", "update32"):
             k = 16
             V = ctx.to_device(rng.standard_normal((k + 1, n, b)).astype(np.float32))
@@ -86,10 +83,10 @@ def main(argv=None) -> int:
             P_ = n * b * 4
             L.check(H.trl_hip_gram(ctx.h, L.F32, n, b, k, V.ptr, W, None, Cd.ptr), "gram")
             if w == "gram32":
-                timed(w, lambda: L.check(H.trl_hip_gram(ctx.h, L.F32, n, b, k, V.ptr, W, None, Cd.ptr), "gram"), (k + 1) * P_, "gram")
+                timed(w, lambda: L.check(H.trl_hip_gram(ctx.h, L.F32, n, b, k, V.ptr, W, None, Cd.ptr), "gram"), (k + 1) * P_)
             else:
                 Cs = ctx.to_device(np.zeros(k * b * b))  # W unchanged: W -= V 0
-                timed(w, lambda: L.check(H.trl_hip_update(ctx.h, L.F32, n, b, k, V.ptr, Cs.ptr, W), "update"), (k + 2) * P_, "update")
+                timed(w, lambda: L.check(H.trl_hip_update(ctx.h, L.F32, n, b, k, V.ptr, Cs.ptr, W), "update"), (k + 2) * P_)
         elif w in ("cholqr32", "cholqr64"):
             f32 = w == "cholqr32"
             Wd = ctx.to_device(rng.standard_normal((n, b)).astype(np.float32 if f32 else np.float64))
@@ -97,23 +94,12 @@ def main(argv=None) -> int:
             passes = 2 if f32 else 1
             s_ = 4 if f32 else 8
             timed(w, lambda: L.check(H.trl_hip_cholqr(ctx.h, L.F32 if f32 else L.F64, n, b, Wd.ptr, None, R.ptr, passes), "cholqr"),
-                  (5 if passes == 2 else 3) * n * b * s_, "cholqr")
+                  (5 if passes == 2 else 3) * n * b * s_)
         elif w == "copy":
             cnt = 64 << 20  # 64 Mi doubles = 512 MiB each way
             src = ctx.to_device(np.ones(cnt))
             dst = ctx.empty((cnt,))
-            for _ in range(2):
-                L.check(H.trl_hip_convert(ctx.h, L.F64, src.ptr, L.F64, dst.ptr, cnt), "copy")
-            ctx.sync()
-            import time
-
-            t = time.perf_counter()
-            for _ in range(a.reps):
-                L.check(H.trl_hip_convert(ctx.h, L.F64, src.ptr, L.F64, dst.ptr, cnt), "copy")
-            ctx.sync()
-            avg = (time.perf_counter() - t) / a.reps
-            print(f"{'copy':14s} launches {a.reps:4d}  mean {avg * 1e6:9.1f} us  algorithmic {2 * cnt * 8 / 1e6:9.1f} MB  "
-                  f"{2 * cnt * 8 / avg / 1e9:7.0f} GB/s (wall clock, includes launch gaps)", flush=True)
+            timed("copy", lambda: L.check(H.trl_hip_convert(ctx.h, L.F64, src.ptr, L.F64, dst.ptr, cnt), "copy"), 2 * cnt * 8)
         else:
             print(f"unknown kernel {w}", file=sys.stderr)
             return 2
