@@ -5,9 +5,12 @@
 // (examples/openmp/csrevp.hh:36-44; one work-item per row in the SYCL form,
 // examples/sycl/csrevp.hh:159-187).  Here the same product has several
 // kernels (one row per lane, column-split rows, nonzero lane groups) and two
-// storage forms (CSR, slot-major ELL).  tune() times them for one block size
-// on this device and keeps the fastest; the ELL copy it may build is owned by
-// the operator.  The CSR arrays are borrowed.
+// storage forms (CSR, slot-major ELL), visited in index order or plane block
+// by plane block.  tune() times them for one block size on this device and
+// keeps the fastest, for the plain product and for the fused axpby step of
+// a Chebyshev filter; tune_panels() does the same for that step on several
+// panels per launch.  The ELL copy tune() may build is owned by the
+// operator.  The CSR arrays are borrowed.
 
 #include <cstdint>
 #include <memory>
@@ -24,7 +27,7 @@ public:
   // Borrows rowptr[n+1], col[nnz], val[nnz] (device) and the optional row
   // scaling dscale[n]; variant as in trl_hip_spmat (0 = untuned default).
   SparseOperator(const Context& ctx, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* dscale = nullptr, int variant = 0)
-      : ctx_(ctx), n_(n), csr_{0, rowptr, 0, col, val, dscale, variant}, op_(csr_), op_axpby_(csr_)
+      : ctx_(ctx), n_(n), csr_{0, rowptr, 0, col, val, dscale, variant, 0, 0}, op_(csr_), op_axpby_(csr_), op_panels_(csr_)
   {
   }
 
@@ -45,18 +48,35 @@ public:
           ell_ = std::make_shared<Ell>(ctx_, n_, width);
           throw_on(trl_hip_csr_to_ell(ctx_.get(), dtype, n_, csr_.rowptr, csr_.col, csr_.val, width, ell_->col.data(), ell_->val.data()), "csr_to_ell");
         }
-        ell_desc_ = trl_hip_spmat{1, nullptr, width, ell_->col.data(), ell_->val.data(), csr_.dscale, 0};
+        ell_desc_ = trl_hip_spmat{1, nullptr, width, ell_->col.data(), ell_->val.data(), csr_.dscale, 0, 0, 0};
         ell = &ell_desc_;
       }
     }
-    double us = 0.0, us_axpby = 0.0;
+    double us = 0.0;
     throw_on(trl_hip_spmm_tune(ctx_.get(), dtype, n_, b, &csr_, ell, &op_, &us), "spmm_tune");
-    throw_on(trl_hip_spmm_tune_axpby(ctx_.get(), dtype, n_, b, &csr_, ell, &op_axpby_, &us_axpby), "spmm_tune_axpby");
+    throw_on(trl_hip_spmm_tune_axpby(ctx_.get(), dtype, n_, b, &csr_, ell, &op_axpby_, &axpby_us_), "spmm_tune_axpby");
     if (op_.format == 0 && op_axpby_.format == 0) ell_.reset(); // the ELL copy is only kept while it is used
     tuned_b_ = b;
     tuned_us_ = us;
+    panels_n_ = 0;
     return us;
   }
+
+  // After tune(b): times the axpby form on npanels panels at once (the
+  // matrix read once per launch) for axpby_panels.  Returns the time per
+  // launch in microseconds.
+  double tune_panels(int b, int npanels)
+  {
+    if (n_ == 0 || npanels < 2) return 0.0;
+    const trl_hip_spmat* ell = ell_ ? &ell_desc_ : nullptr;
+    throw_on(trl_hip_spmm_tune_panels(ctx_.get(), dtype, n_, b, npanels, &csr_, ell, &op_panels_, &panels_us_), "spmm_tune_panels");
+    panels_n_ = npanels;
+    return panels_us_;
+  }
+
+  // True when tune_panels(b, npanels) found one launch over the panels
+  // faster than npanels one-panel launches.
+  bool panels_pay(int npanels) const { return panels_n_ == npanels && panels_us_ < 0.97 * npanels * axpby_us_; }
 
   // Y = diag(dscale) A X (scaled = false: Y = A X).
   void apply(int b, const S* X, S* Y, bool scaled = true) const
@@ -79,11 +99,15 @@ public:
   }
   void axpby_panels(int b, int npanels, const S* X, S* Y, const S* Z, double alpha, double beta, double gamma) const
   {
-    throw_on(trl_hip_spmm_axpby_panels(ctx_.get(), dtype, n_, b, &op_axpby_, npanels, X, Y, Z, alpha, beta, gamma), "apply_axpby_panels");
+    const trl_hip_spmat& d = panels_n_ == npanels ? op_panels_ : op_axpby_;
+    throw_on(trl_hip_spmm_axpby_panels(ctx_.get(), dtype, n_, b, &d, npanels, X, Y, Z, alpha, beta, gamma), "apply_axpby_panels");
   }
 
   const trl_hip_spmat& descriptor() const { return op_; }
   const trl_hip_spmat& axpby_descriptor() const { return op_axpby_; }
+  const trl_hip_spmat& panels_descriptor() const { return op_panels_; }
+  double axpby_us() const { return axpby_us_; }
+  double panels_us() const { return panels_us_; }
   std::size_t size() const { return n_; }
   int tuned_block_size() const { return tuned_b_; }
   double tuned_us() const { return tuned_us_; }
@@ -101,10 +125,12 @@ private:
   trl_hip_spmat csr_;
   trl_hip_spmat op_;
   trl_hip_spmat op_axpby_;
+  trl_hip_spmat op_panels_;
   trl_hip_spmat ell_desc_{};
   std::shared_ptr<Ell> ell_;
   int tuned_b_ = 0;
-  double tuned_us_ = 0.0;
+  double tuned_us_ = 0.0, axpby_us_ = 0.0, panels_us_ = 0.0;
+  int panels_n_ = 0;
 };
 
 } // namespace trl::hip

@@ -149,7 +149,14 @@ int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
  * generalized problem with a diagonal M.  variant selects the kernel:
  *   0 default, 1 one row per lane, -2 / -4 two / four lanes per row splitting
  *   the b columns, 2 / 4 / 8 / 16 lane groups splitting the nonzeros (CSR).
- * Indices and values are loaded in chunks of 8 nonzeros before their gathers. */
+ * Indices and values are loaded in chunks of 8 nonzeros before their gathers.
+ * plane / plane_block select the order in which rows are visited (the
+ * product is the same): plane = 0 visits rows in index order; plane > 0
+ * views row r as (r / plane, r % plane) and visits blocks of plane_block
+ * consecutive rows of every plane in turn (block 0 of all planes, then
+ * block 1, ...).  For a matrix whose far off-diagonals sit +-plane away (a
+ * grid stencil), the rows of X that a block gathers are then reused while
+ * they are cached instead of two planes later.  0 < plane_block <= plane. */
 typedef struct {
   int format;
   const int32_t* rowptr;
@@ -158,6 +165,8 @@ typedef struct {
   const void* val;
   const void* dscale;
   int variant;
+  int plane;
+  int plane_block;
 } trl_hip_spmat;
 
 /* Y = diag(dscale) A X: EVP::apply of CSREVP (examples/openmp/csrevp.hh:36-44;
@@ -172,9 +181,10 @@ int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const
                        double beta, double gamma);
 
 /* The same two products on npanels panels at once (panel j of X, Y, Z at
- * j*n*b elements): the matrix entries of a row range are fetched once for
- * all panels (the workgroups of one XCD take every panel of a row range in
- * turn).  For the Chebyshev filter of a block of Ritz vectors. */
+ * j*n*b elements): each lane applies its matrix row to every panel in turn,
+ * so the matrix entries are fetched once for all panels (ELL rows of at most
+ * 8 slots keep them in registers).  For the Chebyshev filter of a block of
+ * Ritz vectors. */
 int trl_hip_spmm_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y);
 int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y, const void* Z,
                               double alpha, double beta, double gamma);
@@ -195,9 +205,11 @@ int trl_hip_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* 
 int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ell_col,
                        void* ell_val);
 /* Synchronous autotuner of the operator kernel for one matrix and block
- * size: times every applicable variant of `csr` and, if ell is not NULL, of
- * `ell` (same matrix in ELL form) with HIP events on scratch panels, and
- * writes the fastest descriptor to *best (dscale copied from csr) and its
+ * size: times kernel variants of `csr` and, if ell is not NULL, of `ell`
+ * (same matrix in ELL form), and blocked row orders (plane = the matrix
+ * bandwidth) for the fastest of them, with HIP events on scratch panels;
+ * a format whose first variant is far behind is not searched further.
+ * Writes the fastest descriptor to *best (dscale copied from csr) and its
  * mean time per call to *best_us (may be NULL). */
 int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
                       double* best_us);
@@ -206,6 +218,10 @@ int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const 
  * and writes more panel bytes per row). */
 int trl_hip_spmm_tune_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
                             double* best_us);
+/* The same for the axpby form on npanels panels at once
+ * (trl_hip_spmm_axpby_panels with Z = Y); *best_us is per launch. */
+int trl_hip_spmm_tune_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int npanels, const trl_hip_spmat* csr, const trl_hip_spmat* ell,
+                             trl_hip_spmat* best, double* best_us);
 
 /* ---- small b x b blocks (MatrixBlockViewConcept, concepts.hh:26-39) ----- */
 

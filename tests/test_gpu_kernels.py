@@ -199,6 +199,56 @@ def test_spmm_tune(ctx, rng, p, b):
         Zd, Y2 = dev(ctx, Z, dt), ctx.empty((A.n, b), dt)
         L.check(H().trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), A.n, b, C.byref(bax), Xd.ptr, Y2.ptr, Zd.ptr, 0.5, -0.25, 0.125), "axpby")
         assert rel(Y2.download(), 0.5 * oracle.spmm(A, X) - 0.25 * X + 0.125 * Z) < TOL[p]
+        # and the axpby form on three panels at once (in place, Z = Y)
+        bp, usp = L.SpMat(), C.c_double(0)
+        L.check(H().trl_hip_spmm_tune_panels(ctx.h, L.dtype_code(dt), A.n, b, 3, C.byref(csr), C.byref(e) if e else None, C.byref(bp), C.byref(usp)),
+                "tune_panels")
+        print("tuned panels", p, b, "ell" if bp.format else "csr", bp.variant, bp.plane, bp.plane_block, usp.value)
+        assert usp.value > 0 and bp.format in ((0,) if e is None else (0, 1))
+        X3 = rng.standard_normal((3, A.n, b)).astype(dt).astype(np.float64)
+        Z3 = rng.standard_normal((3, A.n, b)).astype(dt).astype(np.float64)
+        X3d, Z3d = dev(ctx, X3, dt), dev(ctx, Z3, dt)
+        L.check(H().trl_hip_spmm_axpby_panels(ctx.h, L.dtype_code(dt), A.n, b, C.byref(bp), 3, X3d.ptr, Z3d.ptr, Z3d.ptr, 0.5, -0.25, 0.125), "panels")
+        got = Z3d.download()
+        for j in range(3):
+            assert rel(got[j], 0.5 * oracle.spmm(A, X3[j]) - 0.25 * X3[j] + 0.125 * Z3[j]) < TOL[p]
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("fmt,variant", [("csr", 1), ("csr", -2), ("csr", 4), ("ell", 1), ("ell", -2), ("ell", -4)])
+def test_spmm_blocked_row_order(ctx, rng, p, fmt, variant):
+    # rows visited plane block by plane block give the same product, for
+    # blocks that do not divide the plane and planes that do not divide n
+    A = P.laplacian_3d(23, 19, 17, rng.uniform(0, 1, 23 * 19 * 17))
+    b, dt = 8, DT[p]
+    X = rng.standard_normal((A.n, b)).astype(dt).astype(np.float64)
+    Z = rng.standard_normal((A.n, b)).astype(dt).astype(np.float64)
+    d = rng.uniform(0.5, 2, A.n).astype(dt)
+    rp, ci, va, dd = dev(ctx, A.rowptr, np.int32), dev(ctx, A.col, np.int32), dev(ctx, A.val, dt), dev(ctx, d, dt)
+    if fmt == "ell":
+        width, hcol, hval = csr_to_ell(A)
+        M = spmat(1, None, width, dev(ctx, hcol, np.int32), dev(ctx, hval, dt), dd, variant)
+    else:
+        M = spmat(0, rp, 0, ci, va, dd, variant)
+    ref = oracle.spmm(A, X, d.astype(np.float64))
+    for plane, pb in [(23 * 19, 23 * 19), (23 * 19, 100), (23 * 19, 7), (500, 64), (A.n, 1000), (3 * A.n, 5)]:
+        M.plane, M.plane_block = plane, pb
+        Xd, Y = dev(ctx, X, dt), ctx.empty((A.n, b), dt)
+        L.check(H().trl_hip_spmm(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), Xd.ptr, Y.ptr), "spmm")
+        assert rel(Y.download(), ref) < TOL[p], (plane, pb)
+        Zd = dev(ctx, Z, dt)
+        L.check(H().trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), Xd.ptr, Zd.ptr, Zd.ptr, 0.5, -0.25, 0.125), "axpby")
+        assert rel(Zd.download(), 0.5 * ref - 0.25 * X + 0.125 * Z) < TOL[p], (plane, pb)
+        # two panels at once
+        X2 = np.stack([X, Z])
+        X2d, Y2 = dev(ctx, X2, dt), ctx.empty((2, A.n, b), dt)
+        L.check(H().trl_hip_spmm_panels(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), 2, X2d.ptr, Y2.ptr), "panels")
+        got = Y2.download()
+        assert rel(got[0], ref) < TOL[p] and rel(got[1], oracle.spmm(A, Z, d.astype(np.float64))) < TOL[p], (plane, pb)
+    # invalid orders are refused
+    for plane, pb in [(-1, 1), (100, 0), (100, 101)]:
+        M.plane, M.plane_block = plane, pb
+        assert H().trl_hip_spmm(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), Xd.ptr, Y.ptr) == -1
 
 
 def test_spmm_empty_rows(ctx, rng):

@@ -28,10 +28,11 @@ c_int_p = C.POINTER(C.c_int)
 class SpMat(C.Structure):
     """trl_hip_spmat: a device-resident sparse operator and its kernel variant.
 
-    format 0 = CSR (rowptr, col, val), 1 = slot-major ELL (width, col, val)."""
+    format 0 = CSR (rowptr, col, val), 1 = slot-major ELL (width, col, val);
+    plane / plane_block: blocked row order (0 = index order)."""
 
     _fields_ = [("format", c_int), ("rowptr", c_void_p), ("width", c_int), ("col", c_void_p), ("val", c_void_p), ("dscale", c_void_p),
-                ("variant", c_int)]
+                ("variant", c_int), ("plane", c_int), ("plane_block", c_int)]
 
 
 c_spmat_p = C.POINTER(SpMat)
@@ -67,6 +68,7 @@ HIP_API = [
     ("trl_hip_csr_to_ell", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_spmm_tune", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_spmat_p, c_spmat_p, c_double_p]),
     ("trl_hip_spmm_tune_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_spmat_p, c_spmat_p, c_double_p]),
+    ("trl_hip_spmm_tune_panels", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_spmat_p, c_spmat_p, c_spmat_p, c_double_p]),
     ("trl_hip_spmm_csr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     ("trl_hip_spmm_csr_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.c_double,
                                        C.c_double, C.c_double]),

@@ -395,6 +395,7 @@ int check_spmat(size_t n, const trl_hip_spmat* A)
   if (A->format == 1 && (A->width < 0 || (A->variant > 1))) return TRL_HIP_EINVAL;
   if (n && (A->format == 0 || A->width > 0) && (!A->col || !A->val)) return TRL_HIP_EINVAL;
   if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
+  if (A->plane < 0 || (A->plane > 0 && (A->plane_block < 1 || A->plane_block > A->plane))) return TRL_HIP_EINVAL;
   return 0;
 }
 
@@ -405,7 +406,7 @@ int trl_hip_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_h
   if (!ctx || !valid_b(b) || check_spmat(n, A) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
   Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 2) : 0.0, t);
-  return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, nullptr, false, 0, 0, 0, A->variant);
+  return launch_spmm(ctx, t, n, b, *A, X, Y, nullptr, false, 0, 0, 0);
 }
 
 int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, const void* X, void* Y, const void* Z, double alpha,
@@ -415,7 +416,7 @@ int trl_hip_spmm_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const
   if (!n) return 0;
   // algorithmic panels: X (gathered rows and the beta term read it once), Z, Y
   Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, Z ? 3 : 2) : 0.0, t);
-  return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant);
+  return launch_spmm(ctx, t, n, b, *A, X, Y, Z, true, alpha, beta, gamma);
 }
 
 int trl_hip_spmm_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y)
@@ -423,8 +424,7 @@ int trl_hip_spmm_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, cons
   if (!ctx || !valid_b(b) || npanels < 1 || check_spmat(n, A) || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   if (!n) return 0;
   Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + 2.0 * npanels * panel_bytes(t, n, b) : 0.0, t);
-  return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, nullptr, false, 0, 0, 0, A->variant,
-                     npanels);
+  return launch_spmm(ctx, t, n, b, *A, X, Y, nullptr, false, 0, 0, 0, npanels);
 }
 
 int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y, const void* Z,
@@ -437,28 +437,27 @@ int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b
   if (x < y + span && y < x + span) return TRL_HIP_EINVAL; // X and Y must not overlap
   if (!n) return 0;
   Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + (Z ? 3.0 : 2.0) * npanels * panel_bytes(t, n, b) : 0.0, t);
-  return launch_spmm(ctx, t, n, b, A->format == 0 ? A->rowptr : nullptr, A->width, A->col, A->val, A->dscale, X, Y, Z, true, alpha, beta, gamma, A->variant,
-                     npanels);
+  return launch_spmm(ctx, t, n, b, *A, X, Y, Z, true, alpha, beta, gamma, npanels);
 }
 
 int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
                      const void* dscale, int variant)
 {
-  const trl_hip_spmat A{0, rowptr, 0, col, val, dscale, variant};
+  const trl_hip_spmat A{0, rowptr, 0, col, val, dscale, variant, 0, 0};
   return trl_hip_spmm(ctx, t, n, b, &A, X, Y);
 }
 
 int trl_hip_spmm_csr_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* dscale,
                            const void* X, void* Y, const void* Z, double alpha, double beta, double gamma)
 {
-  const trl_hip_spmat A{0, rowptr, 0, col, val, dscale, 0};
+  const trl_hip_spmat A{0, rowptr, 0, col, val, dscale, 0, 0, 0};
   return trl_hip_spmm_axpby(ctx, t, n, b, &A, X, Y, Z, alpha, beta, gamma);
 }
 
 int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int width, const int32_t* ell_col, const void* ell_val, const void* X, void* Y,
                      const void* dscale)
 {
-  const trl_hip_spmat A{1, nullptr, width, ell_col, ell_val, dscale, 0};
+  const trl_hip_spmat A{1, nullptr, width, ell_col, ell_val, dscale, 0, 0, 0};
   return trl_hip_spmm(ctx, t, n, b, &A, X, Y);
 }
 
@@ -488,55 +487,110 @@ int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_
 
 namespace {
 
-// Times every applicable variant of csr (and ell) for the plain product or
-// the fused axpby form (with Z = Y) on scratch panels; keeps the fastest.
-int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, bool axpby,
+// Autotuner of the operator kernel: times candidate descriptors on scratch
+// panels (HIP events around a few back-to-back launches) for the plain
+// product, the fused axpby form (Z = Y, the Chebyshev filter's in-place
+// step) or that form on npanels panels at once, and keeps the fastest.
+// Candidates, pruned so that tuning stays a small part of a solve:
+//   - the CSR variants (1, -2, -4, 2, 4, 8), all of them only while the
+//     first is within 1.5x of the ELL default;
+//   - the ELL variants (1, -2, -4) when an ELL copy is given;
+//   - for the fastest of those in index order, blocked row orders (plane =
+//     the matrix bandwidth, blocks of about 1/4, 1/8, 1/16 of a plane).
+// Several panels: only ELL rows of one chunk share the matrix between
+// panels, so the ELL variants are tried in index order and in blocks of 1/8
+// and 1/16 of a plane.
+int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, bool axpby, int npanels,
                    trl_hip_spmat* best, double* best_us)
 {
-  if (!ctx || !valid_b(b) || !best || check_spmat(n, csr) || csr->format != 0 || (ell && (check_spmat(n, ell) || ell->format != 1))) return TRL_HIP_EINVAL;
+  if (!ctx || !valid_b(b) || !best || npanels < 1 || check_spmat(n, csr) || csr->format != 0 || (ell && (check_spmat(n, ell) || ell->format != 1)))
+    return TRL_HIP_EINVAL;
   *best = *csr;
   if (best_us) *best_us = 0.0;
   if (!n) return 0;
-  const size_t bytes = n * size_t(b) * (t == TRL_HIP_F32 ? 4 : 8);
-  void *X = nullptr, *Y = nullptr, *Z = nullptr;
+  const size_t bytes = size_t(npanels) * n * size_t(b) * (t == TRL_HIP_F32 ? 4 : 8);
+  void *X = nullptr, *Y = nullptr;
   int rc = trl_hip_malloc(ctx, bytes, &X);
   if (!rc) rc = trl_hip_malloc(ctx, bytes, &Y);
-  Z = axpby ? Y : nullptr; // in place, as the Chebyshev filter calls it
+  const void* Z = axpby ? Y : nullptr; // in place, as the Chebyshev filter calls it
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (!rc) rc = trl_hip_fill_zero(ctx, X, bytes);
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = TRL_HIP_ENOMEM;
   double best_t = 1e300;
-  const int csr_variants[] = {1, -2, -4, 2, 4, 8};
-  const int ell_variants[] = {1, -2, -4};
-  auto launch = [&](const trl_hip_spmat& A) {
-    return launch_spmm(ctx, t, n, b, A.format == 0 ? A.rowptr : nullptr, A.width, A.col, A.val, A.dscale, X, Y, Z, axpby, 0.5, -0.25, 0.125, A.variant);
-  };
   auto time_one = [&](const trl_hip_spmat& A) -> double {
-    for (int w = 0; w < 2; ++w)
-      if (launch(A)) return -1.0; // variant not applicable to this b / dtype
-    const int reps = 8;
+    (void)hipGetLastError();
+    if (launch_spmm(ctx, t, n, b, A, X, Y, Z, axpby, 0.5, -0.25, 0.125, npanels)) {
+      (void)hipGetLastError();
+      return -1.0; // variant not applicable to this b / dtype
+    }
+    const int reps = npanels > 1 ? 3 : 5;
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return -1.0;
-    for (int r = 0; r < reps; ++r) launch(A);
+    for (int r = 0; r < reps; ++r) (void)launch_spmm(ctx, t, n, b, A, X, Y, Z, axpby, 0.5, -0.25, 0.125, npanels);
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
     return 1e3 * ms / reps;
   };
-  for (int pass = 0; pass < 2 && !rc; ++pass) {
-    const trl_hip_spmat* base = pass == 0 ? csr : ell;
-    if (!base) continue;
-    const int* vs = pass == 0 ? csr_variants : ell_variants;
-    const int nv = pass == 0 ? 6 : 3;
-    for (int i = 0; i < nv; ++i) {
-      trl_hip_spmat A = *base;
-      A.variant = vs[i];
-      A.dscale = csr->dscale;
-      (void)hipGetLastError();
-      const double us = time_one(A);
-      (void)hipGetLastError();
-      if (us > 0.0 && us < best_t) {
-        best_t = us;
-        *best = A;
+  auto consider = [&](const trl_hip_spmat& A) -> double {
+    const double us = time_one(A);
+    if (us > 0.0 && us < best_t) {
+      best_t = us;
+      *best = A;
+    }
+    return us;
+  };
+  // the matrix bandwidth max |col - row|: the plane of the blocked row orders
+  int bw = 0;
+  if (!rc) {
+    int* d = ctx->flags + 16;
+    if (hipMemsetAsync(d, 0, sizeof(int), ctx->stream) != hipSuccess) rc = TRL_HIP_EINVAL;
+    if (!rc) rc = launch_csr_bandwidth(ctx, n, csr->rowptr, csr->col, d);
+    if (!rc && (hipMemcpyAsync(&bw, d, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess))
+      rc = TRL_HIP_EINVAL;
+  }
+  auto with = [&](const trl_hip_spmat& base, int variant, int plane, int block) {
+    trl_hip_spmat A = base;
+    A.variant = variant;
+    A.dscale = csr->dscale;
+    A.plane = plane;
+    A.plane_block = block;
+    return A;
+  };
+  // a blocked order with about 1/k of a plane per block (whole workgroups), or
+  // plane 0 when the far off-diagonals are within a few workgroups
+  auto block_of = [&](int variant, int k) {
+    // rows per workgroup: the one-panel kernels', or 256 lanes of the ELL panel kernel
+    const int R = npanels > 1 ? 256 / (variant < 0 ? -variant : 1) : spmm_rows_per_block(t, b, variant);
+    if (R <= 0 || bw < 8 * R || size_t(bw) >= n) return 0;
+    return std::min(bw, ((bw + k - 1) / k + R - 1) / R * R);
+  };
+  if (!rc && npanels > 1) {
+    if (ell)
+      for (int v : {-2, -4, 1}) {
+        consider(with(*ell, v, 0, 0));
+        for (int k : {8, 16}) {
+          const int pb = block_of(v, k);
+          if (pb) consider(with(*ell, v, bw, pb));
+        }
+      }
+    else
+      consider(with(*csr, 0, 0, 0));
+  }
+  else if (!rc) {
+    double ell_first = -1.0;
+    if (ell)
+      for (int v : {-2, 1, -4}) {
+        const double us = consider(with(*ell, v, 0, 0));
+        if (ell_first < 0.0) ell_first = us;
+      }
+    const double csr_first = consider(with(*csr, 1, 0, 0));
+    if (!(ell_first > 0.0 && csr_first > 1.5 * ell_first))
+      for (int v : {-2, -4, 2, 4, 8}) consider(with(*csr, v, 0, 0));
+    if (best_t < 1e300) {
+      const trl_hip_spmat base = *best;
+      for (int k : {4, 8, 16}) {
+        const int pb = block_of(base.variant, k);
+        if (pb) consider(with(base, base.variant, bw, pb));
       }
     }
   }
@@ -555,13 +609,19 @@ extern "C" {
 int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
                       double* best_us)
 {
-  return spmm_tune_impl(ctx, t, n, b, csr, ell, false, best, best_us);
+  return spmm_tune_impl(ctx, t, n, b, csr, ell, false, 1, best, best_us);
 }
 
 int trl_hip_spmm_tune_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
                             double* best_us)
 {
-  return spmm_tune_impl(ctx, t, n, b, csr, ell, true, best, best_us);
+  return spmm_tune_impl(ctx, t, n, b, csr, ell, true, 1, best, best_us);
+}
+
+int trl_hip_spmm_tune_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int npanels, const trl_hip_spmat* csr, const trl_hip_spmat* ell,
+                             trl_hip_spmat* best, double* best_us)
+{
+  return spmm_tune_impl(ctx, t, n, b, csr, ell, true, npanels, best, best_us);
 }
 
 int trl_hip_small_copy(trl_hip_ctx* ctx, int b, const double* src, double* dst, int transpose)

@@ -14,8 +14,13 @@
 // row's nonzeros sub, sub+L, ... and the partial rows are combined by a
 // butterfly (shfl_xor) inside the group; for long rows.
 //
-// Workgroups are renumbered so that each XCD walks a contiguous range of rows:
-// the stencil neighbours of a row (r +- 1, r +- nx, ...) then share that XCD's L2.
+// Several panels in one launch: every lane applies its row to each panel in
+// turn; a single-chunk ELL row keeps its indices and values in registers for
+// all of them, so the matrix is read once per launch, not once per panel.
+//
+// Workgroups are renumbered so that each XCD walks a contiguous range of rows
+// (in index order, or plane block by plane block, trl_hip_spmat::plane): the
+// stencil neighbours of a row (r +- 1, r +- nx, ...) then share that XCD's L2.
 // The ELL matrix stream (indices, values; every cache line used by one load),
 // the row scaling, Z and Y are touched once and use non-temporal accesses,
 // which leaves the L2 to the gathered rows of X.  CSR index and value lines
@@ -57,15 +62,30 @@ struct Epi {
   double alpha, beta, gamma;
   unsigned npan;  // panels of X, Y, Z processed by one launch
   size_t pstride; // elements between consecutive panels
+  // row order: plane = 0 index order; else blocks of pblock rows of each of
+  // the nz planes in turn (trl_hip_spmat::plane / plane_block)
+  unsigned plane, pblock, nz;
 };
 
-// Workgroup -> (row range, panel): consecutive workgroups in an XCD's
-// contiguous range take every panel of one row range before moving on, so
-// the matrix entries of those rows are read from HBM once and then hit L2.
-struct PanelMap {
-  unsigned pan, wrow;
-};
-__device__ inline PanelMap panel_map(unsigned blk, unsigned npan) { return PanelMap{blk % npan, blk / npan}; }
+// Row handled by local row slot `slot` (0 .. rows_per_wg-1) of row-range
+// workgroup wrow, or -1 when that slot has no row (past n, or past the end
+// of its plane block).  Blocked order: a segment is one block of one plane,
+// covered by ceil(pblock / rows_per_wg) consecutive workgroups; segments run
+// through the planes of block 0, then of block 1, ...
+__device__ inline int row_of(unsigned wrow, unsigned slot, unsigned rows_per_wg, int n, const Epi& e)
+{
+  if (e.plane == 0) {
+    const unsigned r = wrow * rows_per_wg + slot;
+    return r < unsigned(n) ? int(r) : -1;
+  }
+  const unsigned wps = (e.pblock + rows_per_wg - 1) / rows_per_wg;
+  const unsigned seg = wrow / wps, in_seg = (wrow % wps) * rows_per_wg + slot;
+  const unsigned pb = seg / e.nz, z = seg % e.nz;
+  const unsigned p = pb * e.pblock + in_seg;
+  if (in_seg >= e.pblock || p >= e.plane) return -1;
+  const size_t r = size_t(z) * e.plane + p;
+  return r < size_t(n) ? int(r) : -1;
+}
 
 // Address of element i of an array.  O32: i * sizeof(T) fits in 32 bits
 // (checked by the launcher), so the address is a 64-bit workgroup-uniform
@@ -86,10 +106,11 @@ __device__ inline T* elem(T* base, Idx<O32> i)
     return base + i;
 }
 
-// AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z.
-// ro = row * B + off, the first panel element of this lane.
+// AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z (Z
+// may be null).  X, Y, Z point at the lane's panel; ro = row * B + off, the
+// first element of this lane in it.
 template <typename S, int BC, bool AXPBY, bool O32>
-__device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S* __restrict__ X, S* Y, const Epi& e)
+__device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S* __restrict__ X, S* Y, const S* Z, const Epi& e)
 {
   const S* ds = static_cast<const S*>(e.dscale);
   S out[BC];
@@ -99,7 +120,6 @@ __device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, c
     for (int q = 0; q < BC; ++q) out[q] = ds ? acc[q] * d : acc[q];
   }
   else {
-    const S* Z = static_cast<const S*>(e.Z);
     S xr[BC], zr[BC];
     RowIO<S, BC>::load(elem<O32>(X, ro), xr);
     if (Z) RowIO<S, BC>::template load<true>(elem<O32>(Z, ro), zr);
@@ -116,18 +136,13 @@ __device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, c
 
 template <typename S, int B, int CS, bool ELL, bool AXPBY, bool O32>
 __global__ __launch_bounds__((spmm_threads<S, B / CS>())) void k_spmm_rows(const int32_t* __restrict__ rowptr, int width, const int32_t* __restrict__ col,
-                                                        const S* __restrict__ val, const S* __restrict__ X, S* Y, Epi e, int n)
+                                                                           const S* __restrict__ val, const S* __restrict__ X, S* Y, Epi e, int n)
 {
   using I = Idx<O32>;
   constexpr int BC = B / CS;
-  const PanelMap pm = panel_map(xcd_block(blockIdx.x, gridDim.x), e.npan);
-  const int gid = int(pm.wrow * blockDim.x + threadIdx.x);
-  const size_t po = size_t(pm.pan) * e.pstride;
-  X += po;
-  Y += po;
-  const int row = gid / CS;
-  const int off = (gid % CS) * BC;
-  if (row >= n) return;
+  const int row = row_of(xcd_block(blockIdx.x, gridDim.x), threadIdx.x / CS, blockDim.x / CS, n, e);
+  const int off = int(threadIdx.x % CS) * BC;
+  if (row < 0) return;
   S acc[BC];
 #pragma unroll
   for (int c = 0; c < BC; ++c) acc[c] = S(0);
@@ -176,24 +191,62 @@ __global__ __launch_bounds__((spmm_threads<S, B / CS>())) void k_spmm_rows(const
         for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
     }
   }
-  Epi ee = e;
-  if (ee.Z) ee.Z = static_cast<const S*>(ee.Z) + po;
-  epilogue<S, BC, AXPBY, O32>(acc, I(row), I(row) * I(B) + I(off), X, Y, ee);
+  epilogue<S, BC, AXPBY, O32>(acc, I(row), I(row) * I(B) + I(off), X, Y, static_cast<const S*>(e.Z), e);
+}
+
+// Several panels of an ELL matrix of at most kChunk slots: each lane loads
+// its row's indices and values once and applies them to every panel in turn
+// (panel p of X, Y, Z at p * pstride elements), so the matrix is read from
+// HBM once per launch instead of once per panel.  Working through the panels
+// one after another keeps one panel's X, Z and Y streams in flight at a time
+// (measured faster than spreading a workgroup's lanes over all panels at
+// once: nine concurrent write streams cost more than the matrix saves).
+template <typename S, int B, int CS, bool AXPBY, bool O32>
+__global__ __launch_bounds__(256) void k_spmm_ell_panels(int width, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X, S* Y,
+                                                         Epi e, int n)
+{
+  using I = Idx<O32>;
+  constexpr int BC = B / CS;
+  const int row = row_of(xcd_block(blockIdx.x, gridDim.x), threadIdx.x / CS, blockDim.x / CS, n, e);
+  const int off = int(threadIdx.x % CS) * BC;
+  if (row < 0) return;
+  const I ro = I(row) * I(B) + I(off);
+  const S* Z = static_cast<const S*>(e.Z);
+  int ci[kChunk];
+  S vi[kChunk];
+#pragma unroll
+  for (int q = 0; q < kChunk; ++q)
+    if (q < width) {
+      const size_t base = size_t(q) * size_t(n);
+      ci[q] = ld<true>(elem<O32>(col + base, I(row)));
+      vi[q] = ld<true>(elem<O32>(val + base, I(row)));
+    }
+  for (unsigned p = 0; p < e.npan; ++p) {
+    const size_t po = size_t(p) * e.pstride;
+    S x[kChunk][BC];
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q)
+      if (q < width) RowIO<S, BC>::load(elem<O32>(X + po, I(ci[q]) * I(B) + I(off)), x[q]);
+    S acc[BC];
+#pragma unroll
+    for (int c = 0; c < BC; ++c) acc[c] = S(0);
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q)
+      if (q < width)
+#pragma unroll
+        for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
+    epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, X + po, Y + po, Z ? Z + po : nullptr, e);
+  }
 }
 
 template <typename S, int B, int L, bool AXPBY, bool O32>
-__global__ __launch_bounds__((spmm_threads<S, B>())) void k_spmm_lanes(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const S* __restrict__ val,
-                                                         const S* __restrict__ X, S* Y, Epi e, int n)
+__global__ __launch_bounds__((spmm_threads<S, B>())) void k_spmm_lanes(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                                       const S* __restrict__ val, const S* __restrict__ X, S* Y, Epi e, int n)
 {
   using I = Idx<O32>;
-  const PanelMap pm = panel_map(xcd_block(blockIdx.x, gridDim.x), e.npan);
-  const int gid = int(pm.wrow * blockDim.x + threadIdx.x);
-  const size_t po = size_t(pm.pan) * e.pstride;
-  X += po;
-  Y += po;
-  const int row = gid / L;
+  const int row = row_of(xcd_block(blockIdx.x, gridDim.x), threadIdx.x / L, blockDim.x / L, n, e);
   const int sub = int(threadIdx.x) % L;
-  if (row >= n) return; // whole lane groups leave together
+  if (row < 0) return; // whole lane groups leave together
   S acc[B];
 #pragma unroll
   for (int c = 0; c < B; ++c) acc[c] = S(0);
@@ -211,15 +264,22 @@ __global__ __launch_bounds__((spmm_threads<S, B>())) void k_spmm_lanes(const int
 #pragma unroll
     for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
   if (sub != 0) return;
-  Epi ee = e;
-  if (ee.Z) ee.Z = static_cast<const S*>(ee.Z) + po;
-  epilogue<S, B, AXPBY, O32>(acc, I(row), I(row) * I(B), X, Y, ee);
+  epilogue<S, B, AXPBY, O32>(acc, I(row), I(row) * I(B), X, Y, static_cast<const S*>(e.Z), e);
 }
 
-inline unsigned xcd_grid(size_t threads, int per_block)
+// Row-range workgroups for rows_per_wg rows each (in the blocked order, a
+// whole number of workgroups per plane block), rounded up to a multiple of
+// the XCD count so that the renumbering applies.
+inline size_t row_ranges(size_t n, unsigned rows_per_wg, const Epi& e)
 {
-  size_t g = (threads + size_t(per_block) - 1) / size_t(per_block);
-  return unsigned((g + kXcds - 1) / kXcds * kXcds); // a multiple of the XCD count so the renumbering applies
+  size_t g;
+  if (e.plane == 0)
+    g = (n + rows_per_wg - 1) / rows_per_wg;
+  else {
+    const size_t npb = (size_t(e.plane) + e.pblock - 1) / e.pblock;
+    g = npb * e.nz * ((size_t(e.pblock) + rows_per_wg - 1) / rows_per_wg);
+  }
+  return (g + kXcds - 1) / kXcds * kXcds;
 }
 
 // variant: 1 = one row per lane; -2 / -4 = 2 / 4 lanes per row splitting the
@@ -235,7 +295,7 @@ int launch_rows(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, 
   if (variant > 1) {
     if (ell) return TRL_HIP_EINVAL;
     constexpr int T = spmm_threads<S, B>();
-    const dim3 g{xcd_grid(n * lanes, T) * e.npan}, blk{unsigned(T)};
+    const dim3 g{unsigned(row_ranges(n, T / unsigned(lanes), e))}, blk{unsigned(T)};
     switch (variant) {
     case 2:
       hipLaunchKernelGGL((k_spmm_lanes<S, B, 2, AXPBY, O32>), g, blk, 0, ctx->stream, rowptr, col, val, X, Y, e, int(n));
@@ -258,7 +318,7 @@ int launch_rows(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, 
 #define TRL_ROWS(CS_, ELL_)                                                                                                                        \
   do {                                                                                                                                             \
     constexpr int T = spmm_threads<S, B / CS_>();                                                                                                  \
-    const dim3 g{xcd_grid(n * lanes, T) * e.npan}, blk{unsigned(T)};                                                                              \
+    const dim3 g{unsigned(row_ranges(n, T / unsigned(lanes), e))}, blk{unsigned(T)};                                                     \
     hipLaunchKernelGGL((k_spmm_rows<S, B, CS_, ELL_, AXPBY, O32>), g, blk, 0, ctx->stream, rowptr, width, col, val, X, Y, e, int(n));            \
   } while (0)
   if (cs == 1) {
@@ -289,14 +349,59 @@ int launch_rows(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, 
   return 0;
 }
 
+// Several panels, ELL of at most kChunk slots, row variants (1, -2, -4).
+template <typename S, int B, bool AXPBY, bool O32>
+int launch_ell_panels(trl_hip_ctx* ctx, size_t n, int variant, int width, const int32_t* col, const S* val, const S* X, S* Y, const Epi& e)
+{
+  const unsigned cs = unsigned(variant < 0 ? -variant : 1);
+  const dim3 g{unsigned(row_ranges(n, 256 / cs, e))}, blk{256};
+  if (cs == 1)
+    hipLaunchKernelGGL((k_spmm_ell_panels<S, B, 1, AXPBY, O32>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));
+  else if (cs == 2) {
+    if constexpr (B % 2 == 0 && (B / 2) * sizeof(S) >= 8)
+      hipLaunchKernelGGL((k_spmm_ell_panels<S, B, 2, AXPBY, O32>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));
+    else
+      return TRL_HIP_EINVAL;
+  }
+  else if (cs == 4) {
+    if constexpr (B % 4 == 0 && (B / 4) * sizeof(S) >= 8)
+      hipLaunchKernelGGL((k_spmm_ell_panels<S, B, 4, AXPBY, O32>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));
+    else
+      return TRL_HIP_EINVAL;
+  }
+  else
+    return TRL_HIP_EINVAL;
+  TRL_HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+// One panel: the row or lane-group kernel.  Several panels: the ELL panel
+// kernel when every row fits one chunk, else one launch per panel.
+template <typename S, int B, bool AXPBY, bool O32>
+int launch_panels(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, int width, const int32_t* col, const S* val, const S* X, S* Y,
+                  const Epi& e)
+{
+  if (e.npan == 1) return launch_rows<S, B, AXPBY, O32>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
+  if (!rowptr && width <= kChunk && variant <= 1) return launch_ell_panels<S, B, AXPBY, O32>(ctx, n, variant, width, col, val, X, Y, e);
+  for (unsigned p = 0; p < e.npan; ++p) {
+    const size_t po = size_t(p) * e.pstride;
+    Epi ep = e;
+    ep.npan = 1;
+    if (ep.Z) ep.Z = static_cast<const S*>(ep.Z) + po;
+    const int rc = launch_rows<S, B, AXPBY, O32>(ctx, n, variant, rowptr, width, col, val, X + po, Y + po, ep);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 template <typename S, int B, bool AXPBY>
 int launch_rows_any(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr, int width, const int32_t* col, const S* val, const S* X, S* Y,
                     const Epi& e)
 {
   const size_t lim = size_t(UINT32_MAX);
   const bool o32 = n * size_t(B) * sizeof(S) <= lim && (rowptr || size_t(width) * n * std::max<size_t>(sizeof(S), 4) <= lim);
-  if (o32) return launch_rows<S, B, AXPBY, true>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
-  return launch_rows<S, B, AXPBY, false>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
+  if (o32) return launch_panels<S, B, AXPBY, true>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
+  return launch_panels<S, B, AXPBY, false>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
 }
 
 // untuned default: one row per lane; FP64 rows of 8+ columns split in two
@@ -311,6 +416,25 @@ __global__ __launch_bounds__(kThreads) void k_csr_max_row(const int32_t* __restr
   __shared__ int red[kThreads];
   int m = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, rowptr[i + 1] - rowptr[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < unsigned(s)) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
+
+// max |col - row| over the matrix (CSR with sorted columns: the first and
+// last entry of each row)
+__global__ __launch_bounds__(kThreads) void k_csr_bandwidth(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int n, int* __restrict__ out)
+{
+  __shared__ int red[kThreads];
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = rowptr[i], end = rowptr[i + 1];
+    if (end > beg) m = max(m, max(abs(col[beg] - i), abs(col[end - 1] - i)));
+  }
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = kThreads / 2; s > 0; s >>= 1) {
@@ -360,22 +484,54 @@ int launch_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t
 
 bool valid_spmm_variant(int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == -2 || v == -4; }
 
-int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, int width, const int32_t* col, const void* val, const void* dscale,
-                const void* X, void* Y, const void* Z, bool axpby, double alpha, double beta, double gamma, int variant, int npanels)
+int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat& A, const void* X, void* Y, const void* Z, bool axpby, double alpha,
+                double beta, double gamma, int npanels)
 {
-  if (npanels < 1 || size_t(npanels) * ((n * size_t(variant < 0 ? -variant : (variant > 1 ? variant : 1)) + 255) / 256 + 8) > size_t(UINT32_MAX))
-    return TRL_HIP_EINVAL;
-  const Epi e{dscale, Z, alpha, beta, gamma, unsigned(npanels), n * size_t(b)};
+  Epi e{A.dscale, Z, alpha, beta, gamma, unsigned(npanels), n * size_t(b), unsigned(A.plane), unsigned(A.plane_block), 0};
+  if (A.plane > 0) e.nz = unsigned((n + size_t(A.plane) - 1) / size_t(A.plane));
+  const int variant = A.variant;
+  // workgroup count bound: at least 256 lanes per workgroup
+  const unsigned lanes = unsigned(variant < 0 ? -variant : (variant > 1 ? variant : 1));
+  if (npanels < 1 || row_ranges(n, std::max(1u, 256u / lanes), e) > size_t(UINT32_MAX)) return TRL_HIP_EINVAL;
+  const int32_t* rowptr = A.format == 0 ? A.rowptr : nullptr;
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
       const int v = variant == 0 ? default_variant<S, B>() : variant;
-      const S* vv = static_cast<const S*>(val);
+      const S* vv = static_cast<const S*>(A.val);
       const S* x = static_cast<const S*>(X);
       S* y = static_cast<S*>(Y);
-      if (axpby) return launch_rows_any<S, B, true>(ctx, n, v, rowptr, width, col, vv, x, y, e);
-      return launch_rows_any<S, B, false>(ctx, n, v, rowptr, width, col, vv, x, y, e);
+      if (axpby) return launch_rows_any<S, B, true>(ctx, n, v, rowptr, A.width, A.col, vv, x, y, e);
+      return launch_rows_any<S, B, false>(ctx, n, v, rowptr, A.width, A.col, vv, x, y, e);
     });
   });
+  return 0;
+}
+
+int spmm_rows_per_block(trl_hip_dtype t, int b, int variant)
+{
+  TRL_DISPATCH_T(t, {
+    TRL_DISPATCH_B(b, {
+      const int v = variant == 0 ? default_variant<S, B>() : variant;
+      if (v > 1) return spmm_threads<S, B>() / v;
+      if (v == -2) {
+        if constexpr (B % 2 == 0) return spmm_threads<S, B / 2>() / 2;
+        return 0;
+      }
+      if (v == -4) {
+        if constexpr (B % 4 == 0) return spmm_threads<S, B / 4>() / 4;
+        return 0;
+      }
+      return spmm_threads<S, B>();
+    });
+  });
+  return 0;
+}
+
+int launch_csr_bandwidth(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, int* dev_out)
+{
+  hipLaunchKernelGGL(k_csr_bandwidth, dim3(unsigned(std::min<size_t>((n + kThreads - 1) / kThreads, 1024))), dim3(kThreads), 0, ctx->stream, rowptr, col,
+                     int(n), dev_out);
+  TRL_HIP_LAUNCH_CHECK();
   return 0;
 }
 

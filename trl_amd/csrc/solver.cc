@@ -68,11 +68,27 @@ std::shared_ptr<SparseOperator<S>> make_operator(const Context& c, std::size_t n
 {
   auto op = std::make_shared<SparseOperator<S>>(c, n, rowptr, col, val, minv, p.spmm_tune ? 0 : p.lanes_per_row);
   if (p.spmm_tune) op->tune(int(B));
-  if (std::getenv("TRL_HIP_VERBOSE"))
-    std::fprintf(stderr, "[operator] %s b=%u format=%s variant=%d (%.1f us), axpby format=%s variant=%d\n", sizeof(S) == 4 ? "f32" : "f64", B,
-                 op->descriptor().format ? "ell" : "csr", op->descriptor().variant, op->tuned_us(), op->axpby_descriptor().format ? "ell" : "csr",
-                 op->axpby_descriptor().variant);
+  if (std::getenv("TRL_HIP_VERBOSE")) {
+    const auto& d = op->descriptor();
+    const auto& x = op->axpby_descriptor();
+    std::fprintf(stderr, "[operator] %s b=%u %s/v%d/plane%d:%d (%.1f us), axpby %s/v%d/plane%d:%d (%.1f us)\n", sizeof(S) == 4 ? "f32" : "f64", B,
+                 d.format ? "ell" : "csr", d.variant, d.plane, d.plane_block, op->tuned_us(), x.format ? "ell" : "csr", x.variant, x.plane, x.plane_block,
+                 op->axpby_us());
+  }
   return op;
+}
+
+// Tunes the filter step of an operator on the nref panels of a refinement.
+template <class S>
+void tune_panels_of(SparseOperator<S>& op, unsigned B, unsigned nref)
+{
+  op.tune_panels(int(B), int(nref));
+  if (std::getenv("TRL_HIP_VERBOSE")) {
+    const auto& d = op.panels_descriptor();
+    std::fprintf(stderr, "[operator] %s %u panels %s/v%d/plane%d:%d (%.1f us per launch, %.1f us per panel; one panel %.1f us)%s\n",
+                 sizeof(S) == 4 ? "f32" : "f64", nref, d.format ? "ell" : "csr", d.variant, d.plane, d.plane_block, op.panels_us(), op.panels_us() / nref,
+                 op.axpby_us(), op.panels_pay(int(nref)) ? " used" : "");
+  }
 }
 
 // Runs one solve with panel type S and block size B.  If X_out is given it
@@ -223,9 +239,23 @@ struct BlockRefiner {
     bool uniform = true;
     for (unsigned d : deg) uniform = uniform && d == deg.front();
     std::size_t applies = 0;
-    // panel by panel: each panel's X stays cache-resident across the stencil
-    // neighbours of a row range (all panels at once would multiply that
-    // working set by nb)
+    // otherwise panel by panel
+    if (uniform && deg.front() > 0 && op.panels_pay(int(nb))) {
+      // every panel at once: the matrix is read once per step for all of them
+      const unsigned degree = deg.front();
+      S* prev = X;
+      S* cur = spare;
+      op.axpby_panels(int(B), int(nb), prev, cur, nullptr, s1 / e, -s1 * cc / e, 0.0);
+      double sg = s1;
+      for (unsigned i = 2; i <= degree; ++i) {
+        const double sn = 1.0 / (2.0 / s1 - sg);
+        op.axpby_panels(int(B), int(nb), cur, prev, prev, 2.0 * sn / e, -2.0 * sn * cc / e, -sg * sn);
+        std::swap(prev, cur);
+        sg = sn;
+      }
+      if (degree % 2 == 1) std::swap(X, spare);
+      return std::size_t(degree) * nb;
+    }
     for (unsigned j = 0; j < nb; ++j) {
       const unsigned degree = deg[j];
       if (degree == 0) continue;
@@ -470,6 +500,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       if (mp->refine == 1 && op32 && mp->fp32_sweeps > 0) {
         // ---- FP32 refinement sweeps: the same filter in FP32 panels (half
         // the bytes) until the FP32 residuals reach tol_fp32_refine ----------
+        if (p->spmm_tune) tune_panels_of(*op32, B, nref);
         BlockRefiner<float, B> rf32(c, n, *op32, mass32, nref, asc, eta);
         throw_on(trl_hip_memcpy_d2d(c.get(), rf32.X, X32.data(), std::size_t(nref) * n * B * sizeof(float)), "copy X32");
         std::vector<double> th32, rs32;
@@ -485,6 +516,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       bool done = false;
       if (mp->refine == 1) {
         // ---- FP64 phase: block refinement of the FP32 Ritz blocks -----------
+        if (p->spmm_tune) tune_panels_of(*op64, B, nref);
         BlockRefiner<double, B> rf(c, n, *op64, mass64, nref, asc, eta);
         if (mp->phase1_fp64)
           throw_on(trl_hip_memcpy_d2d(c.get(), rf.X, X1.data(), std::size_t(nref) * n * B * sizeof(double)), "copy X1");

@@ -90,13 +90,18 @@ class DeviceOperator:
             if best.format == 0 and best_ax.format == 0:
                 self.ell = None
 
+    @staticmethod
+    def _name(d) -> str:
+        order = f"/plane{d.plane}:{d.plane_block}" if d.plane else ""
+        return f"{'ell' if d.format else 'csr'}/v{d.variant}{order}"
+
     @property
     def name(self) -> str:
-        return f"{'ell' if self.desc.format else 'csr'}/v{self.desc.variant}"
+        return self._name(self.desc)
 
     @property
     def name_axpby(self) -> str:
-        return f"{'ell' if self.desc_axpby.format else 'csr'}/v{self.desc_axpby.variant}"
+        return self._name(self.desc_axpby)
 
     def bytes_per_apply(self) -> float:
         """Algorithmic HBM bytes of one apply: matrix arrays once, X read once, Y written."""
