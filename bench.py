@@ -100,44 +100,61 @@ def cpu_baseline(cfg, fp64_cycles, budget_s):
     }
 
 
-def dominant_kernel_rate(ctx, dA, b, dtype, reps=200) -> dict:
-    """One Chebyshev step Y = alpha A X + beta X + gamma Z with the operator
-    variant autotuned for that fused form, of the given panel dtype (the hottest kernel of the mixed solve),
-    timed alone: one HIP event pair on the library stream around reps
-    back-to-back launches.  Z is Y, as in the filter's three-term step.
-    Algorithmic bytes = the stored matrix entries (values + int32 indices) +
-    X + Z read + Y written."""
+def dominant_kernel_rate(ctx, dA, b, dtype, npanels, reps=200) -> dict:
+    """The Chebyshev filter step of the mixed solve, Y = alpha A X + beta X +
+    gamma Y (in place, as the filter calls it), in the form the solve uses:
+    on all npanels refinement blocks per launch when that tunes faster
+    (DeviceOperator.panels_pay, the solver's own rule), else one block per
+    launch, with the autotuned operator of the given panel dtype.  Timed
+    alone: one HIP event pair on the library stream around reps back-to-back
+    launches.  Algorithmic bytes (the library's count for the call) = the
+    stored matrix entries (values + int32 indices) once + X, Y read + Y
+    written for every panel."""
     import ctypes as C
 
     from trl_amd import _lib as L
 
     op = dA.operator(b, dtype, tune=True)
+    op.tune_panels(npanels)
+    multi = op.panels_pay(npanels)
+    np_ = npanels if multi else 1
+    d = op.desc_panels if multi else op.desc_axpby
     dt = op.dtype
     rng = np.random.default_rng(1)
-    X = ctx.to_device(rng.standard_normal((dA.n, b)).astype(dt))
-    Y = ctx.to_device(rng.standard_normal((dA.n, b)).astype(dt))
+    X = ctx.to_device(rng.standard_normal((np_, dA.n, b)).astype(dt))
+    Y = ctx.to_device(rng.standard_normal((np_, dA.n, b)).astype(dt))
     H = L.hip()
 
-    # in place in Z, as the filter calls it: Y = alpha A X + beta X + gamma Y
     def step():
-        L.check(H.trl_hip_spmm_axpby(ctx.h, L.dtype_code(dt), dA.n, b, C.byref(op.desc_axpby), X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125), "axpby")
+        L.check(H.trl_hip_spmm_axpby_panels(ctx.h, L.dtype_code(dt), dA.n, b, C.byref(d), np_, X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125), "axpby")
 
-    for _ in range(WARMUP_LAUNCHES):
+    for _ in range(WARMUP_LAUNCHES // np_):
         step()
     ctx.prof_enable(True)  # one profiled launch: the library's own byte count for this call
     step()
     per = ctx.prof_read()["spmm"]["bytes"]
     ctx.prof_enable(False)
+    reps = max(20, reps // np_)
     avg = ctx.time_region(step, reps)
     S = "double" if dtype == "f64" else "float"
-    d = op.desc_axpby
     # the launcher's 32-bit offset form: panel and ELL array byte offsets below 4 GiB
     sz = np.dtype(dt).itemsize
     o32 = "true" if dA.n * b * sz <= 0xFFFFFFFF and (d.format == 0 or d.width * dA.n * max(sz, 4) <= 0xFFFFFFFF) else "false"
-    name = f"k_spmm_rows<{S}, {b}, {max(1, -d.variant)}, {'true' if d.format else 'false'}, true, {o32}>" if d.variant <= 1 else \
-        f"k_spmm_lanes<{S}, {b}, {d.variant}, true, {o32}>"
-    return {"kernel": name, "operator": op.name_axpby, "launches": reps, "avg_launch_us": avg * 1e6, "algorithmic_bytes_per_launch": per,
-            "achieved": per / avg / 1e9}
+    cs = max(1, -d.variant)
+    if multi and d.format == 1 and d.width <= 8 and d.variant <= 1:
+        name = f"k_spmm_ell_panels<{S}, {b}, {cs}, true, {o32}>"
+    elif d.variant <= 1:
+        name = f"k_spmm_rows<{S}, {b}, {cs}, {'true' if d.format else 'false'}, true, {o32}>"
+    else:
+        name = f"k_spmm_lanes<{S}, {b}, {d.variant}, true, {o32}>"
+    return {"kernel": name, "operator": DeviceOperatorName(d), "panels_per_launch": np_, "launches": reps, "avg_launch_us": avg * 1e6,
+            "algorithmic_bytes_per_launch": per, "achieved": per / avg / 1e9}
+
+
+def DeviceOperatorName(d) -> str:
+    from trl_amd.solver import DeviceOperator
+
+    return DeviceOperator._name(d)
 
 
 def traffic_of(kernel: str):
@@ -292,12 +309,12 @@ def main(argv=None) -> int:
     if dom == "spmm":
         # the operator kernel of the precision that spent more time in it
         kdt = max(("f32", "f64"), key=lambda t: fam_t[t]["spmm"]["seconds"])
-        k = dominant_kernel_rate(ctx, dA, cfg.b, kdt)
+        k = dominant_kernel_rate(ctx, dA, cfg.b, kdt, a.refine_blocks or cfg.nev // cfg.b + 7)
         traffic, tsrc = traffic_of(k["kernel"])
         roof = {"kernel": k["kernel"], "bound": "hbm", "achieved": k["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": k["achieved"] / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": tsrc, "launches": k["launches"],
                 "avg_launch_us": k["avg_launch_us"], "algorithmic_bytes_per_launch": k["algorithmic_bytes_per_launch"],
-                "operator": k["operator"], "family_in_solve": {"name": dom, "calls": d["calls"], "seconds": d["seconds"],
+                "operator": k["operator"], "panels_per_launch": k["panels_per_launch"], "family_in_solve": {"name": dom, "calls": d["calls"], "seconds": d["seconds"],
                                                                 "GBps": d["bytes"] / d["seconds"] / 1e9,
                                                                 "by_dtype": {t: {"calls": fam_t[t][dom]["calls"], "seconds": fam_t[t][dom]["seconds"]}
                                                                              for t in fam_t}}}

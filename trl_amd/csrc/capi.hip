@@ -559,8 +559,8 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
   // a blocked order with about 1/k of a plane per block (whole workgroups), or
   // plane 0 when the far off-diagonals are within a few workgroups
   auto block_of = [&](int variant, int k) {
-    // rows per workgroup: the one-panel kernels', or 256 lanes of the ELL panel kernel
-    const int R = npanels > 1 ? 256 / (variant < 0 ? -variant : 1) : spmm_rows_per_block(t, b, variant);
+    // rows per workgroup: the one-panel kernels', or those of the ELL panel kernel
+    const int R = spmm_rows_per_block(t, b, variant, npanels > 1);
     if (R <= 0 || bw < 8 * R || size_t(bw) >= n) return 0;
     return std::min(bw, ((bw + k - 1) / k + R - 1) / R * R);
   };

@@ -76,7 +76,9 @@ int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, con
 bool valid_spmm_variant(int v);
 int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat& A, const void* X, void* Y, const void* Z, bool axpby, double alpha,
                 double beta, double gamma, int npanels = 1);
-int spmm_rows_per_block(trl_hip_dtype t, int b, int variant);
+// rows per workgroup of the one-panel (panels = false) or several-panel ELL
+// product kernel of this variant
+int spmm_rows_per_block(trl_hip_dtype t, int b, int variant, bool panels = false);
 int launch_csr_bandwidth(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, int* dev_out);
 int launch_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* dev_out);
 int launch_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ecol,

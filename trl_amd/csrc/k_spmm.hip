@@ -201,8 +201,16 @@ __global__ __launch_bounds__((spmm_threads<S, B / CS>())) void k_spmm_rows(const
 // one after another keeps one panel's X, Z and Y streams in flight at a time
 // (measured faster than spreading a workgroup's lanes over all panels at
 // once: nine concurrent write streams cost more than the matrix saves).
+// Workgroup size of the several-panel kernel: 256 lanes (measured on the
+// 4M-row stencil, b = 8, nine panels: 1024-lane workgroups take 8% longer).
+template <typename S, int BC>
+constexpr int panel_threads()
+{
+  return 256;
+}
+
 template <typename S, int B, int CS, bool AXPBY, bool O32>
-__global__ __launch_bounds__(256) void k_spmm_ell_panels(int width, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X, S* Y,
+__global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panels(int width, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X, S* Y,
                                                          Epi e, int n)
 {
   using I = Idx<O32>;
@@ -212,30 +220,35 @@ __global__ __launch_bounds__(256) void k_spmm_ell_panels(int width, const int32_
   if (row < 0) return;
   const I ro = I(row) * I(B) + I(off);
   const S* Z = static_cast<const S*>(e.Z);
-  int ci[kChunk];
+  // slots past width act as padding (the row's own X row, value 0), so the
+  // panel loop has no branches; their gathers hit the cache
+  I cofs[kChunk];
   S vi[kChunk];
 #pragma unroll
-  for (int q = 0; q < kChunk; ++q)
+  for (int q = 0; q < kChunk; ++q) {
+    int c = row;
+    vi[q] = S(0);
     if (q < width) {
       const size_t base = size_t(q) * size_t(n);
-      ci[q] = ld<true>(elem<O32>(col + base, I(row)));
+      c = ld<true>(elem<O32>(col + base, I(row)));
       vi[q] = ld<true>(elem<O32>(val + base, I(row)));
     }
+    cofs[q] = I(c) * I(B) + I(off);
+  }
   for (unsigned p = 0; p < e.npan; ++p) {
     const size_t po = size_t(p) * e.pstride;
+    const S* Xp = X + po;
     S x[kChunk][BC];
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q)
-      if (q < width) RowIO<S, BC>::load(elem<O32>(X + po, I(ci[q]) * I(B) + I(off)), x[q]);
+    for (int q = 0; q < kChunk; ++q) RowIO<S, BC>::load(elem<O32>(Xp, cofs[q]), x[q]);
     S acc[BC];
 #pragma unroll
     for (int c = 0; c < BC; ++c) acc[c] = S(0);
 #pragma unroll
     for (int q = 0; q < kChunk; ++q)
-      if (q < width)
 #pragma unroll
-        for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
-    epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, X + po, Y + po, Z ? Z + po : nullptr, e);
+      for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
+    epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, Xp, Y + po, Z ? Z + po : nullptr, e);
   }
 }
 
@@ -354,23 +367,29 @@ template <typename S, int B, bool AXPBY, bool O32>
 int launch_ell_panels(trl_hip_ctx* ctx, size_t n, int variant, int width, const int32_t* col, const S* val, const S* X, S* Y, const Epi& e)
 {
   const unsigned cs = unsigned(variant < 0 ? -variant : 1);
-  const dim3 g{unsigned(row_ranges(n, 256 / cs, e))}, blk{256};
+#define TRL_PANELS(CS_)                                                                                                                            \
+  do {                                                                                                                                             \
+    constexpr unsigned T = panel_threads<S, B / CS_>();                                                                                            \
+    const dim3 g{unsigned(row_ranges(n, T / CS_, e))}, blk{T};                                                                                     \
+    hipLaunchKernelGGL((k_spmm_ell_panels<S, B, CS_, AXPBY, O32>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));                      \
+  } while (0)
   if (cs == 1)
-    hipLaunchKernelGGL((k_spmm_ell_panels<S, B, 1, AXPBY, O32>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));
+    TRL_PANELS(1);
   else if (cs == 2) {
     if constexpr (B % 2 == 0 && (B / 2) * sizeof(S) >= 8)
-      hipLaunchKernelGGL((k_spmm_ell_panels<S, B, 2, AXPBY, O32>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));
+      TRL_PANELS(2);
     else
       return TRL_HIP_EINVAL;
   }
   else if (cs == 4) {
     if constexpr (B % 4 == 0 && (B / 4) * sizeof(S) >= 8)
-      hipLaunchKernelGGL((k_spmm_ell_panels<S, B, 4, AXPBY, O32>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));
+      TRL_PANELS(4);
     else
       return TRL_HIP_EINVAL;
   }
   else
     return TRL_HIP_EINVAL;
+#undef TRL_PANELS
   TRL_HIP_LAUNCH_CHECK();
   return 0;
 }
@@ -507,21 +526,21 @@ int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hi
   return 0;
 }
 
-int spmm_rows_per_block(trl_hip_dtype t, int b, int variant)
+int spmm_rows_per_block(trl_hip_dtype t, int b, int variant, bool panels)
 {
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
       const int v = variant == 0 ? default_variant<S, B>() : variant;
-      if (v > 1) return spmm_threads<S, B>() / v;
+      if (v > 1) return panels ? 0 : spmm_threads<S, B>() / v;
       if (v == -2) {
-        if constexpr (B % 2 == 0) return spmm_threads<S, B / 2>() / 2;
+        if constexpr (B % 2 == 0) return (panels ? panel_threads<S, B / 2>() : spmm_threads<S, B / 2>()) / 2;
         return 0;
       }
       if (v == -4) {
-        if constexpr (B % 4 == 0) return spmm_threads<S, B / 4>() / 4;
+        if constexpr (B % 4 == 0) return (panels ? panel_threads<S, B / 4>() : spmm_threads<S, B / 4>()) / 4;
         return 0;
       }
-      return spmm_threads<S, B>();
+      return panels ? panel_threads<S, B>() : spmm_threads<S, B>();
     });
   });
   return 0;

@@ -7,6 +7,7 @@ or hardware counters) sees clean, repeated dispatches of one kernel.
 Kernels (--what, comma separated):
   spmm32, spmm64        Y = A X with the autotuned operator (FP32 / FP64)
   spmm_axpby32/64       Y = alpha A X + beta X + gamma Y (the Chebyshev step, FP32 / FP64)
+  panels32, panels64    the same step on 9 panels per launch (the refinement's filter)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
   cholqr32, cholqr64    CholQR2 of one FP32 panel, CholQR of one FP64 panel
   copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
@@ -34,7 +35,7 @@ def main(argv=None) -> int:
     ap.add_argument("--b", type=int, default=8)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10, help="untimed launches before each kernel's timed ones")
-    ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,gram32,update32,cholqr32,cholqr64,copy")
+    ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,panels32,panels64,gram32,update32,cholqr32,cholqr64,copy")
     a = ap.parse_args(argv)
     what = a.what.split(",")
     cfg = P.config_c3(a.side)
@@ -75,6 +76,22 @@ def main(argv=None) -> int:
                 timed(f"{w}[{op.name_axpby}]", fn, mat_bytes(op.desc_axpby) + 3 * n * b * s)
             else:
                 timed(f"{w}[{op.name}]", lambda op=op, X=X, Y=Y: op.apply(X, Y), mat + 2 * n * b * s)
+        elif w in ("panels32", "panels64"):
+            # the filter step on all nine refinement blocks per launch, in place
+            dt = np.float32 if w == "panels32" else np.float64
+            s = np.dtype(dt).itemsize
+            op = dA.operator(b, "f32" if w == "panels32" else "f64", tune=True)
+            npan = 9
+            op.tune_panels(npan)
+            d = op.desc_panels
+            X = ctx.to_device(rng.standard_normal((npan, n, b)).astype(dt))
+            Y = ctx.to_device(rng.standard_normal((npan, n, b)).astype(dt))
+            mat = (d.width * n * (4 + s)) if d.format == 1 else (A.nnz * (4 + s) + (n + 1) * 4)
+
+            def fn(d=d, X=X, Y=Y, dt=dt):
+                L.check(H.trl_hip_spmm_axpby_panels(ctx.h, L.dtype_code(dt), n, b, C.byref(d), npan, X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125), "panels")
+
+            timed(f"{w}[{op._name(d)}]", fn, mat + 3 * npan * n * b * s)
         elif w in ("gram32", "update32"):
             k = 16
             V = ctx.to_device(rng.standard_normal((k + 1, n, b)).astype(np.float32))

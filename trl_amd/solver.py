@@ -72,6 +72,8 @@ class DeviceOperator:
         self.desc = csr
         self.desc_axpby = csr
         self.us_axpby = None
+        self.desc_panels, self.us_panels, self.npanels = csr, None, 0
+        self._csr, self._ell = csr, None
         if tune:
             w = C.c_int()
             L.check(H.trl_hip_csr_max_row(ctx.h, A.n, A.rowptr.ptr, C.byref(w)), "csr_max_row")
@@ -80,6 +82,7 @@ class DeviceOperator:
                 self.ell = (ctx.empty((w.value, A.n), np.int32), ctx.empty((w.value, A.n), dt))
                 L.check(H.trl_hip_csr_to_ell(ctx.h, L.dtype_code(dt), A.n, A.rowptr.ptr, A.col.ptr, val, w.value, self.ell[0].ptr, self.ell[1].ptr), "csr_to_ell")
                 ell = L.SpMat(1, None, w.value, self.ell[0].ptr, self.ell[1].ptr, dscale, 0)
+                self._ell = ell
             best, us = L.SpMat(), C.c_double()
             L.check(H.trl_hip_spmm_tune(ctx.h, L.dtype_code(dt), A.n, b, C.byref(csr), C.byref(ell) if ell else None, C.byref(best), C.byref(us)), "spmm_tune")
             self.desc, self.us = best, us.value
@@ -89,6 +92,20 @@ class DeviceOperator:
             self.desc_axpby, self.us_axpby = best_ax, us_ax.value
             if best.format == 0 and best_ax.format == 0:
                 self.ell = None
+
+    def tune_panels(self, npanels: int) -> None:
+        """Tunes the axpby form on npanels panels per launch (trl_hip_spmm_tune_panels)."""
+        ctx, H = self.A.ctx, L.hip()
+        csr, ell = self._csr, self._ell if self.ell else None
+        best, us = L.SpMat(), C.c_double()
+        L.check(H.trl_hip_spmm_tune_panels(ctx.h, L.dtype_code(self.dtype), self.A.n, self.b, npanels, C.byref(csr), C.byref(ell) if ell else None,
+                                           C.byref(best), C.byref(us)), "spmm_tune_panels")
+        self.desc_panels, self.us_panels, self.npanels = best, us.value, npanels
+
+    def panels_pay(self, npanels: int) -> bool:
+        """The rule the solver's filter applies: one launch over the panels
+        when it beats npanels one-panel launches by 3%."""
+        return self.npanels == npanels and self.us_panels < 0.97 * npanels * self.us_axpby
 
     @staticmethod
     def _name(d) -> str:
