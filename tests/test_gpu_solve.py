@@ -182,6 +182,28 @@ def test_mixed_fp32_refinement_sweeps(ctx, degree_opt):
     assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))
 
 
+def test_mixed_filter_all_blocks_per_launch(ctx):
+    # a 3D Laplacian large enough that stepping all 9 refinement blocks in one
+    # launch tunes faster than 9 launches (the solver then filters that way):
+    # the same eigenpairs as the plain FP64 solve
+    c = P.config_c3(64)
+    dA = DeviceCSR(ctx, c.csr)
+    op = dA.operator(c.b, "f32", tune=True)
+    op.tune_panels(9)
+    print("panels", op.name_axpby, op.us_axpby, op._name(op.desc_panels), op.us_panels, op.panels_pay(9))
+    if not op.panels_pay(9):
+        pytest.skip("one launch per block tuned faster on this device")
+    V0 = v0(ctx, c.csr.n, c.b)
+    prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
+    o64 = solve(dA, c.b, prm, V0, reorth="dgks", spmm_tune=True)
+    om = solve_mixed(dA, c.b, prm, V0, tol_fp32=5e-2, cheb_degree=80, refine_blocks=9, max_refine=14, fp32_sweeps=8, tol_fp32_refine=3e-5,
+                     reorth="dgks", spmm_tune=True)
+    print("mixed", om.result, om.result32)
+    assert om.result["converged"] == 1
+    assert om.resid.max() <= 1e-8
+    assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))
+
+
 def test_mixed_generalized(ctx):
     # the structural pair K x = lambda M x in mixed precision, against scipy
     import json

@@ -498,8 +498,8 @@ namespace {
 //   - for the fastest of those in index order, blocked row orders (plane =
 //     the matrix bandwidth, blocks of about 1/4, 1/8, 1/16 of a plane).
 // Several panels: only ELL rows of one chunk share the matrix between
-// panels, so the ELL variants are tried in index order and in blocks of 1/8
-// and 1/16 of a plane.
+// panels, so the ELL variants (one lane per row only for narrow rows) are
+// tried in index order and in blocks of 1/8 of a plane.
 int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, bool axpby, int npanels,
                    trl_hip_spmat* best, double* best_us)
 {
@@ -565,13 +565,15 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
     return std::min(bw, ((bw + k - 1) / k + R - 1) / R * R);
   };
   if (!rc && npanels > 1) {
+    // one lane per row only for rows of at most 16 bytes (wider rows hold
+    // too many registers across the panel loop)
+    const bool narrow = size_t(b) * (t == TRL_HIP_F32 ? 4 : 8) <= 16;
     if (ell)
       for (int v : {-2, -4, 1}) {
+        if (v == 1 && !narrow) continue;
         consider(with(*ell, v, 0, 0));
-        for (int k : {8, 16}) {
-          const int pb = block_of(v, k);
-          if (pb) consider(with(*ell, v, bw, pb));
-        }
+        const int pb = block_of(v, 8);
+        if (pb) consider(with(*ell, v, bw, pb));
       }
     else
       consider(with(*csr, 0, 0, 0));
