@@ -290,6 +290,11 @@ int trl_hip_batch_info(const trl_hip_batch* batch, int* ntiles, int* format, int
 /* cholqr_passes 1 (as the reference's orthonormalize) or 2; use_graph 0
  * launches the steps directly instead of replaying a captured graph. */
 int trl_hip_batch_set_options(trl_hip_batch* batch, int cholqr_passes, int use_graph);
+/* adaptive = 0: two classical Gram-Schmidt passes in every step (the
+ * default); 1: the second pass only for the problems whose first pass
+ * cancelled a column below 1/sqrt(2) of its norm (the DGKS test of
+ * trl_hip_reorth with eta = 1/sqrt(2)), decided on the device. */
+int trl_hip_batch_set_reorth(trl_hip_batch* batch, int adaptive);
 /* Host <-> device copy of basis block q (n[p] x b) of problem p. */
 int trl_hip_batch_set_block(trl_hip_batch* batch, int p, int q, const double* host);
 int trl_hip_batch_get_block(trl_hip_batch* batch, int p, int q, double* host);

@@ -112,6 +112,8 @@ typedef struct {
   int cholqr_passes;     /* 0/1: Cholesky-QR as the reference, 2: CholQR2       */
   int tune;              /* 1: autotune the step kernel's tiles and format      */
   int use_graph;         /* 1: replay each cycle's steps from a hipGraph        */
+  int reorth;            /* 0 or 1: two block CGS passes per step; 2: the second
+                            pass only where the DGKS test asks for it          */
 } trl_hip_batch_params;
 
 typedef struct {

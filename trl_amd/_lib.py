@@ -91,6 +91,7 @@ HIP_API = [
     ("trl_hip_batch_tune", c_int, [c_void_p, c_double_p]),
     ("trl_hip_batch_info", c_int, [c_void_p, c_int_p, c_int_p, c_int_p, c_double_p]),
     ("trl_hip_batch_set_options", c_int, [c_void_p, c_int, c_int]),
+    ("trl_hip_batch_set_reorth", c_int, [c_void_p, c_int]),
     ("trl_hip_batch_set_block", c_int, [c_void_p, c_int, c_int, c_void_p]),
     ("trl_hip_batch_get_block", c_int, [c_void_p, c_int, c_int, c_void_p]),
     ("trl_hip_batch_set_active", c_int, [c_void_p, c_void_p]),
@@ -296,7 +297,7 @@ class MixedParams(C.Structure):
 
 class BatchParams(C.Structure):
     _fields_ = [("b", c_int), ("nev", C.c_uint), ("ncv", C.c_uint), ("max_restarts", C.c_uint), ("tol", C.c_double), ("order", c_int),
-                ("cholqr_passes", c_int), ("tune", c_int), ("use_graph", c_int)]
+                ("cholqr_passes", c_int), ("tune", c_int), ("use_graph", c_int), ("reorth", c_int)]
 
 
 class BatchStats(C.Structure):

@@ -23,7 +23,7 @@ class BatchOutput:
 
 
 def solve_batch(ctx: L.Context, mats: list[CSR], b: int, nev: int, ncv: int, V0: list[np.ndarray], *, tol=1e-8, max_restarts=1000, order="ascending",
-                cholqr_passes=1, tune=True, use_graph=True, ritz_vectors=False) -> BatchOutput:
+                cholqr_passes=1, tune=True, use_graph=True, ritz_vectors=False, reorth="cgs2") -> BatchOutput:
     nprob = len(mats)
     n = (C.c_size_t * nprob)(*[m.n for m in mats])
     keep = []
@@ -39,7 +39,8 @@ def solve_batch(ctx: L.Context, mats: list[CSR], b: int, nev: int, ncv: int, V0:
     v0 = (C.c_void_p * nprob)(*[arr(v, np.float64) for v in V0])
     for m_, v in zip(mats, V0):
         assert v.shape == (m_.n, b)
-    prm = L.BatchParams(b, nev, ncv, max_restarts, tol, ORDER.get(order, order), cholqr_passes, 1 if tune else 0, 1 if use_graph else 0)
+    prm = L.BatchParams(b, nev, ncv, max_restarts, tol, ORDER.get(order, order), cholqr_passes, 1 if tune else 0, 1 if use_graph else 0,
+                        {"cgs2": 1, "dgks": 2}[reorth])
     evals = np.zeros((nprob, nev))
     resid = np.zeros((nprob, nev))
     nblk = (nev + b - 1) // b

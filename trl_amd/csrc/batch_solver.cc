@@ -10,6 +10,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
@@ -73,7 +74,7 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
   if (!ctx || nprob < 1 || !n || !rowptr || !col || !val || !V0 || !prm || !evals || !res) return TRL_HIP_EINVAL;
   const int b = prm->b;
   const unsigned nev = prm->nev, ncv = prm->ncv;
-  if (b < 1 || ncv % unsigned(b) || prm->order < 0 || prm->order > 2) return TRL_HIP_EINVAL;
+  if (b < 1 || ncv % unsigned(b) || prm->order < 0 || prm->order > 2 || prm->reorth < 0 || prm->reorth > 2) return TRL_HIP_EINVAL;
   const unsigned m = ncv / unsigned(b), kr = nev / unsigned(b) + 1;
   // the checks of lanczos.hh:45-58, for every problem
   if (nev >= ncv || m < nev / unsigned(b) + 2) return TRL_HIP_EINVAL;
@@ -85,6 +86,7 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
   int rc = trl_hip_batch_create(ctx, nprob, n, rowptr, col, val, b, int(ncv), &g.b);
   if (rc) return rc;
   if ((rc = trl_hip_batch_set_options(g.b, prm->cholqr_passes > 0 ? prm->cholqr_passes : 1, prm->use_graph))) return rc;
+  if ((rc = trl_hip_batch_set_reorth(g.b, prm->reorth == 2 ? 1 : 0))) return rc;
   for (int p = 0; p < nprob; ++p)
     if ((rc = trl_hip_batch_set_block(g.b, p, 0, V0[p]))) return rc;
   double tuned_us = 0.0;
@@ -107,6 +109,7 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
   size_t rows_tiled = 0;
   for (int p = 0; p < nprob; ++p) rows_tiled += n[p];
   unsigned retiles = 0;
+  const bool verbose = std::getenv("TRL_HIP_VERBOSE") != nullptr;
   const double t0 = now();
   const double setup_s = t0 - t_setup;
   auto Tblk = [&](Problem& q, unsigned i, unsigned j) { return q.T.data() + (size_t(i) * m + j) * BB; };
@@ -158,6 +161,12 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
             for (int bj = 0; bj < b; ++bj) yp[(size_t(i) * b + bi) * kr * b + j * b + bj] = q.rr.Y[((size_t(i) * m + j) * b + bi) * b + bj];
     });
     host_s += now() - th;
+    if (verbose) {
+      size_t rows_live = 0;
+      for (int p : live) rows_live += n[p];
+      std::fprintf(stderr, "[batch] cycle %u: %zu problems, %zu rows, extend %.3f ms, host %.3f ms\n", cycles, live.size(), rows_live, 1e3 * (th - te),
+                   1e3 * (now() - th));
+    }
     bool restart_possible = kr < m && kr + 1 < m;
     for (int p : live)
       if (P[p].error) {

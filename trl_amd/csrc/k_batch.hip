@@ -51,7 +51,7 @@ constexpr unsigned kMaxSpins = 1u << 22;
 struct BatchDev {
   int ntiles;
   int m;
-  int slot; // doubles per tile in one partial buffer (m * B * B)
+  int slot; // doubles per tile in one partial buffer ((m + 1) * B * B)
   int nseq; // counter slots per problem (one per launch in a cycle)
   size_t N;
   const int* tile_prob;
@@ -134,13 +134,27 @@ __device__ inline void wg_reduce_store(const double (&acc)[E], double* dst, doub
 // sums are added in lane order.
 __device__ inline void reduce_partials(const double* part, int slot, int tile0, int T, int E, double* out, double* scratch)
 {
-  for (int e0 = 0; e0 < E; e0 += kT) {
-    const int Ec = min(kT, E - e0);
+  // at least 4 lanes per entry, each with 4 loads in flight: the partials
+  // of a large problem are summed in a few memory latencies, not T
+  constexpr int kMinLanes = 4;
+  for (int e0 = 0; e0 < E; e0 += kT / kMinLanes) {
+    const int Ec = min(kT / kMinLanes, E - e0);
     const int S = kT / Ec;
     const int e = threadIdx.x % Ec, s = threadIdx.x / Ec;
     double acc = 0.0;
-    if (s < S)
-      for (int t = s; t < T; t += S) acc += part[size_t(tile0 + t) * slot + e0 + e];
+    if (s < S) {
+      const double* pe = part + size_t(tile0) * slot + e0 + e;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int t = s;
+      for (; t + 3 * S < T; t += 4 * S) {
+        a0 += pe[size_t(t) * slot];
+        a1 += pe[size_t(t + S) * slot];
+        a2 += pe[size_t(t + 2 * S) * slot];
+        a3 += pe[size_t(t + 3 * S) * slot];
+      }
+      for (; t < T; t += S) a0 += pe[size_t(t) * slot];
+      acc = (a0 + a1) + (a2 + a3);
+    }
     scratch[threadIdx.x] = acc;
     __syncthreads();
     if (threadIdx.x < Ec) {
@@ -263,9 +277,13 @@ __device__ inline void sub_xmt(const double (&x)[B], const double* M, double (&y
 // (selects the barrier counter).  The workgroup's rows of the new block W
 // live in the output block itself between phases (written and re-read by
 // the same workgroup, so they stay in its caches); the threads stride over
-// the tile's rows.  Dynamic LDS: m*B*B doubles (CGS coefficients).
+// the tile's rows.  Dynamic LDS: (m+1)*B*B doubles (CGS coefficients, and
+// W^T W for the second-pass test).  adaptive: the second Gram-Schmidt pass
+// runs only for the problems whose first pass cancelled a column below
+// 1/sqrt(2) of its norm (the DGKS test of the single-problem path), decided
+// identically by every workgroup of a problem from the reduced coefficients.
 template <int B, bool ELL>
-__global__ __launch_bounds__(kT, B >= 8 ? 1 : 4) void k_batch_step(BatchDev d, int i, int kind, int seq, int cq_passes)
+__global__ __launch_bounds__(kT, B >= 8 ? 1 : 4) void k_batch_step(BatchDev d, int i, int kind, int seq, int cq_passes, int adaptive)
 {
   constexpr int BB = B * B;
   extern __shared__ double Cs[];
@@ -334,24 +352,48 @@ __global__ __launch_bounds__(kT, B >= 8 ? 1 : 4) void k_batch_step(BatchDev d, i
       }
       put(r, y);
     }
-    // ---- two classical Gram-Schmidt passes against blocks 0..i --------------
+    // ---- classical Gram-Schmidt against blocks 0..i: two passes, or one
+    // when the test after the first finds no cancellation ------------------
     const int nq = i + 1;
+    __shared__ int need_s;
     for (int pass = 0; pass < 2; ++pass) {
-      for (int q = 0; q < nq; ++q) {
-        const double* Vq = V + size_t(q) * NB;
+      const bool test = adaptive && pass == 0;
+      for (int q = 0; q < nq + (test ? 1 : 0); ++q) {
+        const double* Vq = q < nq ? V + size_t(q) * NB : nullptr; // q == nq: W^T W
 #pragma unroll
         for (int e = 0; e < BB; ++e) acc[e] = 0.0;
         for (int r = rb; r < r1; r += kT) {
           double y[B], vq[B];
           row(W, r, y);
-          row(Vq, r, vq);
+          if (Vq)
+            row(Vq, r, vq);
+          else
+#pragma unroll
+            for (int c = 0; c < B; ++c) vq[c] = y[c];
           outer_acc<B>(vq, y, acc);
         }
         wg_reduce_store<BB>(acc, part(buf) + size_t(t) * d.slot + size_t(q) * BB, red);
       }
       if (!barrier()) return;
-      reduce_partials(part(buf), d.slot, tile0, T, nq * BB, Cs, scratch);
+      reduce_partials(part(buf), d.slot, tile0, T, (nq + (test ? 1 : 0)) * BB, Cs, scratch);
       buf ^= 1;
+      if (test) {
+        // column j: ||w_j||^2 after the pass = ||w_j||^2 - sum_q ||c_qj||^2
+        if (threadIdx.x == 0) need_s = 0;
+        __syncthreads();
+        if (threadIdx.x < unsigned(B)) {
+          const int j = int(threadIdx.x);
+          const double w2 = Cs[size_t(nq) * BB + j * B + j];
+          double c2 = 0.0;
+          for (int q = 0; q < nq; ++q)
+            for (int c = 0; c < B; ++c) {
+              const double x = Cs[size_t(q) * BB + c * B + j];
+              c2 = fma(x, x, c2);
+            }
+          if (!(w2 - c2 >= 0.5 * w2) || !(w2 > 0.0)) atomicOr(&need_s, 1);
+        }
+        __syncthreads();
+      }
       for (int r = rb; r < r1; r += kT) {
         double y[B];
         row(W, r, y);
@@ -362,6 +404,7 @@ __global__ __launch_bounds__(kT, B >= 8 ? 1 : 4) void k_batch_step(BatchDev d, i
         }
         put(r, y);
       }
+      if (test && !need_s) break;
     }
   }
   // ---- Cholesky-QR of W (cq_passes passes, R = R_2 R_1) ---------------------
@@ -536,10 +579,12 @@ struct trl_hip_batch {
   size_t partials_cap = 0;
   unsigned* counters = nullptr;
   double *alpha = nullptr, *beta = nullptr, *beta_prev = nullptr, *arrow = nullptr, *Ybuf = nullptr, *lam = nullptr, *rsum = nullptr;
+  size_t ybuf_count = 0; // doubles allocated at Ybuf
   size_t rsum_cap = 0, lam_cap = 0;
   int* pflag = nullptr;
   int* err = nullptr;
   int cq_passes = 1;
+  int adaptive = 0; // second Gram-Schmidt pass only where the DGKS test asks
   bool use_graph = true;
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
   // leading blocks of finished problems (nblk_saved blocks of N x b), kept
@@ -581,7 +626,7 @@ BatchDev make_dev(trl_hip_batch* bt)
   BatchDev d{};
   d.ntiles = bt->ntiles;
   d.m = bt->m;
-  d.slot = bt->m * bt->b * bt->b;
+  d.slot = (bt->m + 1) * bt->b * bt->b;
   d.nseq = bt->m + 1;
   d.N = bt->N;
   d.tile_prob = bt->tile_prob;
@@ -659,7 +704,7 @@ int build_tiles(trl_hip_batch* bt, int rpt, bool all = false)
   }
   const int nt = int(tp.size());
   int occ = 0;
-  const size_t shm = size_t(bt->m) * bt->b * bt->b * sizeof(double);
+  const size_t shm = size_t(bt->m + 1) * bt->b * bt->b * sizeof(double);
   int rc = with_step_kernel(bt->b, bt->format, [&](auto kfn) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, kT, shm) == hipSuccess ? 0 : TRL_HIP_EINVAL;
   });
@@ -681,7 +726,7 @@ int build_tiles(trl_hip_batch* bt, int rpt, bool all = false)
   TRL_HIP_TRY(hipMemcpy(bt->tile_local, loc.data(), nt * sizeof(int), hipMemcpyHostToDevice));
   TRL_HIP_TRY(hipMemcpy(bt->prob_ntiles, pn.data(), bt->nprob * sizeof(int), hipMemcpyHostToDevice));
   TRL_HIP_TRY(hipMemcpy(bt->prob_tile0, pt0.data(), bt->nprob * sizeof(int), hipMemcpyHostToDevice));
-  const size_t need = size_t(2) * nt * bt->m * bt->b * bt->b;
+  const size_t need = size_t(2) * nt * (bt->m + 1) * bt->b * bt->b;
   if (need > bt->partials_cap) {
     dfree(bt->partials);
     if ((rc = dalloc(&bt->partials, need))) return rc;
@@ -707,11 +752,11 @@ int enqueue_cycle(trl_hip_batch* bt, int k, int m_end, int quasi)
   const size_t nc = size_t(bt->nprob) * (bt->m + 1);
   hipLaunchKernelGGL(k_zero_u32, dim3(unsigned(std::min<size_t>((nc + 255) / 256, 64))), dim3(256), 0, bt->ctx->stream, bt->counters, nc);
   TRL_HIP_LAUNCH_CHECK();
-  const size_t shm = size_t(bt->m) * bt->b * bt->b * sizeof(double);
+  const size_t shm = size_t(bt->m + 1) * bt->b * bt->b * sizeof(double);
   int seq = 0;
   return with_step_kernel(bt->b, bt->format, [&](auto kfn) -> int {
     auto launch = [&](int i, int kind) -> int {
-      hipLaunchKernelGGL(kfn, dim3(unsigned(bt->ntiles)), dim3(kT), shm, bt->ctx->stream, d, i, kind, seq, bt->cq_passes);
+      hipLaunchKernelGGL(kfn, dim3(unsigned(bt->ntiles)), dim3(kT), shm, bt->ctx->stream, d, i, kind, seq, bt->cq_passes, bt->adaptive);
       ++seq;
       TRL_HIP_LAUNCH_CHECK();
       return 0;
@@ -883,6 +928,14 @@ int trl_hip_batch_set_options(trl_hip_batch* bt, int cholqr_passes, int use_grap
   return 0;
 }
 
+int trl_hip_batch_set_reorth(trl_hip_batch* bt, int adaptive)
+{
+  if (!bt || (adaptive != 0 && adaptive != 1)) return TRL_HIP_EINVAL;
+  if (bt->adaptive != adaptive) clear_graphs(bt);
+  bt->adaptive = adaptive;
+  return 0;
+}
+
 int trl_hip_batch_set_block(trl_hip_batch* bt, int p, int q, const double* host)
 {
   if (!bt || p < 0 || p >= bt->nprob || q < 0 || q > bt->m || !host) return TRL_HIP_EINVAL;
@@ -969,9 +1022,15 @@ int trl_hip_batch_rotate(trl_hip_batch* bt, const double* Y, int kr)
   const size_t shm = ysz * sizeof(double);
   if (shm > bt->ctx->max_lds) return TRL_HIP_ERANGE;
   hipStream_t s = bt->ctx->stream;
-  dfree(bt->Ybuf);
-  int rc = dalloc(&bt->Ybuf, ysz * bt->nprob);
-  if (rc) return rc;
+  // the coefficient buffer keeps its size (kr is the same every cycle): no
+  // allocation, and no free (which would wait for the device), per cycle
+  if (bt->ybuf_count < ysz * bt->nprob) {
+    dfree(bt->Ybuf);
+    bt->ybuf_count = 0;
+    const int rc = dalloc(&bt->Ybuf, ysz * bt->nprob);
+    if (rc) return rc;
+    bt->ybuf_count = ysz * bt->nprob;
+  }
   TRL_HIP_TRY(hipMemcpyAsync(bt->Ybuf, Y, ysz * bt->nprob * sizeof(double), hipMemcpyHostToDevice, s));
   const BatchDev d = make_dev(bt);
   switch (b) {
