@@ -522,7 +522,7 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
       (void)hipGetLastError();
       return -1.0; // variant not applicable to this b / dtype
     }
-    const int reps = npanels > 1 ? 3 : 5;
+    const int reps = npanels > 1 ? 2 : 3;
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return -1.0;
     for (int r = 0; r < reps; ++r) (void)launch_spmm(ctx, t, n, b, A, X, Y, Z, axpby, 0.5, -0.25, 0.125, npanels);
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;

@@ -67,13 +67,15 @@ std::shared_ptr<SparseOperator<S>> make_operator(const Context& c, std::size_t n
                                                  const trl_hip_solver_params& p)
 {
   auto op = std::make_shared<SparseOperator<S>>(c, n, rowptr, col, val, minv, p.spmm_tune ? 0 : p.lanes_per_row);
+  const auto t0 = std::chrono::steady_clock::now();
   if (p.spmm_tune) op->tune(int(B));
+  const double tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (std::getenv("TRL_HIP_VERBOSE")) {
     const auto& d = op->descriptor();
     const auto& x = op->axpby_descriptor();
-    std::fprintf(stderr, "[operator] %s b=%u %s/v%d/plane%d:%d (%.1f us), axpby %s/v%d/plane%d:%d (%.1f us)\n", sizeof(S) == 4 ? "f32" : "f64", B,
-                 d.format ? "ell" : "csr", d.variant, d.plane, d.plane_block, op->tuned_us(), x.format ? "ell" : "csr", x.variant, x.plane, x.plane_block,
-                 op->axpby_us());
+    std::fprintf(stderr, "[operator] %s b=%u %s/v%d/plane%d:%d (%.1f us), axpby %s/v%d/plane%d:%d (%.1f us); tuned in %.1f ms\n",
+                 sizeof(S) == 4 ? "f32" : "f64", B, d.format ? "ell" : "csr", d.variant, d.plane, d.plane_block, op->tuned_us(), x.format ? "ell" : "csr",
+                 x.variant, x.plane, x.plane_block, op->axpby_us(), tune_ms);
   }
   return op;
 }
@@ -82,12 +84,14 @@ std::shared_ptr<SparseOperator<S>> make_operator(const Context& c, std::size_t n
 template <class S>
 void tune_panels_of(SparseOperator<S>& op, unsigned B, unsigned nref)
 {
+  const auto t0 = std::chrono::steady_clock::now();
   op.tune_panels(int(B), int(nref));
   if (std::getenv("TRL_HIP_VERBOSE")) {
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     const auto& d = op.panels_descriptor();
-    std::fprintf(stderr, "[operator] %s %u panels %s/v%d/plane%d:%d (%.1f us per launch, %.1f us per panel; one panel %.1f us)%s\n",
+    std::fprintf(stderr, "[operator] %s %u panels %s/v%d/plane%d:%d (%.1f us per launch, %.1f us per panel; one panel %.1f us)%s; tuned in %.1f ms\n",
                  sizeof(S) == 4 ? "f32" : "f64", nref, d.format ? "ell" : "csr", d.variant, d.plane, d.plane_block, op.panels_us(), op.panels_us() / nref,
-                 op.axpby_us(), op.panels_pay(int(nref)) ? " used" : "");
+                 op.axpby_us(), op.panels_pay(int(nref)) ? " used" : "", ms);
   }
 }
 
