@@ -215,12 +215,14 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--side", type=int, default=160, help="grid side of the 3D Laplacian (160 = the named workload)")
-    ap.add_argument("--tol-fp32", type=float, default=5e-2)
-    ap.add_argument("--cheb-degree", type=int, default=100)
-    ap.add_argument("--refine-blocks", type=int, default=9, help="Ritz blocks refined together in FP64 (0: nev/b + 7)")
+    ap.add_argument("--tol-fp32", type=float, default=0.4, help="tolerance of the FP32 Lanczos phase (0.4: about one cycle, a starting subspace)")
+    ap.add_argument("--cheb-degree", type=int, default=120)
+    ap.add_argument("--refine-blocks", type=int, default=7, help="Ritz blocks refined together (0: nev/b + 7)")
     ap.add_argument("--max-refine", type=int, default=14, help="FP64 refinement sweeps before the FP64 Lanczos fallback")
-    ap.add_argument("--fp32-sweeps", type=int, default=8, help="FP32 refinement sweeps (at most) before the FP64 ones")
+    ap.add_argument("--fp32-sweeps", type=int, default=10, help="FP32 refinement sweeps (at most) before the FP64 ones")
     ap.add_argument("--tol-fp32-refine", type=float, default=3e-5, help="FP32 residual at which the FP32 sweeps stop")
+    ap.add_argument("--degree-opt", type=int, default=2, choices=[0, 1, 2],
+                    help="per-block filter degrees from measured convergence rates: 0 off, 1 all sweeps, 2 FP64 sweeps only")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -255,7 +257,7 @@ def main(argv=None) -> int:
 
     # ---- warmup + profiled pass -------------------------------------------------
     mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks, max_refine=a.max_refine, spmm_tune=tune,
-                    reorth=a.reorth, fp32_sweeps=a.fp32_sweeps, tol_fp32_refine=a.tol_fp32_refine)
+                    reorth=a.reorth, fp32_sweeps=a.fp32_sweeps, tol_fp32_refine=a.tol_fp32_refine, degree_opt=a.degree_opt)
     for w in range(a.warmup):
         solve_mixed(dA, cfg.b, params, V0, **mixed_kw)
         log(f"warmup {w} done")
@@ -365,7 +367,7 @@ def main(argv=None) -> int:
         "config": {"workload": f"{cfg.name}: 3D 7-point Laplacian {a.side}^3 + diag U[0,1]", "n": cfg.csr.n, "nnz": cfg.csr.nnz, "b": cfg.b, "nev": cfg.nev,
                    "ncv": cfg.ncv, "tol": cfg.tol, "order": "smallest", "tol_fp32": a.tol_fp32, "cheb_degree": a.cheb_degree,
                    "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "max_refine": a.max_refine, "fp32_sweeps": a.fp32_sweeps,
-                   "tol_fp32_refine": a.tol_fp32_refine, "spmm_tune": tune, "reorth": a.reorth},
+                   "tol_fp32_refine": a.tol_fp32_refine, "degree_opt": a.degree_opt, "spmm_tune": tune, "reorth": a.reorth},
         "converged": bool(last.result["converged"]),
         "max_true_residual": float(last.resid.max()),
         "fp32_phase": last.result32,

@@ -99,7 +99,7 @@ public:
   }
   void axpby_panels(int b, int npanels, const S* X, S* Y, const S* Z, double alpha, double beta, double gamma) const
   {
-    const trl_hip_spmat& d = panels_n_ == npanels ? op_panels_ : op_axpby_;
+    const trl_hip_spmat& d = panels_n_ > 1 && npanels > 1 ? op_panels_ : op_axpby_; // the panel form, tuned at panels_n_ panels
     throw_on(trl_hip_spmm_axpby_panels(ctx_.get(), dtype, n_, b, &d, npanels, X, Y, Z, alpha, beta, gamma), "apply_axpby_panels");
   }
 

@@ -77,7 +77,8 @@ typedef struct {
   double tol_fp32_refine;    /* ... stopping once the FP32 residuals are below this   */
   int degree_opt;            /* refine = 1: 1 gives each Ritz block its own filter
                                 degree from the measured convergence rate of its
-                                vectors (0 when converged), 0: cheb_degree on all  */
+                                vectors (0 when converged) in every sweep, 2 only
+                                in the FP64 sweeps, 0: cheb_degree on all         */
 } trl_hip_mixed_params;
 
 /* Mixed precision.  Phase 1: an FP32 thick-restart solve (FP32 basis, SpMM

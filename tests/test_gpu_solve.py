@@ -163,12 +163,12 @@ def test_mixed_refine_laplace3d(ctx, phase1_fp64, reorth):
     assert max_angle(Xh, ref["X"]) < 1e-6
 
 
-@pytest.mark.parametrize("degree_opt", [False, True])
+@pytest.mark.parametrize("degree_opt", [0, 1, 2])
 def test_mixed_fp32_refinement_sweeps(ctx, degree_opt):
     # FP32 Lanczos to a loose 5e-2, FP32 Chebyshev/Rayleigh-Ritz sweeps down to
     # the FP32 floor, then FP64 sweeps (every block at the full degree, or each
-    # block at the degree its convergence rate asks for): the same eigenpairs
-    # as the plain FP64 solve
+    # block at the degree its convergence rate asks for, in every sweep or in
+    # the FP64 ones): the same eigenpairs as the plain FP64 solve
     c = P.config_c3(24)
     dA = DeviceCSR(ctx, c.csr)
     V0 = v0(ctx, c.csr.n, c.b)
@@ -199,6 +199,22 @@ def test_mixed_filter_all_blocks_per_launch(ctx):
     om = solve_mixed(dA, c.b, prm, V0, tol_fp32=5e-2, cheb_degree=80, refine_blocks=9, max_refine=14, fp32_sweeps=8, tol_fp32_refine=3e-5,
                      reorth="dgks", spmm_tune=True)
     print("mixed", om.result, om.result32)
+    assert om.result["converged"] == 1
+    assert om.resid.max() <= 1e-8
+    assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))
+
+
+def test_mixed_bench_schedule(ctx):
+    # the benchmark's schedule on a 48^3 stand-in: one loose FP32 Lanczos
+    # cycle, FP32 sweeps on 7 blocks, FP64 sweeps with per-block degrees
+    c = P.config_c3(48)
+    dA = DeviceCSR(ctx, c.csr)
+    V0 = v0(ctx, c.csr.n, c.b)
+    prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
+    o64 = solve(dA, c.b, prm, V0, reorth="dgks", spmm_tune=True)
+    om = solve_mixed(dA, c.b, prm, V0, tol_fp32=0.4, cheb_degree=120, refine_blocks=7, max_refine=14, fp32_sweeps=10, tol_fp32_refine=3e-5,
+                     reorth="dgks", spmm_tune=True, degree_opt=2)
+    print("bench schedule", om.result, om.result32)
     assert om.result["converged"] == 1
     assert om.resid.max() <= 1e-8
     assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))

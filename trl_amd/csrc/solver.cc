@@ -244,21 +244,33 @@ struct BlockRefiner {
     for (unsigned d : deg) uniform = uniform && d == deg.front();
     std::size_t applies = 0;
     // otherwise panel by panel
-    if (uniform && deg.front() > 0 && op.panels_pay(int(nb))) {
-      // every panel at once: the matrix is read once per step for all of them
-      const unsigned degree = deg.front();
-      S* prev = X;
-      S* cur = spare;
-      op.axpby_panels(int(B), int(nb), prev, cur, nullptr, s1 / e, -s1 * cc / e, 0.0);
+    bool suffix = true; // even, non-decreasing degrees: the blocks still filtered form a suffix
+    for (unsigned j = 0; j < nb; ++j) suffix = suffix && deg[j] % 2 == 0 && (j == 0 || deg[j] >= deg[j - 1]);
+    if ((uniform || suffix) && deg.back() > 0 && op.panels_pay(int(nb))) {
+      // all blocks still being filtered in one launch per step: the matrix
+      // is read once per step for all of them
+      const unsigned degree = deg.back();
+      std::size_t applies = 0;
       double sg = s1;
-      for (unsigned i = 2; i <= degree; ++i) {
-        const double sn = 1.0 / (2.0 / s1 - sg);
-        op.axpby_panels(int(B), int(nb), cur, prev, prev, 2.0 * sn / e, -2.0 * sn * cc / e, -sg * sn);
-        std::swap(prev, cur);
-        sg = sn;
+      for (unsigned i = 1; i <= degree; ++i) {
+        unsigned j0 = 0;
+        while (deg[j0] < i) ++j0; // deg.back() >= i
+        const int np = int(nb - j0);
+        // step i reads the older iterate from the buffer written two steps
+        // ago: X for odd i (X and spare alternate, X holds step 0)
+        S* from = panel(i % 2 == 1 ? X : spare, j0);
+        S* to = panel(i % 2 == 1 ? spare : X, j0);
+        if (i == 1)
+          op.axpby_panels(int(B), np, from, to, nullptr, s1 / e, -s1 * cc / e, 0.0);
+        else {
+          const double sn = 1.0 / (2.0 / s1 - sg);
+          op.axpby_panels(int(B), np, from, to, to, 2.0 * sn / e, -2.0 * sn * cc / e, -sg * sn);
+          sg = sn;
+        }
+        applies += std::size_t(np);
       }
-      if (degree % 2 == 1) std::swap(X, spare);
-      return std::size_t(degree) * nb;
+      if (uniform && degree % 2 == 1) std::swap(X, spare);
+      return applies;
     }
     for (unsigned j = 0; j < nb; ++j) {
       const unsigned degree = deg[j];
@@ -292,7 +304,8 @@ struct BlockRefiner {
 // panel had in the previous sweep.  A vector already at tol needs 0; one
 // without a measured rate needs the full degree.  Panels holding vectors
 // beyond nev (the buffer that keeps the filter cutoff away from the wanted
-// values) take the largest wanted degree.  The degrees aim at tol / 4, so
+// values) take the largest wanted degree, and the degrees are made
+// non-decreasing over the blocks.  The degrees aim at tol / 4, so
 // that a slightly optimistic rate still ends below tol.  Degrees are rounded
 // up to even.
 inline std::vector<unsigned> panel_degrees(unsigned nb, unsigned B, unsigned nev, double tol, unsigned degree, const std::vector<double>& rs,
@@ -320,6 +333,10 @@ inline std::vector<unsigned> panel_degrees(unsigned nb, unsigned B, unsigned nev
   }
   for (unsigned j = 0; j < nb; ++j)
     if ((j + 1) * B > nev) deg[j] = dmax;
+  // non-decreasing over the blocks (the Ritz values ascend with the block
+  // index, and so, mostly, do the degrees they need): the blocks still being
+  // filtered at any step are then a suffix, stepped by one launch
+  for (unsigned j = 1; j < nb; ++j) deg[j] = std::max(deg[j], deg[j - 1]);
   return deg;
 }
 
@@ -510,7 +527,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         std::vector<double> th32, rs32;
         unsigned sw32 = 0;
         std::size_t ap32 = 0;
-        refine_sweeps<float, B>(rf32, c, n, *op32, mass32, p->nev, mp->tol_fp32_refine, mp->fp32_sweeps, mp->cheb_degree, mp->degree_opt != 0, lo_all,
+        refine_sweeps<float, B>(rf32, c, n, *op32, mass32, p->nev, mp->tol_fp32_refine, mp->fp32_sweeps, mp->cheb_degree, mp->degree_opt == 1, lo_all,
                                 hi_all, th32, rs32, sw32, ap32, "fp32");
         throw_on(trl_hip_memcpy_d2d(c.get(), X32.data(), rf32.X, std::size_t(nref) * n * B * sizeof(float)), "copy X32");
         r32.n_op_apply += (sw32 + 1) * nref * 2 + ap32;

@@ -7,7 +7,7 @@ or hardware counters) sees clean, repeated dispatches of one kernel.
 Kernels (--what, comma separated):
   spmm32, spmm64        Y = A X with the autotuned operator (FP32 / FP64)
   spmm_axpby32/64       Y = alpha A X + beta X + gamma Y (the Chebyshev step, FP32 / FP64)
-  panels32, panels64    the same step on 9 panels per launch (the refinement's filter)
+  panels32, panels64    the same step on --panels panels per launch (the refinement's filter)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
   cholqr32, cholqr64    CholQR2 of one FP32 panel, CholQR of one FP64 panel
   copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
@@ -35,6 +35,7 @@ def main(argv=None) -> int:
     ap.add_argument("--b", type=int, default=8)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10, help="untimed launches before each kernel's timed ones")
+    ap.add_argument("--panels", type=int, default=7, help="panels per launch of panels32/panels64 (the refinement's blocks)")
     ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,panels32,panels64,gram32,update32,cholqr32,cholqr64,copy")
     a = ap.parse_args(argv)
     what = a.what.split(",")
@@ -77,11 +78,11 @@ def main(argv=None) -> int:
             else:
                 timed(f"{w}[{op.name}]", lambda op=op, X=X, Y=Y: op.apply(X, Y), mat + 2 * n * b * s)
         elif w in ("panels32", "panels64"):
-            # the filter step on all nine refinement blocks per launch, in place
+            # the filter step on all refinement blocks per launch, in place
             dt = np.float32 if w == "panels32" else np.float64
             s = np.dtype(dt).itemsize
             op = dA.operator(b, "f32" if w == "panels32" else "f64", tune=True)
-            npan = 9
+            npan = a.panels
             op.tune_panels(npan)
             d = op.desc_panels
             X = ctx.to_device(rng.standard_normal((npan, n, b)).astype(dt))
