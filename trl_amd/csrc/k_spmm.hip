@@ -109,6 +109,23 @@ __device__ inline T* elem(T* base, Idx<O32> i)
 // AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z (Z
 // may be null).  X, Y, Z point at the lane's panel; ro = row * B + off, the
 // first element of this lane in it.
+// The AXPBY epilogue with the lane's own X row already in registers.
+template <typename S, int BC, bool O32>
+__device__ inline void epilogue_x(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S (&xr)[BC], S* Y, const S* Z, const Epi& e)
+{
+  const S* ds = static_cast<const S*>(e.dscale);
+  S zr[BC], out[BC];
+  if (Z) RowIO<S, BC>::template load<true>(elem<O32>(Z, ro), zr);
+  const double a = ds ? e.alpha * double(ld<true>(elem<O32>(ds, row))) : e.alpha;
+#pragma unroll
+  for (int q = 0; q < BC; ++q) {
+    double v = a * double(acc[q]) + e.beta * double(xr[q]);
+    if (Z) v += e.gamma * double(zr[q]);
+    out[q] = S(v);
+  }
+  RowIO<S, BC>::template store<true>(elem<O32>(Y, ro), out);
+}
+
 template <typename S, int BC, bool AXPBY, bool O32>
 __device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S* __restrict__ X, S* Y, const S* Z, const Epi& e)
 {
@@ -224,6 +241,7 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
   // panel loop has no branches; their gathers hit the cache
   I cofs[kChunk];
   S vi[kChunk];
+  int self = -1; // a slot that gathers the row itself (the diagonal, or padding)
 #pragma unroll
   for (int q = 0; q < kChunk; ++q) {
     int c = row;
@@ -233,6 +251,7 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
       c = ld<true>(elem<O32>(col + base, I(row)));
       vi[q] = ld<true>(elem<O32>(val + base, I(row)));
     }
+    if (c == row && self < 0) self = q;
     cofs[q] = I(c) * I(B) + I(off);
   }
   for (unsigned p = 0; p < e.npan; ++p) {
@@ -248,7 +267,25 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
     for (int q = 0; q < kChunk; ++q)
 #pragma unroll
       for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
-    epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, Xp, Y + po, Z ? Z + po : nullptr, e);
+    if constexpr (AXPBY) {
+      // beta X needs the lane's own X row: one of the gathered rows, unless
+      // the row has neither a diagonal entry nor padding
+      S xr[BC];
+      if (self >= 0) {
+#pragma unroll
+        for (int c = 0; c < BC; ++c) xr[c] = x[0][c];
+#pragma unroll
+        for (int q = 1; q < kChunk; ++q)
+          if (q == self)
+#pragma unroll
+            for (int c = 0; c < BC; ++c) xr[c] = x[q][c];
+      }
+      else
+        RowIO<S, BC>::load(elem<O32>(Xp, ro), xr);
+      epilogue_x<S, BC, O32>(acc, I(row), ro, xr, Y + po, Z ? Z + po : nullptr, e);
+    }
+    else
+      epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, Xp, Y + po, Z ? Z + po : nullptr, e);
   }
 }
 
