@@ -74,7 +74,10 @@ public:
   // Rayleigh-Ritz on the projected matrix and the converged count
   // (evp_base.hh:79-154): eigen-decompose T in FP64 on the host, order the
   // pairs, keep the eigenvectors on the device, and count the first nev
-  // pairs with ||beta y_last|| / max(|theta|, eps) < tol.
+  // pairs with ||beta y_last|| / max(|theta|, eps) < tol.  All Ritz values
+  // are computed, the Ritz vectors only for the first nev/b + 1 blocks (the
+  // ones the driver reads) or set_ritz_blocks(), if more; the other columns
+  // of the eigenvector matrix are zero.
   virtual std::size_t solve_small_dense(const BlockMatrix& T, BlockMatrixBlockView beta, std::size_t nev)
   {
     ctx_.sync(); // raises a pending Cholesky breakdown
@@ -82,7 +85,12 @@ public:
     const std::size_t nb = T.block_rows();
     const std::vector<double> tb = T.to_host();
     const std::vector<double> bh = beta.to_host();
-    trl_amd::rayleigh_ritz(blocksize, nb, T.block_cols(), tb.data(), bh.data(), nev, static_cast<int>(order_), tolerance_, rr_, dense_);
+    // the driver uses the first k_r = nev/b + 1 blocks of Ritz vectors
+    // (restart rotation, arrow, final vectors: lanczos.hh:105-143); a caller
+    // may ask for more (set_ritz_blocks)
+    const std::size_t blocks = std::max<std::size_t>(nev / blocksize + 1, ritz_blocks_);
+    trl_amd::rayleigh_ritz(blocksize, nb, T.block_cols(), tb.data(), bh.data(), nev, static_cast<int>(order_), tolerance_, rr_, dense_,
+                           blocks * blocksize);
     eigenvalues = rr_.theta;
     evec_host_ = rr_.Y;
     last_residuals_ = rr_.residual;
@@ -114,6 +122,9 @@ public:
 
   // 1: Cholesky-QR as in the reference; 2: CholQR2 (R = R2 R1)
   void set_cholqr_passes(int p) { cholqr_passes_ = p; }
+  // At least this many leading blocks of Ritz vectors are computed by every
+  // Rayleigh-Ritz step (default: the nev/b + 1 the driver uses).
+  void set_ritz_blocks(std::size_t blocks) { ritz_blocks_ = blocks; }
   int get_cholqr_passes() const { return cholqr_passes_; }
 
   // Optional diagonal inner-product weight M (device, panel type, n entries).
@@ -167,6 +178,7 @@ protected:
   std::size_t N;
   const PanelScalar* mass_ = nullptr;
   int cholqr_passes_ = std::is_same_v<PanelScalar, float> ? 2 : 1;
+  std::size_t ritz_blocks_ = 0;
 
   std::vector<double> eigenvalues;
   std::unique_ptr<BlockMatrix> eigenvectors;

@@ -9,4 +9,10 @@ namespace trl_amd::dense {
 // converge.
 int syev(int n, const double* A, double* w, double* Z);
 
+// The same eigenvalues (ascending, all n in w) and only the eigenvectors of
+// eigenvalues w[sel[0]] < w[sel[1]] < ... (k strictly increasing indices):
+// eigenvector of w[sel[j]] in column j of the row-major n x k matrix Z.
+// Returns 0, -1 without convergence, -2 for a bad selection.
+int syev_select(int n, const double* A, double* w, int k, const int* sel, double* Z);
+
 } // namespace trl_amd::dense

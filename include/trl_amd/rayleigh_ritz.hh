@@ -30,19 +30,59 @@ struct RitzResult {
 // ((i*bc + j)*b*b), row-major inside); only the leading nb x nb blocks enter
 // the eigen-decomposition (bc == nb in the driver).  beta: b x b, the last
 // off-diagonal block.  dense: caller-provided workspace (resized here).
+// ncols: the Ritz vectors needed, in the chosen order (the driver uses the
+// first k_r*b = (nev/b + 1)*b: the restart rotation and the arrow,
+// lanczos.hh:105-143); Y's other columns are left zero.  Fewer columns than
+// nb*b take the eigensolver that computes only those eigenvectors.
 inline void rayleigh_ritz(unsigned b, std::size_t nb, std::size_t bc, const double* T, const double* beta, std::size_t nev, int order, double tol,
-                          RitzResult& out, std::vector<double>& dense)
+                          RitzResult& out, std::vector<double>& dense, std::size_t ncols = std::size_t(-1))
 {
   const std::size_t nt = nb * b;
+  ncols = std::min(ncols, nt);
   dense.resize(nt * nt);
   for (std::size_t i = 0; i < nb; ++i)
     for (std::size_t j = 0; j < nb; ++j)
       for (unsigned bi = 0; bi < b; ++bi)
         for (unsigned bj = 0; bj < b; ++bj) dense[(i * b + bi) * nt + (j * b + bj)] = T[((i * bc + j) * b + bi) * b + bj];
-  std::vector<double> w(nt), Z(nt * nt);
-  if (dense::syev(int(nt), dense.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
-
+  std::vector<double> w(nt);
   std::vector<unsigned> idx(nt);
+  std::vector<double> Z;
+  // zcol[j]: the column of Z holding the eigenvector of ascending eigenvalue j
+  std::vector<std::size_t> zcol(nt, 0);
+  std::size_t zld = nt;
+  if (ncols == nt) {
+    Z.resize(nt * nt);
+    if (dense::syev(int(nt), dense.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
+    std::iota(zcol.begin(), zcol.end(), std::size_t(0));
+  }
+  else {
+    // the order depends only on the eigenvalues: find them first (the
+    // selection below then asks for the eigenvectors of the leading ncols)
+    std::vector<int> sel;
+    if (order == 1)
+      for (std::size_t j = 0; j < ncols; ++j) sel.push_back(int(j));
+    else if (order == 2)
+      for (std::size_t j = nt - ncols; j < nt; ++j) sel.push_back(int(j));
+    else {
+      if (dense::syev_select(int(nt), dense.data(), w.data(), 0, nullptr, nullptr) != 0) throw std::runtime_error("Eigendecomposition failed");
+      std::iota(idx.begin(), idx.end(), 0u);
+      std::stable_sort(idx.begin(), idx.end(), [&](unsigned x, unsigned y) { return std::abs(w[x]) > std::abs(w[y]); });
+      for (std::size_t j = 0; j < ncols; ++j) sel.push_back(int(idx[j]));
+      std::sort(sel.begin(), sel.end());
+    }
+    Z.resize(nt * ncols);
+    if (dense::syev_select(int(nt), dense.data(), w.data(), int(ncols), sel.data(), Z.data()) == 0) {
+      for (std::size_t c = 0; c < ncols; ++c) zcol[std::size_t(sel[c])] = c;
+      zld = ncols;
+    }
+    else {
+      // inverse iteration gave up (a non-finite iterate): the full solver
+      Z.resize(nt * nt);
+      if (dense::syev(int(nt), dense.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
+      std::iota(zcol.begin(), zcol.end(), std::size_t(0));
+    }
+  }
+
   std::iota(idx.begin(), idx.end(), 0u);
   if (order == 0)
     std::stable_sort(idx.begin(), idx.end(), [&](unsigned x, unsigned y) { return std::abs(w[x]) > std::abs(w[y]); });
@@ -55,12 +95,15 @@ inline void rayleigh_ritz(unsigned b, std::size_t nb, std::size_t bc, const doub
   for (std::size_t i = 0; i < nb; ++i)
     for (std::size_t j = 0; j < nb; ++j)
       for (unsigned bi = 0; bi < b; ++bi)
-        for (unsigned bj = 0; bj < b; ++bj) out.Y[((i * bc + j) * b + bi) * b + bj] = Z[(i * b + bi) * nt + idx[j * b + bj]];
+        for (unsigned bj = 0; bj < b; ++bj) {
+          const std::size_t col = j * b + bj;
+          if (col < ncols) out.Y[((i * bc + j) * b + bi) * b + bj] = Z[(i * b + bi) * zld + zcol[idx[col]]];
+        }
 
   out.nconv = 0;
   const std::size_t last = nb - 1;
   const double eps = std::numeric_limits<double>::epsilon();
-  const std::size_t n_check = std::min(nev, nt);
+  const std::size_t n_check = std::min(std::min(nev, nt), ncols);
   out.residual.assign(n_check, 0.0);
   for (std::size_t col = 0; col < n_check; ++col) {
     const std::size_t blk = col / b, cb = col % b;

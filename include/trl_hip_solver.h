@@ -147,6 +147,13 @@ int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t* n, const 
 /* Host FP64 symmetric eigensolver used by the Rayleigh-Ritz step:
  * A (n x n row-major) -> ascending w, eigenvectors in the columns of Z. */
 int trl_hip_dense_syev(int n, const double* A, double* w, double* Z);
+/* The same eigenvalues (all n, ascending, in w) and only the eigenvectors of
+ * w[sel[0]] < w[sel[1]] < ... (k strictly increasing indices) in the columns
+ * of the row-major n x k matrix Z: inverse iteration on the tridiagonal form
+ * with Gram-Schmidt inside clusters, then the Householder back-transform.
+ * The Rayleigh-Ritz step uses it for the k_r*b Ritz vectors the driver
+ * needs. */
+int trl_hip_dense_syev_select(int n, const double* A, double* w, int k, const int* sel, double* Z);
 
 #ifdef __cplusplus
 }

@@ -64,3 +64,61 @@ def test_dense_syev_degenerate():
     w, Z = dense_syev((Q * d) @ Q.T)
     assert np.allclose(w, np.sort(d), atol=1e-12)
     assert np.abs(Z.T @ Z - np.eye(48)).max() < 1e-12
+
+
+@pytest.mark.parametrize("case", ["random", "degenerate", "laplace2d", "lanczos_t"])
+def test_dense_syev_select(case):
+    # eigenvalues bitwise those of the full solver; selected eigenvectors with
+    # eigen-residuals and orthogonality at rounding level, also inside
+    # clusters of equal eigenvalues; an eigenvector of a simple eigenvalue
+    # equals the full solver's up to sign
+    from trl_amd.solver import dense_syev, dense_syev_select
+
+    rng = np.random.default_rng(7)
+    if case == "random":
+        A = rng.standard_normal((200, 200))
+        A = (A + A.T) / 2
+    elif case == "degenerate":
+        Q, _ = np.linalg.qr(rng.standard_normal((96, 96)))
+        A = (Q * np.repeat(np.arange(24.0), 4)) @ Q.T
+    elif case == "laplace2d":  # multiplicity 2 (lambda_ij = lambda_ji)
+        s = 12
+        T1 = 2 * np.eye(s) - np.eye(s, k=1) - np.eye(s, k=-1)
+        A = np.kron(T1, np.eye(s)) + np.kron(np.eye(s), T1)
+    else:  # block tridiagonal b = 4 plus an arrow, the shape of T after a restart
+        b, m, kr = 4, 16, 3
+        A = np.zeros((b * m, b * m))
+        A[: kr * b, : kr * b] = np.diag(np.sort(rng.uniform(0, 1, kr * b)))
+        for i in range(kr, m):
+            Bi = rng.standard_normal((b, b))
+            A[i * b:(i + 1) * b, i * b:(i + 1) * b] = Bi + Bi.T
+            if i + 1 < m:
+                C = np.triu(rng.standard_normal((b, b)))
+                A[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b] = C
+                A[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b] = C.T
+        arrow = rng.standard_normal((b, kr * b))
+        A[kr * b:(kr + 1) * b, : kr * b] = arrow
+        A[: kr * b, kr * b:(kr + 1) * b] = arrow.T
+    n = A.shape[0]
+    wf, Zf = dense_syev(A)
+    sel = np.r_[np.arange(0, 20), np.arange(n - 6, n)]
+    w, Z = dense_syev_select(A, sel)
+    assert np.array_equal(w, wf)
+    scale = np.abs(A).max()
+    assert np.abs(A @ Z - Z * w[sel]).max() < 1e-12 * scale * n
+    assert np.abs(Z.T @ Z - np.eye(len(sel))).max() < 1e-12 * n
+    gaps = np.minimum(np.abs(np.diff(wf, prepend=-np.inf)), np.abs(np.diff(wf, append=np.inf)))
+    for c, j in enumerate(sel):
+        if gaps[j] > 1e-6 * scale:
+            assert abs(abs(Z[:, c] @ Zf[:, j]) - 1) < 1e-10
+
+
+def test_dense_syev_select_arguments():
+    from trl_amd import _lib as L
+
+    A = np.eye(4)
+    w = np.empty(4)
+    Z = np.empty((4, 2))
+    bad = np.array([2, 1], np.int32)  # not increasing
+    assert L.solver().trl_hip_dense_syev_select(4, A.ctypes.data, w.ctypes.data, 2, bad.ctypes.data, Z.ctypes.data) == -1
+    assert L.solver().trl_hip_dense_syev_select(4, A.ctypes.data, w.ctypes.data, 5, bad.ctypes.data, Z.ctypes.data) == -1

@@ -316,6 +316,7 @@ SOLVER_API = [
                                         C.POINTER(SolverParams), C.POINTER(MixedParams), c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(SolverResult),
                                         C.POINTER(SolverResult)]),
     ("trl_hip_dense_syev", c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_dense_syev_select", c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_batch_solve_csr", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(BatchParams), c_void_p, c_void_p,
                                         c_void_p, c_void_p, C.POINTER(BatchStats)]),
 ]

@@ -146,7 +146,7 @@ extern "C" int trl_hip_batch_solve_csr(trl_hip_ctx* ctx, int nprob, const size_t
       q.cycles = cycles;
       const double* blast = beta.data() + (size_t(p) * m + (m - 1)) * BB;
       try {
-        trl_amd::rayleigh_ritz(unsigned(b), m, m, q.T.data(), blast, nev, prm->order, prm->tol, q.rr, q.dense);
+        trl_amd::rayleigh_ritz(unsigned(b), m, m, q.T.data(), blast, nev, prm->order, prm->tol, q.rr, q.dense, size_t(kr) * b);
       }
       catch (const std::exception&) {
         q.error = TRL_HIP_EEIG;

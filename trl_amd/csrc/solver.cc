@@ -107,6 +107,7 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
   evp->set_order(static_cast<trl::hip::EigenvalueOrder>(p.order));
   evp->set_operator(op);
   if (p.cholqr_passes > 0) evp->set_cholqr_passes(p.cholqr_passes);
+  if (X_out) evp->set_ritz_blocks(nblk_out); // the mixed solve keeps more Ritz blocks than the driver needs
   trl_amd::EigensolverParams ep{p.nev, p.ncv, p.max_restarts, p.tol};
 
   auto run = [&](auto& lz) {
@@ -423,6 +424,13 @@ int trl_hip_dense_syev(int n, const double* A, double* w, double* Z)
 {
   if (n < 0 || (n > 0 && (!A || !w || !Z))) return TRL_HIP_EINVAL;
   return trl_amd::dense::syev(n, A, w, Z) == 0 ? 0 : TRL_HIP_EEIG;
+}
+
+int trl_hip_dense_syev_select(int n, const double* A, double* w, int k, const int* sel, double* Z)
+{
+  if (n < 0 || k < 0 || k > n || (n > 0 && (!A || !w)) || (k > 0 && (!sel || !Z))) return TRL_HIP_EINVAL;
+  const int rc = trl_amd::dense::syev_select(n, A, w, k, sel, Z);
+  return rc == 0 ? 0 : (rc == -2 ? TRL_HIP_EINVAL : TRL_HIP_EEIG);
 }
 
 int trl_hip_solve_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, const void* mass,

@@ -194,3 +194,15 @@ def dense_syev(A: np.ndarray):
     Z = np.empty((n, n))
     L.check(L.solver().trl_hip_dense_syev(n, A.ctypes.data, w.ctypes.data, Z.ctypes.data), "trl_hip_dense_syev")
     return w, Z
+
+
+def dense_syev_select(A: np.ndarray, sel):
+    """All eigenvalues and the eigenvectors of the selected ones (for tests)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    sel = np.ascontiguousarray(sel, dtype=np.int32)
+    w = np.empty(n)
+    Z = np.empty((n, len(sel)))
+    L.check(L.solver().trl_hip_dense_syev_select(n, A.ctypes.data, w.ctypes.data, len(sel), sel.ctypes.data, Z.ctypes.data),
+            "trl_hip_dense_syev_select")
+    return w, Z
