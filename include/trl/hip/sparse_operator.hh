@@ -33,8 +33,9 @@ public:
 
   // Times every kernel for block size b, on CSR and, when the longest row
   // pads the ELL form by at most max_padding (slots / nnz - 1), on ELL, and
-  // keeps the fastest.  Returns its time per call in microseconds.
-  double tune(int b, bool allow_ell = true, double max_padding = 0.25)
+  // keeps the fastest, for the plain product and (axpby) the fused axpby
+  // form.  Returns the plain product's time per call in microseconds.
+  double tune(int b, bool allow_ell = true, double max_padding = 0.25, bool axpby = true)
   {
     if (n_ == 0) return 0.0;
     const trl_hip_spmat* ell = nullptr;
@@ -54,7 +55,12 @@ public:
     }
     double us = 0.0;
     throw_on(trl_hip_spmm_tune(ctx_.get(), dtype, n_, b, &csr_, ell, &op_, &us), "spmm_tune");
-    throw_on(trl_hip_spmm_tune_axpby(ctx_.get(), dtype, n_, b, &csr_, ell, &op_axpby_, &axpby_us_), "spmm_tune_axpby");
+    if (axpby)
+      throw_on(trl_hip_spmm_tune_axpby(ctx_.get(), dtype, n_, b, &csr_, ell, &op_axpby_, &axpby_us_), "spmm_tune_axpby");
+    else {
+      op_axpby_ = op_; // untimed: the plain product's choice
+      axpby_us_ = 0.0;
+    }
     if (op_.format == 0 && op_axpby_.format == 0) ell_.reset(); // the ELL copy is only kept while it is used
     tuned_b_ = b;
     tuned_us_ = us;
@@ -101,6 +107,12 @@ public:
   {
     const trl_hip_spmat& d = panels_n_ > 1 && npanels > 1 ? op_panels_ : op_axpby_; // the panel form, tuned at panels_n_ panels
     throw_on(trl_hip_spmm_axpby_panels(ctx_.get(), dtype, n_, b, &d, npanels, X, Y, Z, alpha, beta, gamma), "apply_axpby_panels");
+  }
+  // The same plus the forcing term F_j diag(g_j) (g: device, npanels*b).
+  void axpby_forced(int b, int npanels, const S* X, S* Y, const S* Z, double alpha, double beta, double gamma, const S* F, const double* g) const
+  {
+    const trl_hip_spmat& d = panels_n_ > 1 && npanels > 1 ? op_panels_ : op_axpby_;
+    throw_on(trl_hip_spmm_axpby_forced(ctx_.get(), dtype, n_, b, &d, npanels, X, Y, Z, alpha, beta, gamma, F, g), "apply_axpby_forced");
   }
 
   const trl_hip_spmat& descriptor() const { return op_; }

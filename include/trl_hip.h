@@ -189,6 +189,26 @@ int trl_hip_spmm_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, cons
 int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y, const void* Z,
                               double alpha, double beta, double gamma);
 
+/* The panel step with a forcing term: Y_j = alpha diag(dscale) A X_j +
+ * beta X_j + gamma Z_j + F_j diag(g_j) for j < npanels, F laid out as X,
+ * g device doubles, g[j*b + c] the coefficient of column c of panel j.  The
+ * step of the residual-driven Chebyshev filter (E_i = p_i(A) x - p_i(theta)
+ * x for a Ritz pair (theta, x), forced by the residual A x - theta x), which
+ * lets an FP32 recurrence refine FP64 Ritz vectors.  No reference
+ * counterpart. */
+int trl_hip_spmm_axpby_forced(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y, const void* Z,
+                              double alpha, double beta, double gamma, const void* F, const double* g);
+
+/* Residual panels of Ritz pairs for that filter: R_j = AX_j - X_j
+ * diag(theta_j) computed in FP64 (AX, X: npanels FP64 panels; theta: device,
+ * npanels*b) and stored in FP32 as R and, times s, as E. */
+int trl_hip_residual_split(trl_hip_ctx* ctx, size_t n, int b, int npanels, const double* AX, const double* X, const double* theta, double s, float* R,
+                           float* E);
+
+/* X_j = X_j diag(h_j) + E_j for j < npanels (X FP64, E FP32, h device
+ * doubles, npanels*b): the filtered Ritz vectors p(A) x = p(theta) x + E. */
+int trl_hip_panel_combine(trl_hip_ctx* ctx, size_t n, int b, int npanels, double* X, const double* h, const float* E);
+
 /* CSR convenience forms of the two calls above (variant as in trl_hip_spmat). */
 int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
                      const void* dscale, int variant);

@@ -79,6 +79,10 @@ typedef struct {
                                 degree from the measured convergence rate of its
                                 vectors (0 when converged) in every sweep, 2 only
                                 in the FP64 sweeps, 0: cheb_degree on all         */
+  int residual_filter;       /* refine = 1 with FP32 sweeps: 1 runs the FP64 sweeps'
+                                filter in FP32, on E = p(A) x - p(theta) x driven
+                                by the FP64 residual A x - theta x of each Ritz
+                                pair (x <- p(theta) x + E in FP64); 0: FP64 filter */
 } trl_hip_mixed_params;
 
 /* Mixed precision.  Phase 1: an FP32 thick-restart solve (FP32 basis, SpMM
@@ -90,7 +94,13 @@ typedef struct {
  *      Rayleigh-Ritz, and while a true residual exceeds p->tol, a scaled
  *      Chebyshev filter of degree cheb_degree (Zhou & Saad, J. Comput. Phys.
  *      2007) that damps the spectrum beyond the kept Ritz values, then
- *      Rayleigh-Ritz again; after max_refine sweeps it falls back to 0;
+ *      Rayleigh-Ritz again; after max_refine sweeps it falls back to 0.
+ *      With mp->residual_filter the filter of the FP64 sweeps runs in FP32
+ *      panels: for a Ritz pair (theta, x) with residual r = A x - theta x,
+ *      E_i = p_i(A) x - p_i(theta) x obeys the filter's three-term
+ *      recurrence plus alpha_i p_{i-1}(theta) r, and |E_i| is of the order
+ *      of the error of x, so FP32 rounding in E stays below the FP64
+ *      tolerance; x <- p(theta) x + E is formed in FP64;
  *   0: an FP64 thick-restart solve started from V0 = X G, G a seeded
  *      Gaussian (nev/b+1)*b x b mixing matrix.
  * Order must be ascending or descending for refine = 1.

@@ -171,6 +171,65 @@ def test_spmm_panels(ctx, rng, p, b, fmt, variant):
 
 
 @pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b", [1, 4, 8])
+@pytest.mark.parametrize("fmt,variant", [(0, 0), (0, 4), (1, 1), (1, -2), (1, -4)])
+@pytest.mark.parametrize("npan", [1, 3])
+def test_spmm_forced(ctx, rng, p, b, fmt, variant, npan):
+    # the step with a forcing term, Y_j = alpha d A X_j + beta X_j + gamma Z_j
+    # + F_j diag(g_j), against the oracle's product; Z may be absent
+    if variant < 0 and not split_ok(p, b, -variant):
+        pytest.skip("column split needs 8 bytes per lane")
+    A = P.laplacian_3d(9, 7, 6, rng.uniform(0, 1, 378))
+    dt = DT[p]
+    X = rng.standard_normal((npan, A.n, b)).astype(dt)
+    Z = rng.standard_normal((npan, A.n, b)).astype(dt)
+    F = rng.standard_normal((npan, A.n, b)).astype(dt)
+    g = rng.standard_normal((npan, b))
+    d = rng.uniform(0.5, 2, A.n).astype(dt)
+    rp, ci, va = dev(ctx, A.rowptr, np.int32), dev(ctx, A.col, np.int32), dev(ctx, A.val, dt)
+    width, hcol, hval = csr_to_ell(A)
+    ec, ev, dd = dev(ctx, hcol, np.int32), dev(ctx, hval, dt), dev(ctx, d, dt)
+    M = spmat(0, rp, 0, ci, va, dd, variant) if fmt == 0 else spmat(1, None, width, ec, ev, dd, variant)
+    Xd, Zd, Fd, gd = dev(ctx, X, dt), dev(ctx, Z, dt), dev(ctx, F, dt), dev(ctx, g, np.float64)
+    Y = ctx.empty((npan, A.n, b), dt)
+    for with_z in (True, False):
+        L.check(H().trl_hip_spmm_axpby_forced(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), npan, Xd.ptr, Y.ptr, Zd.ptr if with_z else None, 0.5, -1.0,
+                                              0.25, Fd.ptr, gd.ptr), "forced")
+        got = Y.download()
+        for j in range(npan):
+            ref = 0.5 * oracle.spmm(A, X[j].astype(np.float64), d.astype(np.float64)) - X[j] + F[j] * g[j]
+            if with_z:
+                ref = ref + 0.25 * Z[j]
+            assert rel(got[j], ref) < TOL[p], (j, with_z)
+    # F overlapping Y, or a missing coefficient array, is refused
+    assert H().trl_hip_spmm_axpby_forced(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), npan, Xd.ptr, Y.ptr, None, 1.0, 0.0, 0.0, Y.ptr, gd.ptr) == -1
+    assert H().trl_hip_spmm_axpby_forced(ctx.h, L.dtype_code(dt), A.n, b, C.byref(M), npan, Xd.ptr, Y.ptr, None, 1.0, 0.0, 0.0, Fd.ptr, None) == -1
+
+
+@pytest.mark.parametrize("b", [1, 6, 8])
+def test_residual_split_and_combine(ctx, rng, b):
+    # R = AX - X diag(theta) in FP64 stored in FP32 (and times s), then
+    # X diag(h) + E in FP64
+    n, npan = 1000, 3
+    AX = rng.standard_normal((npan, n, b))
+    X = rng.standard_normal((npan, n, b))
+    th = rng.uniform(-2, 2, (npan, b))
+    AXd, Xd, thd = dev(ctx, AX, np.float64), dev(ctx, X, np.float64), dev(ctx, th, np.float64)
+    R, E = ctx.empty((npan, n, b), np.float32), ctx.empty((npan, n, b), np.float32)
+    L.check(H().trl_hip_residual_split(ctx.h, n, b, npan, AXd.ptr, Xd.ptr, thd.ptr, -0.75, R.ptr, E.ptr), "split")
+    ref = AX - X * th[:, None, :]
+    # FP32 rounding of the FP64 value (an FMA may move the FP64 value by an ulp)
+    assert np.all(np.abs(R.download() - ref) <= 1.2e-7 * np.abs(ref))
+    assert np.all(np.abs(E.download() + 0.75 * ref) <= 1.2e-7 * np.abs(0.75 * ref))
+    h = rng.uniform(0.5, 1.5, (npan, b))
+    hd = dev(ctx, h, np.float64)
+    Ef = E.download().astype(np.float64)
+    L.check(H().trl_hip_panel_combine(ctx.h, n, b, npan, Xd.ptr, hd.ptr, E.ptr), "combine")
+    assert np.abs(Xd.download() - (X * h[:, None, :] + Ef)).max() <= 1e-15 * np.abs(X).max() * 4
+    assert H().trl_hip_panel_combine(ctx.h, n, 5, npan, Xd.ptr, hd.ptr, E.ptr) == -1
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
 @pytest.mark.parametrize("b", [1, 8])
 def test_spmm_tune(ctx, rng, p, b):
     # the tuner returns a descriptor that computes the same product

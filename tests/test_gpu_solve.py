@@ -164,7 +164,8 @@ def test_mixed_refine_laplace3d(ctx, phase1_fp64, reorth):
 
 
 @pytest.mark.parametrize("degree_opt", [0, 1, 2])
-def test_mixed_fp32_refinement_sweeps(ctx, degree_opt):
+@pytest.mark.parametrize("residual_filter", [False, True])
+def test_mixed_fp32_refinement_sweeps(ctx, degree_opt, residual_filter):
     # FP32 Lanczos to a loose 5e-2, FP32 Chebyshev/Rayleigh-Ritz sweeps down to
     # the FP32 floor, then FP64 sweeps (every block at the full degree, or each
     # block at the degree its convergence rate asks for, in every sweep or in
@@ -175,8 +176,8 @@ def test_mixed_fp32_refinement_sweeps(ctx, degree_opt):
     prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
     o64 = solve(dA, c.b, prm, V0, reorth="dgks")
     om = solve_mixed(dA, c.b, prm, V0, tol_fp32=5e-2, cheb_degree=60, refine_blocks=9, max_refine=14, fp32_sweeps=8, tol_fp32_refine=3e-5,
-                     reorth="dgks", spmm_tune=True, degree_opt=degree_opt)
-    print("mixed with fp32 sweeps", degree_opt, om.result, om.result32)
+                     reorth="dgks", spmm_tune=True, degree_opt=degree_opt, residual_filter=residual_filter)
+    print("mixed with fp32 sweeps", degree_opt, residual_filter, om.result, om.result32)
     assert om.result["converged"] == 1
     assert om.resid.max() <= 1e-8
     assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))
@@ -204,17 +205,19 @@ def test_mixed_filter_all_blocks_per_launch(ctx):
     assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))
 
 
-def test_mixed_bench_schedule(ctx):
+@pytest.mark.parametrize("residual_filter", [False, True])
+def test_mixed_bench_schedule(ctx, residual_filter):
     # the benchmark's schedule on a 48^3 stand-in: one loose FP32 Lanczos
     # cycle, FP32 sweeps on 7 blocks, FP64 sweeps with per-block degrees
+    # (their filter in FP64, or in FP32 driven by the FP64 residuals)
     c = P.config_c3(48)
     dA = DeviceCSR(ctx, c.csr)
     V0 = v0(ctx, c.csr.n, c.b)
     prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
     o64 = solve(dA, c.b, prm, V0, reorth="dgks", spmm_tune=True)
     om = solve_mixed(dA, c.b, prm, V0, tol_fp32=0.4, cheb_degree=120, refine_blocks=7, max_refine=14, fp32_sweeps=10, tol_fp32_refine=3e-5,
-                     reorth="dgks", spmm_tune=True, degree_opt=2)
-    print("bench schedule", om.result, om.result32)
+                     reorth="dgks", spmm_tune=True, degree_opt=2, residual_filter=residual_filter)
+    print("bench schedule", residual_filter, om.result, om.result32)
     assert om.result["converged"] == 1
     assert om.resid.max() <= 1e-8
     assert np.all(np.abs(om.evals - o64.evals) <= 1e-10 * np.abs(o64.evals))
@@ -231,5 +234,13 @@ def test_mixed_generalized(ctx):
     om = solve_mixed(dA, c.b, EigensolverParams(c.nev, c.ncv), v0(ctx, c.csr.n, c.b), tol_fp32=1e-4, cheb_degree=40)
     ref = np.array(g["eigenvalues"])
     print("mixed generalized", om.result, om.result32, (np.abs(om.evals - ref) / ref).max())
+    assert np.all(np.abs(om.evals - ref) <= 1e-10 * ref)
+    assert om.resid.max() <= 1e-8
+    # FP32 sweeps, then the FP64 sweeps' filter in FP32 on M^{-1} K driven by
+    # the FP64 residuals
+    om = solve_mixed(dA, c.b, EigensolverParams(c.nev, c.ncv), v0(ctx, c.csr.n, c.b), tol_fp32=1e-2, cheb_degree=40, max_refine=14, fp32_sweeps=6,
+                     tol_fp32_refine=3e-5, residual_filter=True)
+    print("mixed generalized, residual filter", om.result, om.result32, (np.abs(om.evals - ref) / ref).max())
+    assert om.result["converged"] == 1
     assert np.all(np.abs(om.evals - ref) <= 1e-10 * ref)
     assert om.resid.max() <= 1e-8

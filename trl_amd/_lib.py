@@ -64,6 +64,10 @@ HIP_API = [
     ("trl_hip_spmm_panels", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_spmm_axpby_panels", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_int, c_void_p, c_void_p, c_void_p, C.c_double, C.c_double,
                                           C.c_double]),
+    ("trl_hip_spmm_axpby_forced", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_int, c_void_p, c_void_p, c_void_p, C.c_double, C.c_double,
+                                          C.c_double, c_void_p, c_void_p]),
+    ("trl_hip_residual_split", c_int, [c_void_p, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, C.c_double, c_void_p, c_void_p]),
+    ("trl_hip_panel_combine", c_int, [c_void_p, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_csr_max_row", c_int, [c_void_p, c_size, c_void_p, c_int_p]),
     ("trl_hip_csr_to_ell", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_spmm_tune", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_spmat_p, c_spmat_p, c_double_p]),
@@ -292,7 +296,7 @@ class SolverResult(C.Structure):
 class MixedParams(C.Structure):
     _fields_ = [("tol_fp32", C.c_double), ("max_restarts_fp32", C.c_uint), ("seed", C.c_ulonglong), ("refine", c_int), ("cheb_degree", C.c_uint),
                 ("max_refine", C.c_uint), ("phase1_fp64", c_int), ("refine_blocks", C.c_uint), ("fp32_sweeps", C.c_uint), ("tol_fp32_refine", C.c_double),
-                ("degree_opt", c_int)]
+                ("degree_opt", c_int), ("residual_filter", c_int)]
 
 
 class BatchParams(C.Structure):

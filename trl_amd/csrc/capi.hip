@@ -1,6 +1,8 @@
 // C-ABI entry points of libtrl_hip.so: argument checks, context and memory
 // management, dispatch to the kernel families.
+#include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -440,6 +442,39 @@ int trl_hip_spmm_axpby_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b
   return launch_spmm(ctx, t, n, b, *A, X, Y, Z, true, alpha, beta, gamma, npanels);
 }
 
+int trl_hip_spmm_axpby_forced(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* A, int npanels, const void* X, void* Y, const void* Z,
+                              double alpha, double beta, double gamma, const void* F, const double* g)
+{
+  if (!ctx || !valid_b(b) || npanels < 1 || check_spmat(n, A) || (n && (!X || !Y || !F || !g))) return TRL_HIP_EINVAL;
+  const size_t es = t == TRL_HIP_F32 ? 4 : 8;
+  const char *x = static_cast<const char*>(X), *y = static_cast<const char*>(Y), *f = static_cast<const char*>(F);
+  const size_t span = size_t(npanels) * n * b * es;
+  if (x < y + span && y < x + span) return TRL_HIP_EINVAL; // X and Y must not overlap
+  if (f < y + span && y < f + span) return TRL_HIP_EINVAL; // nor F and Y
+  if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_SPMM, ctx->prof_on ? spmm_bytes(ctx, t, n, b, A, 0) + (Z ? 4.0 : 3.0) * npanels * panel_bytes(t, n, b) : 0.0, t);
+  return launch_spmm(ctx, t, n, b, *A, X, Y, Z, true, alpha, beta, gamma, npanels, F, g);
+}
+
+int trl_hip_residual_split(trl_hip_ctx* ctx, size_t n, int b, int npanels, const double* AX, const double* X, const double* theta, double s, float* R,
+                           float* E)
+{
+  if (!ctx || !valid_b(b) || npanels < 1 || npanels > 65535 || (n && (!AX || !X || !theta || !R || !E))) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_PANEL, ctx->prof_on ? double(npanels) * (2.0 * panel_bytes(TRL_HIP_F64, n, b) + 2.0 * panel_bytes(TRL_HIP_F32, n, b)) : 0.0,
+          TRL_HIP_F64);
+  return launch_residual_split(ctx, n, b, npanels, AX, X, theta, s, R, E);
+}
+
+int trl_hip_panel_combine(trl_hip_ctx* ctx, size_t n, int b, int npanels, double* X, const double* h, const float* E)
+{
+  if (!ctx || !valid_b(b) || npanels < 1 || npanels > 65535 || (n && (!X || !h || !E))) return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  Prof pr(ctx, TRL_HIP_PROF_PANEL, ctx->prof_on ? double(npanels) * (2.0 * panel_bytes(TRL_HIP_F64, n, b) + panel_bytes(TRL_HIP_F32, n, b)) : 0.0,
+          TRL_HIP_F64);
+  return launch_combine(ctx, n, b, npanels, X, h, E);
+}
+
 int trl_hip_spmm_csr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const int32_t* rowptr, const int32_t* col, const void* val, const void* X, void* Y,
                      const void* dscale, int variant)
 {
@@ -499,7 +534,8 @@ namespace {
 //     the matrix bandwidth, blocks of about 1/4, 1/8, 1/16 of a plane).
 // Several panels: only ELL rows of one chunk share the matrix between
 // panels, so the ELL variants (one lane per row only for narrow rows) are
-// tried in index order and in blocks of 1/8 of a plane.
+// tried in blocks of 1/8 of a plane (index order when the matrix has no far
+// off-diagonals), one timed launch each.
 int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, bool axpby, int npanels,
                    trl_hip_spmat* best, double* best_us)
 {
@@ -508,10 +544,14 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
   *best = *csr;
   if (best_us) *best_us = 0.0;
   if (!n) return 0;
+  const bool verbose = std::getenv("TRL_HIP_VERBOSE") != nullptr;
+  using clock = std::chrono::steady_clock;
+  const auto t_start = clock::now();
   const size_t bytes = size_t(npanels) * n * size_t(b) * (t == TRL_HIP_F32 ? 4 : 8);
   void *X = nullptr, *Y = nullptr;
   int rc = trl_hip_malloc(ctx, bytes, &X);
   if (!rc) rc = trl_hip_malloc(ctx, bytes, &Y);
+  const double alloc_ms = std::chrono::duration<double, std::milli>(clock::now() - t_start).count();
   const void* Z = axpby ? Y : nullptr; // in place, as the Chebyshev filter calls it
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = TRL_HIP_ENOMEM;
@@ -522,7 +562,7 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
       (void)hipGetLastError();
       return -1.0; // variant not applicable to this b / dtype
     }
-    const int reps = npanels > 1 ? 2 : 3;
+    const int reps = npanels > 1 ? 1 : 3;
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return -1.0;
     for (int r = 0; r < reps; ++r) (void)launch_spmm(ctx, t, n, b, A, X, Y, Z, axpby, 0.5, -0.25, 0.125, npanels);
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;
@@ -532,6 +572,9 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
   };
   auto consider = [&](const trl_hip_spmat& A) -> double {
     const double us = time_one(A);
+    if (verbose)
+      std::fprintf(stderr, "[tune] %s b=%d %s x%d %s/v%d/plane%d:%d %.1f us\n", t == TRL_HIP_F32 ? "f32" : "f64", b, axpby ? "axpby" : "apply", npanels,
+                   A.format ? "ell" : "csr", A.variant, A.plane, A.plane_block, us);
     if (us > 0.0 && us < best_t) {
       best_t = us;
       *best = A;
@@ -568,12 +611,15 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
     // one lane per row only for rows of at most 16 bytes (wider rows hold
     // too many registers across the panel loop)
     const bool narrow = size_t(b) * (t == TRL_HIP_F32 ? 4 : 8) <= 16;
+    // index order only where no blocked order applies: the rows of the
+    // panels gathered between two uses of a row two planes apart do not
+    // fit in L2 (measured on the 160^3 stencil, 7 panels: 1.14x to 1.36x
+    // slower than blocks of 1/8 of a plane for every variant)
     if (ell)
       for (int v : {-2, -4, 1}) {
         if (v == 1 && !narrow) continue;
-        consider(with(*ell, v, 0, 0));
         const int pb = block_of(v, 8);
-        if (pb) consider(with(*ell, v, bw, pb));
+        consider(pb ? with(*ell, v, bw, pb) : with(*ell, v, 0, 0));
       }
     else
       consider(with(*csr, 0, 0, 0));
@@ -601,6 +647,9 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
   if (e1) (void)hipEventDestroy(e1);
   if (X) trl_hip_free(ctx, X);
   if (Y) trl_hip_free(ctx, Y);
+  if (verbose)
+    std::fprintf(stderr, "[tune] %s b=%d %s x%d: %.1f ms (scratch %.1f ms)\n", t == TRL_HIP_F32 ? "f32" : "f64", b, axpby ? "axpby" : "apply", npanels,
+                 std::chrono::duration<double, std::milli>(clock::now() - t_start).count(), alloc_ms);
   return rc;
 }
 

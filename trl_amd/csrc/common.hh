@@ -74,8 +74,10 @@ int launch_dgks(trl_hip_ctx* ctx, int b, int k, const double* C, double eta2, in
 int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes);
 int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int ybc, int kc, void* Out);
 bool valid_spmm_variant(int v);
+// F, g (both or neither, axpby only): the forcing term + F_p diag(g_p), F
+// laid out as X, g device doubles, b per panel
 int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat& A, const void* X, void* Y, const void* Z, bool axpby, double alpha,
-                double beta, double gamma, int npanels = 1);
+                double beta, double gamma, int npanels = 1, const void* F = nullptr, const double* g = nullptr);
 // rows per workgroup of the one-panel (panels = false) or several-panel ELL
 // product kernel of this variant
 int spmm_rows_per_block(trl_hip_dtype t, int b, int variant, bool panels = false);
@@ -84,6 +86,9 @@ int launch_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* d
 int launch_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ecol,
                       void* eval);
 int launch_small(trl_hip_ctx* ctx, int op, int b, const double* A, const double* B, double* C, const double* hostvals);
+int launch_residual_split(trl_hip_ctx* ctx, size_t n, int b, int npanels, const double* AX, const double* X, const double* theta, double s, float* R,
+                          float* E);
+int launch_combine(trl_hip_ctx* ctx, size_t n, int b, int npanels, double* X, const double* h, const float* E);
 int launch_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* md, const double* lambda, double* out);
 
 inline int hip_status(hipError_t e) { return e == hipSuccess ? 0 : static_cast<int>(e); }

@@ -167,6 +167,53 @@ __global__ __launch_bounds__(kThreads) void k_residual(const S* __restrict__ X, 
   }
 }
 
+// Residual panels of Ritz pairs: R_p = AX_p - X_p diag(theta_p) in FP64,
+// stored in FP32 as R and, times s, as E (panel p = blockIdx.y).
+template <int B>
+__global__ __launch_bounds__(kThreads) void k_residual_split(const double* __restrict__ AX, const double* __restrict__ X, const double* __restrict__ theta,
+                                                             double s, size_t n, float* __restrict__ R, float* __restrict__ E)
+{
+  const size_t po = size_t(blockIdx.y) * n * B;
+  double th[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) th[c] = theta[size_t(blockIdx.y) * B + c];
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += stride) {
+    double ax[B], x[B];
+    RowIO<double, B>::template load<true>(AX + po + r * B, ax);
+    RowIO<double, B>::template load<true>(X + po + r * B, x);
+    float rr[B], ee[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+      const double v = ax[c] - th[c] * x[c];
+      rr[c] = float(v);
+      ee[c] = float(s * v);
+    }
+    RowIO<float, B>::template store<true>(R + po + r * B, rr);
+    RowIO<float, B>::template store<true>(E + po + r * B, ee);
+  }
+}
+
+// X_p = X_p diag(h_p) + E_p, E in FP32 (panel p = blockIdx.y).
+template <int B>
+__global__ __launch_bounds__(kThreads) void k_combine(double* __restrict__ X, const double* __restrict__ h, const float* __restrict__ E, size_t n)
+{
+  const size_t po = size_t(blockIdx.y) * n * B;
+  double hh[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) hh[c] = h[size_t(blockIdx.y) * B + c];
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += stride) {
+    double x[B];
+    float e[B];
+    RowIO<double, B>::template load<true>(X + po + r * B, x);
+    RowIO<float, B>::template load<true>(E + po + r * B, e);
+#pragma unroll
+    for (int c = 0; c < B; ++c) x[c] = hh[c] * x[c] + double(e[c]);
+    RowIO<double, B>::template store<true>(X + po + r * B, x);
+  }
+}
+
 __global__ void k_sum_partials(const double* __restrict__ partials, int nwg, int P, double* __restrict__ out)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -190,6 +237,23 @@ int launch_fill_zero(trl_hip_ctx* ctx, void* dst, size_t bytes)
   else {
     hipLaunchKernelGGL(k_fill_zero1, dim3(grid_for(bytes, kThreads, 4096)), dim3(kThreads), 0, ctx->stream, static_cast<unsigned char*>(dst), bytes);
   }
+  TRL_HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_residual_split(trl_hip_ctx* ctx, size_t n, int b, int npanels, const double* AX, const double* X, const double* theta, double s, float* R,
+                          float* E)
+{
+  const dim3 g(grid_for(n, kThreads, 2048), unsigned(npanels)), blk(kThreads);
+  TRL_DISPATCH_B(b, { hipLaunchKernelGGL((k_residual_split<B>), g, blk, 0, ctx->stream, AX, X, theta, s, n, R, E); });
+  TRL_HIP_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_combine(trl_hip_ctx* ctx, size_t n, int b, int npanels, double* X, const double* h, const float* E)
+{
+  const dim3 g(grid_for(n, kThreads, 2048), unsigned(npanels)), blk(kThreads);
+  TRL_DISPATCH_B(b, { hipLaunchKernelGGL((k_combine<B>), g, blk, 0, ctx->stream, X, h, E, n); });
   TRL_HIP_LAUNCH_CHECK();
   return 0;
 }

@@ -65,6 +65,10 @@ struct Epi {
   // row order: plane = 0 index order; else blocks of pblock rows of each of
   // the nz planes in turn (trl_hip_spmat::plane / plane_block)
   unsigned plane, pblock, nz;
+  // forcing term of the axpby form (F may be null): + F_p diag(g_p), F laid
+  // out as X, g[p * B + c] the coefficient of column c of panel p
+  const void* F;
+  const double* g;
 };
 
 // Row handled by local row slot `slot` (0 .. rows_per_wg-1) of row-range
@@ -106,28 +110,33 @@ __device__ inline T* elem(T* base, Idx<O32> i)
     return base + i;
 }
 
-// AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z (Z
-// may be null).  X, Y, Z point at the lane's panel; ro = row * B + off, the
-// first element of this lane in it.
+// AXPBY = false: Y = d*acc; true: Y = alpha*d*acc + beta*X + gamma*Z
+// + F diag(g) (Z, F may be null).  X, Y, Z, F point at the lane's panel;
+// ro = row * B + off, the first element of this lane in it; g (with F) at
+// the coefficient of the lane's first column.
 // The AXPBY epilogue with the lane's own X row already in registers.
 template <typename S, int BC, bool O32>
-__device__ inline void epilogue_x(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S (&xr)[BC], S* Y, const S* Z, const Epi& e)
+__device__ inline void epilogue_x(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S (&xr)[BC], S* Y, const S* Z, const S* F, const double* g,
+                                  const Epi& e)
 {
   const S* ds = static_cast<const S*>(e.dscale);
-  S zr[BC], out[BC];
+  S zr[BC], fr[BC], out[BC];
   if (Z) RowIO<S, BC>::template load<true>(elem<O32>(Z, ro), zr);
+  if (F) RowIO<S, BC>::template load<true>(elem<O32>(F, ro), fr);
   const double a = ds ? e.alpha * double(ld<true>(elem<O32>(ds, row))) : e.alpha;
 #pragma unroll
   for (int q = 0; q < BC; ++q) {
     double v = a * double(acc[q]) + e.beta * double(xr[q]);
     if (Z) v += e.gamma * double(zr[q]);
+    if (F) v += g[q] * double(fr[q]);
     out[q] = S(v);
   }
   RowIO<S, BC>::template store<true>(elem<O32>(Y, ro), out);
 }
 
 template <typename S, int BC, bool AXPBY, bool O32>
-__device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S* __restrict__ X, S* Y, const S* Z, const Epi& e)
+__device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S* __restrict__ X, S* Y, const S* Z, const S* F, const double* g,
+                                const Epi& e)
 {
   const S* ds = static_cast<const S*>(e.dscale);
   S out[BC];
@@ -137,14 +146,16 @@ __device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, c
     for (int q = 0; q < BC; ++q) out[q] = ds ? acc[q] * d : acc[q];
   }
   else {
-    S xr[BC], zr[BC];
+    S xr[BC], zr[BC], fr[BC];
     RowIO<S, BC>::load(elem<O32>(X, ro), xr);
     if (Z) RowIO<S, BC>::template load<true>(elem<O32>(Z, ro), zr);
+    if (F) RowIO<S, BC>::template load<true>(elem<O32>(F, ro), fr);
     const double a = ds ? e.alpha * double(ld<true>(elem<O32>(ds, row))) : e.alpha;
 #pragma unroll
     for (int q = 0; q < BC; ++q) {
       double v = a * double(acc[q]) + e.beta * double(xr[q]);
       if (Z) v += e.gamma * double(zr[q]);
+      if (F) v += g[q] * double(fr[q]);
       out[q] = S(v);
     }
   }
@@ -208,7 +219,8 @@ __global__ __launch_bounds__((spmm_threads<S, B / CS>())) void k_spmm_rows(const
         for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
     }
   }
-  epilogue<S, BC, AXPBY, O32>(acc, I(row), I(row) * I(B) + I(off), X, Y, static_cast<const S*>(e.Z), e);
+  epilogue<S, BC, AXPBY, O32>(acc, I(row), I(row) * I(B) + I(off), X, Y, static_cast<const S*>(e.Z), static_cast<const S*>(e.F), e.g ? e.g + off : nullptr,
+                              e);
 }
 
 // Several panels of an ELL matrix of at most kChunk slots: each lane loads
@@ -226,7 +238,7 @@ constexpr int panel_threads()
   return 256;
 }
 
-template <typename S, int B, int CS, bool AXPBY, bool O32>
+template <typename S, int B, int CS, bool AXPBY, bool O32, bool FORCE>
 __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panels(int width, const int32_t* __restrict__ col, const S* __restrict__ val, const S* __restrict__ X, S* Y,
                                                          Epi e, int n)
 {
@@ -237,6 +249,9 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
   if (row < 0) return;
   const I ro = I(row) * I(B) + I(off);
   const S* Z = static_cast<const S*>(e.Z);
+  // the forcing term has an instantiation of its own (its registers would
+  // cost the plain step a wave per SIMD)
+  const S* F = FORCE ? static_cast<const S*>(e.F) : nullptr;
   // slots past width act as padding (the row's own X row, value 0), so the
   // panel loop has no branches; their gathers hit the cache
   I cofs[kChunk];
@@ -282,10 +297,10 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
       }
       else
         RowIO<S, BC>::load(elem<O32>(Xp, ro), xr);
-      epilogue_x<S, BC, O32>(acc, I(row), ro, xr, Y + po, Z ? Z + po : nullptr, e);
+      epilogue_x<S, BC, O32>(acc, I(row), ro, xr, Y + po, Z ? Z + po : nullptr, F ? F + po : nullptr, F ? e.g + p * B + off : nullptr, e);
     }
     else
-      epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, Xp, Y + po, Z ? Z + po : nullptr, e);
+      epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, Xp, Y + po, Z ? Z + po : nullptr, nullptr, nullptr, e);
   }
 }
 
@@ -314,7 +329,7 @@ __global__ __launch_bounds__((spmm_threads<S, B>())) void k_spmm_lanes(const int
 #pragma unroll
     for (int q = 0; q < B; ++q) acc[q] += __shfl_xor(acc[q], off, L);
   if (sub != 0) return;
-  epilogue<S, B, AXPBY, O32>(acc, I(row), I(row) * I(B), X, Y, static_cast<const S*>(e.Z), e);
+  epilogue<S, B, AXPBY, O32>(acc, I(row), I(row) * I(B), X, Y, static_cast<const S*>(e.Z), static_cast<const S*>(e.F), e.g, e);
 }
 
 // Row-range workgroups for rows_per_wg rows each (in the blocked order, a
@@ -408,7 +423,10 @@ int launch_ell_panels(trl_hip_ctx* ctx, size_t n, int variant, int width, const 
   do {                                                                                                                                             \
     constexpr unsigned T = panel_threads<S, B / CS_>();                                                                                            \
     const dim3 g{unsigned(row_ranges(n, T / CS_, e))}, blk{T};                                                                                     \
-    hipLaunchKernelGGL((k_spmm_ell_panels<S, B, CS_, AXPBY, O32>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));                      \
+    if (AXPBY && e.F)                                                                                                                              \
+      hipLaunchKernelGGL((k_spmm_ell_panels<S, B, CS_, AXPBY, O32, AXPBY>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));             \
+    else                                                                                                                                           \
+      hipLaunchKernelGGL((k_spmm_ell_panels<S, B, CS_, AXPBY, O32, false>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));             \
   } while (0)
   if (cs == 1)
     TRL_PANELS(1);
@@ -444,6 +462,10 @@ int launch_panels(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr
     Epi ep = e;
     ep.npan = 1;
     if (ep.Z) ep.Z = static_cast<const S*>(ep.Z) + po;
+    if (ep.F) {
+      ep.F = static_cast<const S*>(ep.F) + po;
+      ep.g += size_t(p) * B;
+    }
     const int rc = launch_rows<S, B, AXPBY, O32>(ctx, n, variant, rowptr, width, col, val, X + po, Y + po, ep);
     if (rc) return rc;
   }
@@ -541,9 +563,10 @@ int launch_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t
 bool valid_spmm_variant(int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == -2 || v == -4; }
 
 int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat& A, const void* X, void* Y, const void* Z, bool axpby, double alpha,
-                double beta, double gamma, int npanels)
+                double beta, double gamma, int npanels, const void* F, const double* g)
 {
-  Epi e{A.dscale, Z, alpha, beta, gamma, unsigned(npanels), n * size_t(b), unsigned(A.plane), unsigned(A.plane_block), 0};
+  if ((F != nullptr) != (g != nullptr) || (F && !axpby)) return TRL_HIP_EINVAL;
+  Epi e{A.dscale, Z, alpha, beta, gamma, unsigned(npanels), n * size_t(b), unsigned(A.plane), unsigned(A.plane_block), 0, F, g};
   if (A.plane > 0) e.nz = unsigned((n + size_t(A.plane) - 1) / size_t(A.plane));
   const int variant = A.variant;
   // workgroup count bound: at least 256 lanes per workgroup

@@ -11,6 +11,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "trl/hip/csrevp.hh"
@@ -61,14 +62,14 @@ void true_residuals(const Context& c, std::size_t n, const SparseOperator<S>& op
 
 // The operator of one solve: the CSR arrays with the caller's kernel
 // variant, or the fastest kernel and storage for block size B when tuning
-// is asked for.
+// is asked for (the fused axpby form too unless axpby is false).
 template <class S, unsigned B>
 std::shared_ptr<SparseOperator<S>> make_operator(const Context& c, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* minv,
-                                                 const trl_hip_solver_params& p)
+                                                 const trl_hip_solver_params& p, bool axpby = true)
 {
   auto op = std::make_shared<SparseOperator<S>>(c, n, rowptr, col, val, minv, p.spmm_tune ? 0 : p.lanes_per_row);
   const auto t0 = std::chrono::steady_clock::now();
-  if (p.spmm_tune) op->tune(int(B));
+  if (p.spmm_tune) op->tune(int(B), true, 0.25, axpby);
   const double tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (std::getenv("TRL_HIP_VERBOSE")) {
     const auto& d = op->descriptor();
@@ -296,6 +297,117 @@ struct BlockRefiner {
   }
 };
 
+// The FP64 sweeps' Chebyshev filter run in FP32 panels.  For a Ritz pair
+// (theta, x) of the FP64 refiner, r = A x - theta x, and the filter's
+// three-term recurrence Y_i = alpha_i A Y_{i-1} + beta_i Y_{i-1} + gamma_i
+// Y_{i-2} (Y_0 = x) gives for E_i = Y_i - p_i(theta) x
+//   E_i = alpha_i A E_{i-1} + beta_i E_{i-1} + gamma_i E_{i-2} + alpha_i p_{i-1}(theta) r,
+// E_0 = 0, E_1 = alpha_1 r, with p_i the filter polynomial (the same
+// recurrence on scalars).  |E_i| stays of the order of the error of x (the
+// components p_i(lambda) - p_i(theta) are bounded by the scaled filter and
+// multiply small coefficients), so FP32 rounding of E and of the FP32
+// operator contributes orders of magnitude below the error being removed;
+// r and x <- p_d(theta) x + E_d are formed in FP64.
+template <unsigned B>
+struct ResidualFilter {
+  const Context& c;
+  std::size_t n;
+  unsigned nb;
+  const SparseOperator<float>& op32;
+  trl::hip::DeviceBuffer<float> R, Ea, Eb;
+  trl::hip::DeviceBuffer<double> AX, table;
+  unsigned table_steps = 0;
+
+  ResidualFilter(const Context& ctx, std::size_t n_, unsigned nb_, const SparseOperator<float>& o)
+      : c(ctx), n(n_), nb(nb_), op32(o), R(ctx, std::size_t(nb_) * n_ * B), Ea(ctx, std::size_t(nb_) * n_ * B), Eb(ctx, std::size_t(nb_) * n_ * B),
+        AX(ctx, std::size_t(nb_) * n_ * B), table(ctx, 1)
+  {
+  }
+
+  template <class T>
+  static T* panel(T* base, std::size_t n, unsigned j) { return base + std::size_t(j) * n * B; }
+
+  // Filters the Ritz panels rf.X (Ritz values theta, in panel order) with
+  // degree deg[j] on panel j; the degrees are non-decreasing.  Returns the
+  // operator applications (FP64 and FP32).
+  std::size_t filter(BlockRefiner<double, B>& rf, const SparseOperator<double>& op64, const std::vector<unsigned>& deg, const std::vector<double>& theta,
+                     double lo, double hi, double a0)
+  {
+    const double e = 0.5 * (hi - lo), cc = 0.5 * (hi + lo);
+    const double s1 = e / (a0 - cc);
+    const unsigned dmax = deg.back();
+    unsigned j1 = 0; // first panel filtered at all
+    while (j1 < nb && deg[j1] == 0) ++j1;
+    if (dmax == 0 || j1 == nb) return 0;
+    for (unsigned j = 1; j < nb; ++j)
+      if (deg[j] < deg[j - 1]) throw std::invalid_argument("residual filter: degrees must be non-decreasing");
+    // step coefficients of the scaled recurrence (alpha, beta, gamma)
+    std::vector<double> al(dmax + 1), be(dmax + 1), ga(dmax + 1, 0.0);
+    al[1] = s1 / e;
+    be[1] = -s1 * cc / e;
+    double sg = s1;
+    for (unsigned i = 2; i <= dmax; ++i) {
+      const double sn = 1.0 / (2.0 / s1 - sg);
+      al[i] = 2.0 * sn / e;
+      be[i] = -2.0 * sn * cc / e;
+      ga[i] = -sg * sn;
+      sg = sn;
+    }
+    // device table: theta (nb*B), forcing coefficients of steps 2..dmax
+    // (nb*B each) and the final p_d(theta) per column (nb*B)
+    const std::size_t nbB = std::size_t(nb) * B;
+    std::vector<double> host((dmax + 2) * nbB, 0.0);
+    std::copy(theta.begin(), theta.begin() + nbB, host.begin());
+    for (std::size_t k = 0; k < nbB; ++k) {
+      const unsigned dj = deg[k / B];
+      double pm = 1.0, pc = al[1] * theta[k] + be[1]; // p_0, p_1
+      for (unsigned i = 2; i <= dj; ++i) {
+        host[std::size_t(i) * nbB + k] = al[i] * pc; // alpha_i p_{i-1}(theta)
+        const double pn = (al[i] * theta[k] + be[i]) * pc + ga[i] * pm;
+        pm = pc;
+        pc = pn;
+      }
+      host[std::size_t(dmax + 1) * nbB + k] = dj == 0 ? 1.0 : pc;
+    }
+    if (table_steps < dmax + 2) {
+      table = trl::hip::DeviceBuffer<double>(c, host.size());
+      table_steps = dmax + 2;
+    }
+    throw_on(trl_hip_memcpy_h2d(c.get(), table.data(), host.data(), host.size() * sizeof(double)), "residual filter table");
+    const unsigned np1 = nb - j1;
+    // r = A x - theta x in FP64 for the filtered panels; R = r, E_1 = alpha_1 r in FP32
+    for (unsigned j = j1; j < nb; ++j) op64.apply(int(B), rf.panel(rf.X, j), panel(AX.data(), n, j));
+    throw_on(trl_hip_residual_split(c.get(), n, int(B), int(np1), panel(AX.data(), n, j1), rf.panel(rf.X, j1), table.data() + std::size_t(j1) * B, al[1],
+                                    panel(R.data(), n, j1), panel(Eb.data(), n, j1)),
+             "residual split");
+    std::size_t applies = np1;
+    // steps 2..dmax on the panels still being filtered (a suffix): step i
+    // writes Ea for even i, Eb for odd i, and reads the other as E_{i-1};
+    // the older iterate E_{i-2} is the output buffer (E_0 = 0: none at i = 2)
+    for (unsigned i = 2; i <= dmax; ++i) {
+      unsigned j0 = j1;
+      while (deg[j0] < i) ++j0;
+      const int np = int(nb - j0);
+      float* from = panel(i % 2 == 1 ? Ea.data() : Eb.data(), n, j0);
+      float* to = panel(i % 2 == 1 ? Eb.data() : Ea.data(), n, j0);
+      op32.axpby_forced(int(B), np, from, to, i == 2 ? nullptr : to, al[i], be[i], ga[i], panel(R.data(), n, j0),
+                        table.data() + std::size_t(i) * nbB + std::size_t(j0) * B);
+      applies += std::size_t(np);
+    }
+    // x <- p_d(theta) x + E_d, E_d in Ea for even d, Eb for odd d
+    const double* h = table.data() + std::size_t(dmax + 1) * nbB;
+    for (unsigned j = j1; j < nb;) {
+      unsigned k = j;
+      while (k < nb && deg[k] % 2 == deg[j] % 2) ++k;
+      throw_on(trl_hip_panel_combine(c.get(), n, int(B), int(k - j), rf.panel(rf.X, j), h + std::size_t(j) * B,
+                                     panel(deg[j] % 2 == 0 ? Ea.data() : Eb.data(), n, j)),
+               "residual combine");
+      j = k;
+    }
+    return applies;
+  }
+};
+
 // Per-panel filter degrees for the next sweep (degree optimisation as in
 // ChASE: Winkelmann, Springer, Di Napoli, "ChASE: Chebyshev accelerated
 // subspace iteration eigensolver for sequences of Hermitian eigenvalue
@@ -346,10 +458,11 @@ inline std::vector<unsigned> panel_degrees(unsigned nb, unsigned B, unsigned nev
 // degree_opt: per-panel degrees (panel_degrees) instead of `degree` on every
 // panel from the second filter on.  Returns true on convergence; theta and
 // rs hold the last values, applies counts the filter's operator applications.
+// rfilt (FP64 sweeps only, may be null): filter in FP32 through it.
 template <class S, unsigned B>
 bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, const SparseOperator<S>& op, const S* mass, unsigned nev, double tol,
                    unsigned max_sweeps, unsigned degree, bool degree_opt, double lo_all, double hi_all, std::vector<double>& theta,
-                   std::vector<double>& rs, unsigned& sweeps, std::size_t& applies, const char* tag)
+                   std::vector<double>& rs, unsigned& sweeps, std::size_t& applies, const char* tag, ResidualFilter<B>* rfilt = nullptr)
 {
   const double margin = 0.01 * (hi_all - lo_all);
   rs.assign(nev, 0.0);
@@ -379,10 +492,16 @@ bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, cons
       std::fprintf(stderr, "\n");
     }
     const double far = theta.back(); // last kept Ritz value
-    if (rf.ascending)
-      applies += rf.filter(deg, far, hi_all + margin, theta.front());
-    else
-      applies += rf.filter(deg, lo_all - margin, far, theta.front());
+    const double flo = rf.ascending ? far : lo_all - margin, fhi = rf.ascending ? hi_all + margin : far;
+    if constexpr (std::is_same_v<S, double>) {
+      if (rfilt) {
+        applies += rfilt->filter(rf, op, deg, theta, flo, fhi, theta.front());
+        rs_prev = rs;
+        deg_prev = deg;
+        continue;
+      }
+    }
+    applies += rf.filter(deg, flo, fhi, theta.front());
     rs_prev = rs;
     deg_prev = deg;
   }
@@ -492,7 +611,10 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       const unsigned nblk = (p->nev + B - 1) / B;
       c.sync();
       const double t0 = now();
-      const auto op64 = make_operator<double, B>(c, n, rowptr, col, val64, minv64, *p);
+      // the FP64 operator runs the fused filter step only without the
+      // residual-driven FP32 filter
+      const bool rfil = mp->refine == 1 && mp->residual_filter && !mp->phase1_fp64;
+      const auto op64 = make_operator<double, B>(c, n, rowptr, col, val64, minv64, *p, !rfil);
       // ---- FP32 phase -------------------------------------------------------
       trl::hip::DeviceBuffer<float> V0f(c, n * B);
       throw_on(trl_hip_convert(c.get(), TRL_HIP_F64, V0, TRL_HIP_F32, V0f.data(), n * B), "convert V0");
@@ -545,7 +667,9 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       bool done = false;
       if (mp->refine == 1) {
         // ---- FP64 phase: block refinement of the FP32 Ritz blocks -----------
-        if (p->spmm_tune) tune_panels_of(*op64, B, nref);
+        if (p->spmm_tune && !rfil) tune_panels_of(*op64, B, nref);
+        std::unique_ptr<ResidualFilter<B>> rfilt;
+        if (rfil) rfilt = std::make_unique<ResidualFilter<B>>(c, n, nref, *op32);
         BlockRefiner<double, B> rf(c, n, *op64, mass64, nref, asc, eta);
         if (mp->phase1_fp64)
           throw_on(trl_hip_memcpy_d2d(c.get(), rf.X, X1.data(), std::size_t(nref) * n * B * sizeof(double)), "copy X1");
@@ -555,7 +679,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         unsigned sweep = 0;
         std::size_t ap64 = 0;
         done = refine_sweeps<double, B>(rf, c, n, *op64, mass64, p->nev, p->tol, mp->max_refine, mp->cheb_degree, mp->degree_opt != 0, lo_all, hi_all, theta,
-                                        rs, sweep, ap64, "fp64");
+                                        rs, sweep, ap64, "fp64", rfilt.get());
         if (done) {
           for (unsigned i = 0; i < p->nev; ++i) evals[i] = theta[i];
           if (X) throw_on(trl_hip_memcpy_d2d(c.get(), X, rf.X, std::size_t(nblk) * n * B * sizeof(double)), "copy X");
