@@ -8,6 +8,8 @@ Kernels (--what, comma separated):
   spmm32, spmm64        Y = A X with the autotuned operator (FP32 / FP64)
   spmm_axpby32/64       Y = alpha A X + beta X + gamma Y (the Chebyshev step, FP32 / FP64)
   panels32, panels64    the same step on --panels panels per launch (the refinement's filter)
+  forced32              that FP32 step plus the forcing term F diag(g) (the residual-driven
+                        filter of the FP64 sweeps)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
   cholqr32, cholqr64    CholQR2 of one FP32 panel, CholQR of one FP64 panel
   copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
@@ -35,8 +37,8 @@ def main(argv=None) -> int:
     ap.add_argument("--b", type=int, default=8)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10, help="untimed launches before each kernel's timed ones")
-    ap.add_argument("--panels", type=int, default=7, help="panels per launch of panels32/panels64 (the refinement's blocks)")
-    ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,panels32,panels64,gram32,update32,cholqr32,cholqr64,copy")
+    ap.add_argument("--panels", type=int, default=6, help="panels per launch of panels32/panels64 (the refinement's blocks)")
+    ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,panels32,panels64,forced32,gram32,update32,cholqr32,cholqr64,copy")
     a = ap.parse_args(argv)
     what = a.what.split(",")
     cfg = P.config_c3(a.side)
@@ -93,6 +95,22 @@ def main(argv=None) -> int:
                 L.check(H.trl_hip_spmm_axpby_panels(ctx.h, L.dtype_code(dt), n, b, C.byref(d), npan, X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125), "panels")
 
             timed(f"{w}[{op._name(d)}]", fn, mat + 3 * npan * n * b * s)
+        elif w == "forced32":
+            # the residual-driven filter step: + F diag(g) read from a fourth panel set
+            op = dA.operator(b, "f32", tune=True)
+            npan = a.panels
+            op.tune_panels(npan)
+            d = op.desc_panels
+            X = ctx.to_device(rng.standard_normal((npan, n, b)).astype(np.float32))
+            Y = ctx.to_device(rng.standard_normal((npan, n, b)).astype(np.float32))
+            F = ctx.to_device(rng.standard_normal((npan, n, b)).astype(np.float32))
+            g = ctx.to_device(rng.standard_normal((npan, b)))
+            mat = (d.width * n * 8) if d.format == 1 else (A.nnz * 8 + (n + 1) * 4)
+
+            def fn(d=d, X=X, Y=Y, F=F, g=g):
+                L.check(H.trl_hip_spmm_axpby_forced(ctx.h, L.F32, n, b, C.byref(d), npan, X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125, F.ptr, g.ptr), "forced")
+
+            timed(f"{w}[{op._name(d)}]", fn, mat + 4 * npan * n * b * 4)
         elif w in ("gram32", "update32"):
             k = 16
             V = ctx.to_device(rng.standard_normal((k + 1, n, b)).astype(np.float32))
