@@ -216,11 +216,11 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--side", type=int, default=160, help="grid side of the 3D Laplacian (160 = the named workload)")
     ap.add_argument("--tol-fp32", type=float, default=0.4, help="tolerance of the FP32 Lanczos phase (0.4: about one cycle, a starting subspace)")
-    ap.add_argument("--cheb-degree", type=int, default=120)
-    ap.add_argument("--refine-blocks", type=int, default=6, help="Ritz blocks refined together (0: nev/b + 7)")
+    ap.add_argument("--cheb-degree", type=int, default=140)
+    ap.add_argument("--refine-blocks", type=int, default=5, help="Ritz blocks refined together (0: nev/b + 7)")
     ap.add_argument("--max-refine", type=int, default=14, help="FP64 refinement sweeps before the FP64 Lanczos fallback")
-    ap.add_argument("--fp32-sweeps", type=int, default=10, help="FP32 refinement sweeps (at most) before the FP64 ones")
-    ap.add_argument("--tol-fp32-refine", type=float, default=3e-5, help="FP32 residual at which the FP32 sweeps stop")
+    ap.add_argument("--fp32-sweeps", type=int, default=12, help="FP32 refinement sweeps (at most) before the FP64 ones")
+    ap.add_argument("--tol-fp32-refine", type=float, default=1e-4, help="FP32 residual at which the FP32 sweeps stop")
     ap.add_argument("--degree-opt", type=int, default=1, choices=[0, 1, 2],
                     help="per-block filter degrees from measured convergence rates: 0 off, 1 all sweeps, 2 FP64 sweeps only")
     ap.add_argument("--residual-filter", type=int, default=1, choices=[0, 1],

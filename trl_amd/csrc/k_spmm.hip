@@ -49,6 +49,8 @@ constexpr int spmm_threads()
 }
 constexpr unsigned kXcds = 8;
 constexpr int kChunk = 8;
+// forcing coefficients one launch of the several-panel kernel stages in LDS
+constexpr int kForceCoefs = 256;
 
 __device__ inline unsigned xcd_block(unsigned bid, unsigned nb)
 {
@@ -244,6 +246,14 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
 {
   using I = Idx<O32>;
   constexpr int BC = B / CS;
+  // the forcing coefficients of all panels, staged in LDS (npan * B <=
+  // kForceCoefs, checked by the launcher): read just before their use, they
+  // hold no registers across the gathers
+  __shared__ double gsh[FORCE ? kForceCoefs : 1];
+  if constexpr (FORCE) {
+    for (unsigned i = threadIdx.x; i < e.npan * B; i += blockDim.x) gsh[i] = e.g[i];
+    __syncthreads();
+  }
   const int row = row_of(xcd_block(blockIdx.x, gridDim.x), threadIdx.x / CS, blockDim.x / CS, n, e);
   const int off = int(threadIdx.x % CS) * BC;
   if (row < 0) return;
@@ -274,7 +284,14 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
     const S* Xp = X + po;
     S x[kChunk][BC];
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q) RowIO<S, BC>::load(elem<O32>(Xp, cofs[q]), x[q]);
+    for (int q = 0; q < kChunk; ++q) {
+      I o = cofs[q];
+      // O32: the 32-bit offset stays a 32-bit VGPR operand of the load (an
+      // SGPR base plus a VGPR offset) instead of being widened to 64 bits
+      // once outside the panel loop, which would hold two VGPRs per slot
+      if constexpr (O32) asm volatile("" : "+v"(o));
+      RowIO<S, BC>::load(elem<O32>(Xp, o), x[q]);
+    }
     S acc[BC];
 #pragma unroll
     for (int c = 0; c < BC; ++c) acc[c] = S(0);
@@ -283,6 +300,9 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
 #pragma unroll
       for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
     if constexpr (AXPBY) {
+      // the forced step loads F after the products (kept from moving up
+      // among the gathers, where its registers would cost a wave per SIMD)
+      if constexpr (FORCE) __builtin_amdgcn_sched_barrier(0);
       // beta X needs the lane's own X row: one of the gathered rows, unless
       // the row has neither a diagonal entry nor padding
       S xr[BC];
@@ -297,7 +317,7 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
       }
       else
         RowIO<S, BC>::load(elem<O32>(Xp, ro), xr);
-      epilogue_x<S, BC, O32>(acc, I(row), ro, xr, Y + po, Z ? Z + po : nullptr, F ? F + po : nullptr, F ? e.g + p * B + off : nullptr, e);
+      epilogue_x<S, BC, O32>(acc, I(row), ro, xr, Y + po, Z ? Z + po : nullptr, F ? F + po : nullptr, F ? gsh + p * B + off : nullptr, e);
     }
     else
       epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, Xp, Y + po, Z ? Z + po : nullptr, nullptr, nullptr, e);
@@ -456,7 +476,23 @@ int launch_panels(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowptr
                   const Epi& e)
 {
   if (e.npan == 1) return launch_rows<S, B, AXPBY, O32>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
-  if (!rowptr && width <= kChunk && variant <= 1) return launch_ell_panels<S, B, AXPBY, O32>(ctx, n, variant, width, col, val, X, Y, e);
+  if (!rowptr && width <= kChunk && variant <= 1) {
+    // a forced step stages npan * B coefficients in LDS: larger panel counts
+    // go in chunks of panels
+    const unsigned cap = unsigned(kForceCoefs / B);
+    if (!e.F || e.npan <= cap) return launch_ell_panels<S, B, AXPBY, O32>(ctx, n, variant, width, col, val, X, Y, e);
+    for (unsigned p0 = 0; p0 < e.npan; p0 += cap) {
+      const size_t po = size_t(p0) * e.pstride;
+      Epi ep = e;
+      ep.npan = std::min(cap, e.npan - p0);
+      if (ep.Z) ep.Z = static_cast<const S*>(ep.Z) + po;
+      ep.F = static_cast<const S*>(ep.F) + po;
+      ep.g += size_t(p0) * B;
+      const int rc = launch_ell_panels<S, B, AXPBY, O32>(ctx, n, variant, width, col, val, X + po, Y + po, ep);
+      if (rc) return rc;
+    }
+    return 0;
+  }
   for (unsigned p = 0; p < e.npan; ++p) {
     const size_t po = size_t(p) * e.pstride;
     Epi ep = e;

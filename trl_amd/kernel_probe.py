@@ -37,7 +37,7 @@ def main(argv=None) -> int:
     ap.add_argument("--b", type=int, default=8)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10, help="untimed launches before each kernel's timed ones")
-    ap.add_argument("--panels", type=int, default=6, help="panels per launch of panels32/panels64 (the refinement's blocks)")
+    ap.add_argument("--panels", type=int, default=5, help="panels per launch of panels32/panels64 (the refinement's blocks)")
     ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,panels32,panels64,forced32,gram32,update32,cholqr32,cholqr64,copy")
     a = ap.parse_args(argv)
     what = a.what.split(",")
