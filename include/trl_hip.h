@@ -42,6 +42,7 @@ extern "C" {
 #define TRL_HIP_EEIG (-4)   /* dense eigensolver did not converge (evp_base.hh:96)   */
 #define TRL_HIP_ERANGE (-5) /* block index out of range (multivector.hh:87)           */
 #define TRL_HIP_EBARRIER (-6) /* a batched step's device barrier timed out             */
+#define TRL_HIP_EIO (-7)      /* a file cannot be opened or parsed                      */
 
 typedef enum { TRL_HIP_F32 = 0, TRL_HIP_F64 = 1 } trl_hip_dtype;
 

@@ -165,6 +165,23 @@ int trl_hip_dense_syev(int n, const double* A, double* w, double* Z);
  * needs. */
 int trl_hip_dense_syev_select(int n, const double* A, double* w, int k, const int* sel, double* Z);
 
+/* ---- Matrix Market coordinate files (host) -------------------------------
+ * Replaces load_matrix_market of examples/sycl/csrevp.hh:26-124: '%' comment
+ * lines, a "symmetric" banner mirrors the off-diagonal entries, 1-based
+ * indices.  The CSR is canonical (columns ascending within a row, duplicate
+ * entries summed); real, integer and pattern (value 1) fields, general or
+ * symmetric.  trl_hip_mm_read returns TRL_HIP_EIO when the file cannot be
+ * opened ("Error: cannot open file <path>", the reference's message) or is
+ * malformed, unsupported or truncated; trl_hip_mm_last_error gives the
+ * calling thread's message.  trl_hip_mm_copy fills caller-owned host arrays
+ * of rows+1, nnz, nnz entries. */
+typedef struct trl_hip_mm trl_hip_mm;
+int trl_hip_mm_read(const char* path, trl_hip_mm** out);
+int trl_hip_mm_info(const trl_hip_mm* m, size_t* rows, size_t* cols, size_t* nnz, int* symmetric);
+int trl_hip_mm_copy(const trl_hip_mm* m, int32_t* rowptr, int32_t* col, double* val);
+void trl_hip_mm_free(trl_hip_mm* m);
+const char* trl_hip_mm_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -323,6 +323,11 @@ SOLVER_API = [
     ("trl_hip_dense_syev_select", c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_batch_solve_csr", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(BatchParams), c_void_p, c_void_p,
                                         c_void_p, c_void_p, C.POINTER(BatchStats)]),
+    ("trl_hip_mm_read", c_int, [C.c_char_p, C.POINTER(c_void_p)]),
+    ("trl_hip_mm_info", c_int, [c_void_p, C.POINTER(c_size), C.POINTER(c_size), C.POINTER(c_size), c_int_p]),
+    ("trl_hip_mm_copy", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_mm_free", None, [c_void_p]),
+    ("trl_hip_mm_last_error", C.c_char_p, []),
 ]
 
 _solver = None

@@ -118,6 +118,8 @@ const char* trl_hip_strerror(int code)
     return "block index out of range";
   case TRL_HIP_EBARRIER:
     return "device barrier of a batched step timed out";
+  case TRL_HIP_EIO:
+    return "cannot open or parse the file";
   default:
     if (code > 0) return hipGetErrorString(static_cast<hipError_t>(code));
     return "unknown trl_hip error";

@@ -116,26 +116,6 @@ __device__ inline T* elem(T* base, Idx<O32> i)
 // + F diag(g) (Z, F may be null).  X, Y, Z, F point at the lane's panel;
 // ro = row * B + off, the first element of this lane in it; g (with F) at
 // the coefficient of the lane's first column.
-// The AXPBY epilogue with the lane's own X row already in registers.
-template <typename S, int BC, bool O32>
-__device__ inline void epilogue_x(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S (&xr)[BC], S* Y, const S* Z, const S* F, const double* g,
-                                  const Epi& e)
-{
-  const S* ds = static_cast<const S*>(e.dscale);
-  S zr[BC], fr[BC], out[BC];
-  if (Z) RowIO<S, BC>::template load<true>(elem<O32>(Z, ro), zr);
-  if (F) RowIO<S, BC>::template load<true>(elem<O32>(F, ro), fr);
-  const double a = ds ? e.alpha * double(ld<true>(elem<O32>(ds, row))) : e.alpha;
-#pragma unroll
-  for (int q = 0; q < BC; ++q) {
-    double v = a * double(acc[q]) + e.beta * double(xr[q]);
-    if (Z) v += e.gamma * double(zr[q]);
-    if (F) v += g[q] * double(fr[q]);
-    out[q] = S(v);
-  }
-  RowIO<S, BC>::template store<true>(elem<O32>(Y, ro), out);
-}
-
 template <typename S, int BC, bool AXPBY, bool O32>
 __device__ inline void epilogue(const S (&acc)[BC], Idx<O32> row, Idx<O32> ro, const S* __restrict__ X, S* Y, const S* Z, const S* F, const double* g,
                                 const Epi& e)
@@ -279,9 +259,20 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
     if (c == row && self < 0) self = q;
     cofs[q] = I(c) * I(B) + I(off);
   }
+  // the product's factor for this row (alpha, times the row scale when
+  // given), the same for every panel
+  const S* ds = static_cast<const S*>(e.dscale);
+  const double arow = (AXPBY && ds) ? e.alpha * double(ld<true>(elem<O32>(ds, I(row)))) : e.alpha;
   for (unsigned p = 0; p < e.npan; ++p) {
     const size_t po = size_t(p) * e.pstride;
     const S* Xp = X + po;
+    // the step's own streams (Z, F) are loaded first, so that they are in
+    // flight with the gathers: one memory round trip per panel, not two
+    S zr[BC] = {}, fr[BC] = {};
+    if constexpr (AXPBY) {
+      if (Z) RowIO<S, BC>::template load<true>(elem<O32>(Z + po, ro), zr);
+      if constexpr (FORCE) RowIO<S, BC>::template load<true>(elem<O32>(F + po, ro), fr);
+    }
     S x[kChunk][BC];
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
@@ -300,9 +291,6 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
 #pragma unroll
       for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
     if constexpr (AXPBY) {
-      // the forced step loads F after the products (kept from moving up
-      // among the gathers, where its registers would cost a wave per SIMD)
-      if constexpr (FORCE) __builtin_amdgcn_sched_barrier(0);
       // beta X needs the lane's own X row: one of the gathered rows, unless
       // the row has neither a diagonal entry nor padding
       S xr[BC];
@@ -317,7 +305,15 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
       }
       else
         RowIO<S, BC>::load(elem<O32>(Xp, ro), xr);
-      epilogue_x<S, BC, O32>(acc, I(row), ro, xr, Y + po, Z ? Z + po : nullptr, F ? F + po : nullptr, F ? gsh + p * B + off : nullptr, e);
+      S out[BC];
+#pragma unroll
+      for (int c = 0; c < BC; ++c) {
+        double v = arow * double(acc[c]) + e.beta * double(xr[c]);
+        if (Z) v += e.gamma * double(zr[c]);
+        if constexpr (FORCE) v += gsh[p * B + off + c] * double(fr[c]);
+        out[c] = S(v);
+      }
+      RowIO<S, BC>::template store<true>(elem<O32>(Y + po, ro), out);
     }
     else
       epilogue<S, BC, AXPBY, O32>(acc, I(row), ro, Xp, Y + po, Z ? Z + po : nullptr, nullptr, nullptr, e);
