@@ -157,12 +157,16 @@ def DeviceOperatorName(d) -> str:
     return DeviceOperator._name(d)
 
 
-def traffic_of(kernel: str):
+def traffic_of(kernel: str, panels: int):
     """Measured HBM bytes per launch of a kernel from the committed counter
-    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated on a copy kernel)."""
+    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated on a copy kernel);
+    None when it is missing or, for the several-panel kernel, was measured
+    on another panel count."""
     f = ROOT / "profiles" / "spmm_laplace3d_160_traffic.json"
     try:
         t = json.loads(f.read_text())
+        if "ell_panels" in kernel and t.get("panels_per_launch") != panels:
+            return None, None
         k = t["kernels"][kernel]
         return k["fetch_bytes"] + k["write_bytes"], f"profiles/{f.name}"
     except Exception:
@@ -317,7 +321,7 @@ def main(argv=None) -> int:
         # the operator kernel of the precision that spent more time in it
         kdt = max(("f32", "f64"), key=lambda t: fam_t[t]["spmm"]["seconds"])
         k = dominant_kernel_rate(ctx, dA, cfg.b, kdt, a.refine_blocks or cfg.nev // cfg.b + 7)
-        traffic, tsrc = traffic_of(k["kernel"])
+        traffic, tsrc = traffic_of(k["kernel"], k["panels_per_launch"])
         roof = {"kernel": k["kernel"], "bound": "hbm", "achieved": k["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": k["achieved"] / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": tsrc, "launches": k["launches"],
                 "avg_launch_us": k["avg_launch_us"], "algorithmic_bytes_per_launch": k["algorithmic_bytes_per_launch"],
