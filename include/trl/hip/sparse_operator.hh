@@ -45,11 +45,7 @@ public:
       int32_t nnz = 0;
       throw_on(trl_hip_memcpy_d2h(ctx_.get(), &nnz, csr_.rowptr + n_, sizeof(int32_t)), "nnz");
       if (width > 0 && double(width) * double(n_) <= (1.0 + max_padding) * double(nnz)) {
-        if (!ell_ || ell_->width != width) {
-          ell_ = std::make_shared<Ell>(ctx_, n_, width);
-          throw_on(trl_hip_csr_to_ell(ctx_.get(), dtype, n_, csr_.rowptr, csr_.col, csr_.val, width, ell_->col.data(), ell_->val.data()), "csr_to_ell");
-        }
-        ell_desc_ = trl_hip_spmat{1, nullptr, width, ell_->col.data(), ell_->val.data(), csr_.dscale, 0, 0, 0};
+        make_ell(width);
         ell = &ell_desc_;
       }
     }
@@ -66,6 +62,27 @@ public:
     tuned_us_ = us;
     panels_n_ = 0;
     return us;
+  }
+
+  // Takes another operator's choice of storage and kernel (format, ELL
+  // width, variant, row order; e.g. the FP32 copy's, tuned) without timing
+  // anything: for an operator that runs too few products to repay a tune.
+  void adopt(int b, const trl_hip_spmat& like)
+  {
+    if (n_ == 0) return;
+    if (like.format == 1) {
+      make_ell(like.width);
+      op_ = ell_desc_;
+    }
+    else
+      op_ = csr_;
+    op_.variant = like.variant;
+    op_.plane = like.plane;
+    op_.plane_block = like.plane_block;
+    op_axpby_ = op_;
+    tuned_b_ = b;
+    tuned_us_ = axpby_us_ = 0.0;
+    panels_n_ = 0;
   }
 
   // After tune(b): times the axpby form on npanels panels at once (the
@@ -131,6 +148,16 @@ private:
     DeviceBuffer<S> val;
     Ell(const Context& c, std::size_t n, int w) : width(w), col(c, std::size_t(w) * n), val(c, std::size_t(w) * n) {}
   };
+
+  // the slot-major ELL copy of the CSR arrays with `width` slots
+  void make_ell(int width)
+  {
+    if (!ell_ || ell_->width != width) {
+      ell_ = std::make_shared<Ell>(ctx_, n_, width);
+      throw_on(trl_hip_csr_to_ell(ctx_.get(), dtype, n_, csr_.rowptr, csr_.col, csr_.val, width, ell_->col.data(), ell_->val.data()), "csr_to_ell");
+    }
+    ell_desc_ = trl_hip_spmat{1, nullptr, width, ell_->col.data(), ell_->val.data(), csr_.dscale, 0, 0, 0};
+  }
 
   Context ctx_;
   std::size_t n_;

@@ -65,11 +65,11 @@ void true_residuals(const Context& c, std::size_t n, const SparseOperator<S>& op
 // is asked for (the fused axpby form too unless axpby is false).
 template <class S, unsigned B>
 std::shared_ptr<SparseOperator<S>> make_operator(const Context& c, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* minv,
-                                                 const trl_hip_solver_params& p, bool axpby = true)
+                                                 const trl_hip_solver_params& p, bool axpby = true, bool tune = true)
 {
   auto op = std::make_shared<SparseOperator<S>>(c, n, rowptr, col, val, minv, p.spmm_tune ? 0 : p.lanes_per_row);
   const auto t0 = std::chrono::steady_clock::now();
-  if (p.spmm_tune) op->tune(int(B), true, 0.25, axpby);
+  if (p.spmm_tune && tune) op->tune(int(B), true, 0.25, axpby);
   const double tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (std::getenv("TRL_HIP_VERBOSE")) {
     const auto& d = op->descriptor();
@@ -611,10 +611,12 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       const unsigned nblk = (p->nev + B - 1) / B;
       c.sync();
       const double t0 = now();
-      // the FP64 operator runs the fused filter step only without the
-      // residual-driven FP32 filter
+      // With the residual-driven FP32 filter the FP64 operator only forms
+      // the sweeps' Rayleigh-Ritz and residual products (a few dozen):
+      // instead of a tune of its own it takes the tuned FP32 operator's
+      // storage and kernel (below).  Without it, it also runs the filter.
       const bool rfil = mp->refine == 1 && mp->residual_filter && !mp->phase1_fp64;
-      const auto op64 = make_operator<double, B>(c, n, rowptr, col, val64, minv64, *p, !rfil);
+      const auto op64 = make_operator<double, B>(c, n, rowptr, col, val64, minv64, *p, !rfil, !rfil);
       // ---- FP32 phase -------------------------------------------------------
       trl::hip::DeviceBuffer<float> V0f(c, n * B);
       throw_on(trl_hip_convert(c.get(), TRL_HIP_F64, V0, TRL_HIP_F32, V0f.data(), n * B), "convert V0");
@@ -637,6 +639,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       std::shared_ptr<SparseOperator<float>> op32;
       if (!mp->phase1_fp64) {
         op32 = make_operator<float, B>(c, n, rowptr, col, val32, minv32, *p);
+        if (rfil && p->spmm_tune) op64->adopt(int(B), op32->descriptor());
         solve_typed<float, B>(c, n, rowptr, col, val32, mass32, minv32, op32, p32, V0f.data(), ev32.data(), X32.data(), nref, nullptr, r32, &all32, &got);
       }
       nref = got;
