@@ -229,6 +229,7 @@ def main(argv=None) -> int:
                     help="per-block filter degrees from measured convergence rates: 0 off, 1 all sweeps, 2 FP64 sweeps only")
     ap.add_argument("--residual-filter", type=int, default=1, choices=[0, 1],
                     help="1: the FP64 sweeps' filter runs in FP32, driven by the FP64 residuals of the Ritz pairs")
+    ap.add_argument("--ncv-fp32", type=int, default=192, help="Krylov space of the FP32 Lanczos phase (the FP64 solves keep ncv)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -264,7 +265,7 @@ def main(argv=None) -> int:
     # ---- warmup + profiled pass -------------------------------------------------
     mixed_kw = dict(tol_fp32=a.tol_fp32, cheb_degree=a.cheb_degree, refine_blocks=a.refine_blocks, max_refine=a.max_refine, spmm_tune=tune,
                     reorth=a.reorth, fp32_sweeps=a.fp32_sweeps, tol_fp32_refine=a.tol_fp32_refine, degree_opt=a.degree_opt,
-                    residual_filter=bool(a.residual_filter))
+                    residual_filter=bool(a.residual_filter), ncv_fp32=a.ncv_fp32)
     for w in range(a.warmup):
         solve_mixed(dA, cfg.b, params, V0, **mixed_kw)
         log(f"warmup {w} done")
@@ -374,7 +375,7 @@ def main(argv=None) -> int:
         "config": {"workload": f"{cfg.name}: 3D 7-point Laplacian {a.side}^3 + diag U[0,1]", "n": cfg.csr.n, "nnz": cfg.csr.nnz, "b": cfg.b, "nev": cfg.nev,
                    "ncv": cfg.ncv, "tol": cfg.tol, "order": "smallest", "tol_fp32": a.tol_fp32, "cheb_degree": a.cheb_degree,
                    "refine_blocks": a.refine_blocks or cfg.nev // cfg.b + 7, "max_refine": a.max_refine, "fp32_sweeps": a.fp32_sweeps,
-                   "tol_fp32_refine": a.tol_fp32_refine, "degree_opt": a.degree_opt, "residual_filter": a.residual_filter, "spmm_tune": tune, "reorth": a.reorth},
+                   "tol_fp32_refine": a.tol_fp32_refine, "degree_opt": a.degree_opt, "residual_filter": a.residual_filter, "ncv_fp32": a.ncv_fp32, "spmm_tune": tune, "reorth": a.reorth},
         "converged": bool(last.result["converged"]),
         "max_true_residual": float(last.resid.max()),
         "fp32_phase": last.result32,

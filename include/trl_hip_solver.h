@@ -83,6 +83,8 @@ typedef struct {
                                 filter in FP32, on E = p(A) x - p(theta) x driven
                                 by the FP64 residual A x - theta x of each Ritz
                                 pair (x <- p(theta) x + E in FP64); 0: FP64 filter */
+  unsigned ncv_fp32;         /* phase 1's Krylov space (a multiple of b; 0: ncv); the
+                                FP64 Lanczos fallback keeps ncv                   */
 } trl_hip_mixed_params;
 
 /* Mixed precision.  Phase 1: an FP32 thick-restart solve (FP32 basis, SpMM

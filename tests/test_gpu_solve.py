@@ -227,16 +227,16 @@ def test_mixed_filter_all_blocks_per_launch(ctx):
 @pytest.mark.parametrize("residual_filter", [False, True])
 def test_mixed_bench_schedule(ctx, residual_filter):
     # the benchmark's schedule on a 48^3 stand-in: one loose FP32 Lanczos
-    # cycle, FP32 sweeps on 5 blocks, FP64 sweeps, with per-block degrees
-    # (the FP64 sweeps' filter in FP64, or in FP32 driven by the FP64
-    # residuals)
+    # cycle over 192 columns, FP32 sweeps on 5 blocks, FP64 sweeps, with
+    # per-block degrees (the FP64 sweeps' filter in FP64, or in FP32 driven
+    # by the FP64 residuals)
     c = P.config_c3(48)
     dA = DeviceCSR(ctx, c.csr)
     V0 = v0(ctx, c.csr.n, c.b)
     prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
     o64 = solve(dA, c.b, prm, V0, reorth="dgks", spmm_tune=True)
     om = solve_mixed(dA, c.b, prm, V0, tol_fp32=0.4, cheb_degree=140, refine_blocks=5, max_refine=14, fp32_sweeps=12, tol_fp32_refine=1e-4,
-                     reorth="dgks", spmm_tune=True, degree_opt=1, residual_filter=residual_filter)
+                     reorth="dgks", spmm_tune=True, degree_opt=1, residual_filter=residual_filter, ncv_fp32=192)
     print("bench schedule", residual_filter, om.result, om.result32)
     assert om.result["converged"] == 1
     assert om.resid.max() <= 1e-8

@@ -296,7 +296,7 @@ class SolverResult(C.Structure):
 class MixedParams(C.Structure):
     _fields_ = [("tol_fp32", C.c_double), ("max_restarts_fp32", C.c_uint), ("seed", C.c_ulonglong), ("refine", c_int), ("cheb_degree", C.c_uint),
                 ("max_refine", C.c_uint), ("phase1_fp64", c_int), ("refine_blocks", C.c_uint), ("fp32_sweeps", C.c_uint), ("tol_fp32_refine", C.c_double),
-                ("degree_opt", c_int), ("residual_filter", c_int)]
+                ("degree_opt", c_int), ("residual_filter", c_int), ("ncv_fp32", C.c_uint)]
 
 
 class BatchParams(C.Structure):

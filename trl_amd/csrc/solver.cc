@@ -620,13 +620,16 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       // ---- FP32 phase -------------------------------------------------------
       trl::hip::DeviceBuffer<float> V0f(c, n * B);
       throw_on(trl_hip_convert(c.get(), TRL_HIP_F64, V0, TRL_HIP_F32, V0f.data(), n * B), "convert V0");
-      const unsigned m = p->ncv / B;
+      // phase 1's Krylov space: ncv_fp32 columns (0: the solve's ncv)
+      const unsigned ncv1 = mp->ncv_fp32 ? mp->ncv_fp32 : p->ncv;
+      const unsigned m = ncv1 / B;
       unsigned nref = mp->refine == 1 ? (mp->refine_blocks ? mp->refine_blocks : kr + 6) : kr;
       if (nref < kr) nref = kr;
       if (nref > m) nref = m;
       trl::hip::DeviceBuffer<float> X32(c, std::size_t(nref) * n * B);
       std::vector<double> ev32(std::max(p->nev, nref * B), 0.0), all32;
       trl_hip_solver_params p32 = *p;
+      p32.ncv = ncv1;
       p32.tol = mp->tol_fp32;
       p32.max_restarts = mp->max_restarts_fp32;
       if (p32.cholqr_passes == 0) p32.cholqr_passes = 2;

@@ -166,7 +166,7 @@ def solve(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *,
 
 def solve_mixed(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *, tol_fp32=1e-5, max_restarts_fp32=1000, seed=7, order="ascending",
                 reorth="cgs2", cholqr_passes=0, lanes_per_row=0, spmm_tune=False, refine=1, cheb_degree=80, max_refine=8, refine_blocks=0,
-                phase1_fp64=False, fp32_sweeps=0, tol_fp32_refine=1e-4, degree_opt=False, residual_filter=False) -> SolveOutput:
+                phase1_fp64=False, fp32_sweeps=0, tol_fp32_refine=1e-4, degree_opt=False, residual_filter=False, ncv_fp32=0) -> SolveOutput:
     ctx = A.ctx
     assert V0.dtype == np.float64
     nblk = (params.nev + b - 1) // b
@@ -177,7 +177,7 @@ def solve_mixed(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArr
     res32 = L.SolverResult()
     p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row, spmm_tune)
     mp = L.MixedParams(tol_fp32, max_restarts_fp32, seed, refine, cheb_degree, max_refine, 1 if phase1_fp64 else 0, refine_blocks, fp32_sweeps,
-                       tol_fp32_refine, int(degree_opt), 1 if residual_filter else 0)
+                       tol_fp32_refine, int(degree_opt), 1 if residual_filter else 0, ncv_fp32)
     v64, m64, mi64 = A.ptrs("f64")
     v32, m32, mi32 = A.ptrs("f32")
     rc = L.solver().trl_hip_solve_csr_mixed(ctx.h, A.n, A.rowptr.ptr, A.col.ptr, v64, v32, m64, mi64, m32, mi32, C.byref(p), C.byref(mp), V0.ptr,
