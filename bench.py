@@ -220,7 +220,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--side", type=int, default=160, help="grid side of the 3D Laplacian (160 = the named workload)")
     ap.add_argument("--tol-fp32", type=float, default=0.4, help="tolerance of the FP32 Lanczos phase (0.4: about one cycle, a starting subspace)")
-    ap.add_argument("--cheb-degree", type=int, default=140)
+    ap.add_argument("--cheb-degree", type=int, default=160)
     ap.add_argument("--refine-blocks", type=int, default=5, help="Ritz blocks refined together (0: nev/b + 7)")
     ap.add_argument("--max-refine", type=int, default=14, help="FP64 refinement sweeps before the FP64 Lanczos fallback")
     ap.add_argument("--fp32-sweeps", type=int, default=12, help="FP32 refinement sweeps (at most) before the FP64 ones")

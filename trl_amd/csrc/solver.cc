@@ -459,10 +459,15 @@ inline std::vector<unsigned> panel_degrees(unsigned nb, unsigned B, unsigned nev
 // panel from the second filter on.  Returns true on convergence; theta and
 // rs hold the last values, applies counts the filter's operator applications.
 // rfilt (FP64 sweeps only, may be null): filter in FP32 through it.
+// ramp: the first `ramp` filters run at degree / 2^ramp, ..., degree / 2
+// (at most): the cutoff of the first sweeps, the last kept Ritz value of a
+// rough start, sits far above the eigenvalue it stands for, so a full-degree
+// filter would spend its steps amplifying unwanted eigenvalues below it.
 template <class S, unsigned B>
 bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, const SparseOperator<S>& op, const S* mass, unsigned nev, double tol,
                    unsigned max_sweeps, unsigned degree, bool degree_opt, double lo_all, double hi_all, std::vector<double>& theta,
-                   std::vector<double>& rs, unsigned& sweeps, std::size_t& applies, const char* tag, ResidualFilter<B>* rfilt = nullptr)
+                   std::vector<double>& rs, unsigned& sweeps, std::size_t& applies, const char* tag, ResidualFilter<B>* rfilt = nullptr,
+                   unsigned ramp = 0)
 {
   const double margin = 0.01 * (hi_all - lo_all);
   rs.assign(nev, 0.0);
@@ -486,6 +491,11 @@ bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, cons
     if (sweeps >= max_sweeps) return false;
     ++sweeps;
     std::vector<unsigned> deg = degree_opt ? panel_degrees(rf.nb, B, nev, tol, degree, rs, rs_prev, deg_prev) : std::vector<unsigned>(rf.nb, degree);
+    if (sweeps <= ramp) {
+      // even, at least 2: the all-blocks launch steps them in pairs
+      const unsigned cap = std::max(2u, ((degree >> (ramp + 1 - sweeps)) + 1) & ~1u);
+      for (unsigned& d : deg) d = std::min(d, cap);
+    }
     if (std::getenv("TRL_HIP_VERBOSE")) {
       std::fprintf(stderr, "[refine %s] degrees", tag);
       for (unsigned d : deg) std::fprintf(stderr, " %u", d);
@@ -664,7 +674,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         unsigned sw32 = 0;
         std::size_t ap32 = 0;
         refine_sweeps<float, B>(rf32, c, n, *op32, mass32, p->nev, mp->tol_fp32_refine, mp->fp32_sweeps, mp->cheb_degree, mp->degree_opt == 1, lo_all,
-                                hi_all, th32, rs32, sw32, ap32, "fp32");
+                                hi_all, th32, rs32, sw32, ap32, "fp32", nullptr, 2);
         throw_on(trl_hip_memcpy_d2d(c.get(), X32.data(), rf32.X, std::size_t(nref) * n * B * sizeof(float)), "copy X32");
         r32.n_op_apply += (sw32 + 1) * nref * 2 + ap32;
         r32.iterations += sw32;
@@ -684,8 +694,11 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         std::vector<double> theta, rs;
         unsigned sweep = 0;
         std::size_t ap64 = 0;
+        // the degree ramp belongs to the first sweeps after phase 1: here
+        // when no FP32 sweeps ran (the all-FP64 schedule among them)
+        const unsigned ramp64 = (op32 && mp->fp32_sweeps > 0) ? 0 : 2;
         done = refine_sweeps<double, B>(rf, c, n, *op64, mass64, p->nev, p->tol, mp->max_refine, mp->cheb_degree, mp->degree_opt != 0, lo_all, hi_all, theta,
-                                        rs, sweep, ap64, "fp64", rfilt.get());
+                                        rs, sweep, ap64, "fp64", rfilt.get(), ramp64);
         if (done) {
           for (unsigned i = 0; i < p->nev; ++i) evals[i] = theta[i];
           if (X) throw_on(trl_hip_memcpy_d2d(c.get(), X, rf.X, std::size_t(nblk) * n * B * sizeof(double)), "copy X");
