@@ -224,7 +224,7 @@ def main(argv=None) -> int:
     ap.add_argument("--refine-blocks", type=int, default=5, help="Ritz blocks refined together (0: nev/b + 7)")
     ap.add_argument("--max-refine", type=int, default=14, help="FP64 refinement sweeps before the FP64 Lanczos fallback")
     ap.add_argument("--fp32-sweeps", type=int, default=12, help="FP32 refinement sweeps (at most) before the FP64 ones")
-    ap.add_argument("--tol-fp32-refine", type=float, default=1e-4, help="FP32 residual at which the FP32 sweeps stop")
+    ap.add_argument("--tol-fp32-refine", type=float, default=1e-3, help="FP32 residual at which the FP32 sweeps stop")
     ap.add_argument("--degree-opt", type=int, default=1, choices=[0, 1, 2],
                     help="per-block filter degrees from measured convergence rates: 0 off, 1 all sweeps, 2 FP64 sweeps only")
     ap.add_argument("--residual-filter", type=int, default=1, choices=[0, 1],

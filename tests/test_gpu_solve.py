@@ -235,7 +235,7 @@ def test_mixed_bench_schedule(ctx, residual_filter):
     V0 = v0(ctx, c.csr.n, c.b)
     prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
     o64 = solve(dA, c.b, prm, V0, reorth="dgks", spmm_tune=True)
-    om = solve_mixed(dA, c.b, prm, V0, tol_fp32=0.4, cheb_degree=160, refine_blocks=5, max_refine=14, fp32_sweeps=12, tol_fp32_refine=1e-4,
+    om = solve_mixed(dA, c.b, prm, V0, tol_fp32=0.4, cheb_degree=160, refine_blocks=5, max_refine=14, fp32_sweeps=12, tol_fp32_refine=1e-3,
                      reorth="dgks", spmm_tune=True, degree_opt=1, residual_filter=residual_filter, ncv_fp32=192)
     print("bench schedule", residual_filter, om.result, om.result32)
     assert om.result["converged"] == 1
