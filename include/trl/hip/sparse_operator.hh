@@ -67,6 +67,8 @@ public:
   // Takes another operator's choice of storage and kernel (format, ELL
   // width, variant, row order; e.g. the FP32 copy's, tuned) without timing
   // anything: for an operator that runs too few products to repay a tune.
+  // The axpby form takes the same descriptor, untimed (axpby_us() is 0, so
+  // panels_pay() stays false until a tune).
   void adopt(int b, const trl_hip_spmat& like)
   {
     if (n_ == 0) return;
