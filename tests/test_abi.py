@@ -122,3 +122,31 @@ def test_dense_syev_select_arguments():
     bad = np.array([2, 1], np.int32)  # not increasing
     assert L.solver().trl_hip_dense_syev_select(4, A.ctypes.data, w.ctypes.data, 2, bad.ctypes.data, Z.ctypes.data) == -1
     assert L.solver().trl_hip_dense_syev_select(4, A.ctypes.data, w.ctypes.data, 5, bad.ctypes.data, Z.ctypes.data) == -1
+
+
+def test_dense_results_do_not_depend_on_the_thread_count(tmp_path):
+    # the threaded Householder reduction (n >= 128) sums each entry on one
+    # thread in a fixed order: one and several OpenMP threads agree bitwise
+    import os
+    import subprocess
+    import sys
+
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((160, 160))
+    np.save(tmp_path / "A.npy", (A + A.T) / 2)
+    code = (
+        "import sys, numpy as np\n"
+        "from trl_amd.solver import dense_syev, dense_syev_select\n"
+        "A = np.load(sys.argv[1])\n"
+        "w, Z = dense_syev(A)\n"
+        "ws, Zs = dense_syev_select(A, np.arange(12))\n"
+        "np.savez(sys.argv[2], w=w, Z=Z, ws=ws, Zs=Zs)\n"
+    )
+    out = {}
+    for t in (1, 5):
+        env = dict(os.environ, OMP_NUM_THREADS=str(t))
+        f = tmp_path / f"r{t}.npz"
+        subprocess.run([sys.executable, "-c", code, str(tmp_path / "A.npy"), str(f)], env=env, cwd=ROOT, check=True, timeout=120)
+        out[t] = np.load(f)
+    for k in ("w", "Z", "ws", "Zs"):
+        assert np.array_equal(out[1][k], out[5][k]), k

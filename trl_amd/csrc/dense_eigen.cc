@@ -30,13 +30,26 @@ namespace trl_amd::dense {
 
 namespace {
 
+// Trailing blocks at least this many rows wide are reduced by all OpenMP
+// threads; smaller ones serially (a parallel region costs a few microseconds,
+// a serial step over 128 rows about as much).  Each entry is summed by one
+// thread in a fixed order, so the result does not depend on the thread count.
+constexpr int kParallelRows = 128;
+
 // Householder tridiagonalisation of the row-major symmetric A (Golub & Van
 // Loan Alg. 8.3.1): Q^T A Q = tridiag(d, e) with Q = H_0 H_1 ... H_{n-3},
 // H_k = I - betas[k] v_k v_k^T, v_k stored in row k of vs (entries k+1..).
 void tridiagonalise(int n, const double* A, std::vector<double>& vs, std::vector<double>& betas, std::vector<double>& d, std::vector<double>& e)
 {
+  const int wide = kParallelRows;
   const std::size_t N = std::size_t(n);
-  std::vector<double> a(A, A + N * N); // working copy, row-major; only its lower triangle is used
+  // working copy, row-major.  While the trailing block is at least
+  // `wide` rows wide both triangles are kept (rows contiguous for the
+  // threads); after that only the lower triangle is used.
+  std::vector<double> a(A, A + N * N);
+  if (n >= wide)
+    for (std::size_t i = 0; i < N; ++i)
+      for (std::size_t j = i + 1; j < N; ++j) a[i * N + j] = a[j * N + i];
   vs.assign(N * N, 0.0);
   betas.assign(N, 0.0);
   std::vector<double> p(N), wv(N);
@@ -57,29 +70,52 @@ void tridiagonalise(int n, const double* A, std::vector<double>& vs, std::vector
     if (vn2 == 0.0) continue;
     const double beta = 2.0 / vn2;
     betas[k] = beta;
-    // p = beta * A22 v from the lower triangle of A22 (row i, j <= i holds
-    // A(i, j) = A(j, i)): one pass over it for both halves of the product
-    for (int i = k + 1; i < n; ++i) p[i] = 0.0;
-    for (int i = k + 1; i < n; ++i) {
-      const double* ai = a.data() + std::size_t(i) * N;
-      const double vi = v[i];
-      double s = ai[i] * vi;
-      for (int j = k + 1; j < i; ++j) {
-        s += ai[j] * v[j];
-        p[j] += ai[j] * vi;
+    // p = beta * A22 v.  Wide trailing blocks: full rows, one thread per
+    // block of rows.  Narrow ones: from the lower triangle (row i, j <= i
+    // holds A(i, j) = A(j, i)), one pass over it for both halves.
+    const bool par = n - k >= wide;
+    if (par) {
+#pragma omp parallel for schedule(static)
+      for (int i = k + 1; i < n; ++i) {
+        const double* ai = a.data() + std::size_t(i) * N;
+        double s = 0.0;
+        for (int j = k + 1; j < n; ++j) s += ai[j] * v[j];
+        p[i] = beta * s;
       }
-      p[i] += s;
     }
-    for (int i = k + 1; i < n; ++i) p[i] *= beta;
+    else {
+      for (int i = k + 1; i < n; ++i) p[i] = 0.0;
+      for (int i = k + 1; i < n; ++i) {
+        const double* ai = a.data() + std::size_t(i) * N;
+        const double vi = v[i];
+        double s = ai[i] * vi;
+        for (int j = k + 1; j < i; ++j) {
+          s += ai[j] * v[j];
+          p[j] += ai[j] * vi;
+        }
+        p[i] += s;
+      }
+      for (int i = k + 1; i < n; ++i) p[i] *= beta;
+    }
     double pv = 0.0;
     for (int i = k + 1; i < n; ++i) pv += p[i] * v[i];
     const double K = 0.5 * beta * pv;
     for (int i = k + 1; i < n; ++i) wv[i] = p[i] - K * v[i];
-    // rank-2 update of the lower triangle only
-    for (int i = k + 1; i < n; ++i) {
-      double* ai = a.data() + std::size_t(i) * N;
-      const double vi = v[i], wi = wv[i];
-      for (int j = k + 1; j <= i; ++j) ai[j] -= vi * wv[j] + wi * v[j];
+    // rank-2 update: full rows while wide, else the lower triangle only
+    if (par) {
+#pragma omp parallel for schedule(static)
+      for (int i = k + 1; i < n; ++i) {
+        double* ai = a.data() + std::size_t(i) * N;
+        const double vi = v[i], wi = wv[i];
+        for (int j = k + 1; j < n; ++j) ai[j] -= vi * wv[j] + wi * v[j];
+      }
+    }
+    else {
+      for (int i = k + 1; i < n; ++i) {
+        double* ai = a.data() + std::size_t(i) * N;
+        const double vi = v[i], wi = wv[i];
+        for (int j = k + 1; j <= i; ++j) ai[j] -= vi * wv[j] + wi * v[j];
+      }
     }
     a[std::size_t(k + 1) * N + k] = alpha;
   }
@@ -331,7 +367,8 @@ int syev_select(int n, const double* A, double* w, int k, const int* sel, double
       for (int i = 0; i < n; ++i) x[i] /= nrm;
     }
   }
-  // ---- back-transformation z = H_0 H_1 ... H_{n-3} y -------------------------
+  // ---- back-transformation z = H_0 H_1 ... H_{n-3} y, one vector a thread ---
+#pragma omp parallel for schedule(static) if (n >= kParallelRows)
   for (int j = 0; j < k; ++j) {
     double* x = y.data() + std::size_t(j) * N;
     for (int kk = n - 3; kk >= 0; --kk) {
