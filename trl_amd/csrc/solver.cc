@@ -206,16 +206,23 @@ struct BlockRefiner {
   {
     const unsigned nt = nb * B;
     for (unsigned j = 0; j < nb; ++j) op.apply(int(B), panel(X, j), panel(AX.data(), j));
+    // H = X^T A X is symmetric: only its blocks (i, j) with i <= j are
+    // formed, block column j from X_0..X_j and A X_j ((j + 2) panels read
+    // instead of nb + 1)
     double* C = coef.data();
     for (unsigned j = 0; j < nb; ++j)
-      throw_on(trl_hip_gram(c.get(), t, n, B, int(nb), X, panel(AX.data(), j), mass, C + std::size_t(j) * nt * B), "refine projection");
+      throw_on(trl_hip_gram(c.get(), t, n, B, int(j + 1), X, panel(AX.data(), j), mass, C + std::size_t(j) * nt * B), "refine projection");
     std::vector<double> Cj(std::size_t(nt) * nt);
     throw_on(trl_hip_memcpy_d2h(c.get(), Cj.data(), C, Cj.size() * sizeof(double)), "refine download");
-    // C + j*nt*B holds rows r = 0..nt-1 (panel i = r/B, column r%B) of H[:, j*B:(j+1)*B]
+    // C + j*nt*B holds rows r = 0..(j+1)*B-1 (panel i = r/B, column r%B) of
+    // H[:, j*B:(j+1)*B]; the rows below are mirrored from the later columns
     std::vector<double> H(std::size_t(nt) * nt), w(nt), Z(std::size_t(nt) * nt);
     for (unsigned j = 0; j < nb; ++j)
-      for (unsigned r = 0; r < nt; ++r)
+      for (unsigned r = 0; r < (j + 1) * B; ++r)
         for (unsigned q = 0; q < B; ++q) H[std::size_t(r) * nt + j * B + q] = Cj[std::size_t(j) * nt * B + std::size_t(r) * B + q];
+    for (unsigned j = 0; j < nb; ++j)
+      for (unsigned r = (j + 1) * B; r < nt; ++r)
+        for (unsigned q = 0; q < B; ++q) H[std::size_t(r) * nt + j * B + q] = H[std::size_t(j * B + q) * nt + r];
     for (unsigned r = 0; r < nt; ++r)
       for (unsigned q = r + 1; q < nt; ++q) {
         const double a = 0.5 * (H[std::size_t(r) * nt + q] + H[std::size_t(q) * nt + r]);
