@@ -511,6 +511,8 @@ __global__ __launch_bounds__(kThreads) void k_update(const S* __restrict__ V, si
 // The needed part of Y (m*B x nb*B, padded to NT*16 columns) is staged in
 // LDS in the panel type; each wave computes 16 rows x NT*16 columns with K
 // running over the m*B basis columns.
+constexpr int kRotUnroll = 8;
+
 template <typename S, int B, int NT>
 __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
                                                 S* __restrict__ Out, size_t n)
@@ -542,18 +544,29 @@ __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t 
     acc_t acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = acc_t{0, 0, 0, 0};
-    for (int s = 0; s < ksteps; ++s) {
-      const int kk = 4 * s + q4;
-      const bool kok = kk < K;
-      S a = 0;
-      if (rok && kok) {
-        const int j = kk / B, c = kk % B;
-        a = V[size_t(j) * ldv + ra * B + c];
+    // the basis loads of kRotUnroll k-steps are issued together, ahead of
+    // their products: one memory round trip per chunk, not per k-step
+    for (int s0 = 0; s0 < ksteps; s0 += kRotUnroll) {
+      S a[kRotUnroll];
+#pragma unroll
+      for (int u = 0; u < kRotUnroll; ++u) {
+        const int kk = 4 * (s0 + u) + q4;
+        a[u] = S(0);
+        if (rok && kk < K) {
+          const int j = kk / B, c = kk % B;
+          a[u] = V[size_t(j) * ldv + ra * B + c];
+        }
       }
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const S bv = kok ? Ys[kk * ldy + t * 16 + i16] : S(0);
-        acc[t] = M::op(a, bv, acc[t]);
+      for (int u = 0; u < kRotUnroll; ++u) {
+        if (s0 + u >= ksteps) break; // uniform over the wave
+        const int kk = 4 * (s0 + u) + q4;
+        const bool kok = kk < K;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const S bv = kok ? Ys[kk * ldy + t * 16 + i16] : S(0);
+          acc[t] = M::op(a[u], bv, acc[t]);
+        }
       }
     }
 #pragma unroll
