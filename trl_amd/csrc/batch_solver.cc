@@ -16,6 +16,8 @@
 #include <thread>
 #include <vector>
 
+#include <omp.h>
+
 #include "trl_amd/rayleigh_ritz.hh"
 #include "trl_hip_solver.h"
 
@@ -52,6 +54,11 @@ void parallel_for(const std::vector<int>& items, unsigned nthreads, F&& f)
   std::vector<std::thread> pool;
   for (unsigned t = 0; t < nthreads; ++t)
     pool.emplace_back([&]() {
+      // the dense eigensolver threads its larger reductions with OpenMP;
+      // inside this pool each worker runs them serially (the same results:
+      // they do not depend on the thread count) instead of each starting a
+      // team of its own
+      omp_set_num_threads(1);
       for (size_t i; (i = next.fetch_add(1)) < items.size();) f(items[i]);
     });
   for (auto& th : pool) th.join();
