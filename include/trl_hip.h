@@ -66,6 +66,13 @@ int trl_hip_ctx_check(trl_hip_ctx* ctx, int* pivot);
 void* trl_hip_ctx_stream(trl_hip_ctx* ctx);
 /* Number of compute units of the context's device. */
 int trl_hip_ctx_num_cus(trl_hip_ctx* ctx);
+/* Context options (no reference counterpart; for tests of paths that the
+ * problem sizes at hand would not take):
+ *   TRL_HIP_OPT_OFFSETS64, value 1: the operator kernels use their 64-bit
+ *   element offsets, which they otherwise take only when a panel or an ELL
+ *   array exceeds 4 GiB; value 0 restores the size-based choice. */
+#define TRL_HIP_OPT_OFFSETS64 1
+int trl_hip_ctx_set_option(trl_hip_ctx* ctx, int option, int value);
 const char* trl_hip_strerror(int code);
 
 /* Zero-initialised device allocation (multivector.hh:44-48 zero-fills). */
@@ -341,6 +348,11 @@ int trl_hip_batch_save_blocks(trl_hip_batch* batch, int p, int nblk);
 int trl_hip_batch_residuals(trl_hip_batch* batch, int nblk, const double* lambda, double* out);
 /* Clears the breakdown pivots and the barrier error flag. */
 int trl_hip_batch_reset_flags(trl_hip_batch* batch);
+/* Sets the barrier error flag as a timed-out device barrier does (for tests
+ * of the error path): the next trl_hip_batch_extend then returns
+ * TRL_HIP_EBARRIER, its step kernels returning at once, until
+ * trl_hip_batch_reset_flags. */
+int trl_hip_batch_raise_error(trl_hip_batch* batch);
 
 /* ---- profiling (no reference counterpart) --------------------------------- */
 

@@ -7,7 +7,13 @@
 //    (include/trl/concepts.hh:26-188, eigensolvers/reorthogonalization.hh:29-32).
 // 2. main(): the acceptance checks of tests/cpp/hip_test_helper.hh run with
 //    DRIVER = trl::BlockLanczos (include/trl/eigensolvers/lanczos.hh), with
-//    the default trl::ModifiedGS and with trl::hip::BlockCGS2.
+//    the default trl::ModifiedGS and with trl::hip::BlockCGS2, and the 2D
+//    Laplacian 128^2 (8 smallest, ncv = 256) against its closed form and the
+//    CPU oracle.
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
 #include <trl/concepts.hh>
 #include <trl/eigensolvers/lanczos.hh>
 #include <trl/eigensolvers/params.hh>
@@ -50,6 +56,73 @@ bool check_cgs2(const trl::hip::Context& ctx)
   return ok;
 }
 
+// The oracle's restated driver (oracle/trl_oracle.cc, linked from
+// oracle/lib/liboracle.so): the same solve on the CPU from the same start block.
+extern "C" int oracle_lanczos_solve(size_t n, const int32_t* rowptr, const int32_t* col, const double* val, const double* dscale, const double* mdiag,
+                                    int b, unsigned nev, unsigned ncv, unsigned max_restarts, double tol, int order, const double* V0, double* evals_out,
+                                    double* X_out, int* info);
+
+// The 2D 5-point Dirichlet Laplacian 128 x 128 (n = 16384), b = 4, the 8
+// smallest eigenpairs, ncv = 256, tol 1e-8, solved by the UNCHANGED
+// trl::BlockLanczos (ModifiedGS) on trl::hip in ascending order.  Checked
+// against the closed form 4 - 2cos(i pi/129) - 2cos(j pi/129) and against the
+// oracle's restated driver from the same start block (the convergence check
+// of tests/test_lanczos_convergence.hh:12-52 on this problem), with the
+// restart counts of both printed.
+bool check_laplace2d_128(const trl::hip::Context& ctx)
+{
+  constexpr unsigned bs = 4;
+  const int side = 128;
+  const std::size_t N = std::size_t(side) * side;
+  std::vector<int32_t> rp{0}, ci;
+  std::vector<double> v;
+  for (int y = 0; y < side; ++y)
+    for (int x = 0; x < side; ++x) {
+      const int r = y * side + x;
+      const int nb[5] = {r - side, r - 1, r, r + 1, r + side};
+      const bool ok[5] = {y > 0, x > 0, true, x + 1 < side, y + 1 < side};
+      for (int q = 0; q < 5; ++q)
+        if (ok[q]) {
+          ci.push_back(nb[q]);
+          v.push_back(q == 2 ? 4.0 : -1.0);
+        }
+      rp.push_back(int32_t(ci.size()));
+    }
+  using EVP = trl::hip::CSREVP<double, bs>;
+  auto evp = std::shared_ptr<EVP>(EVP::from_host(ctx, N, rp, ci, v).release());
+  evp->set_order(trl::hip::EigenvalueOrder::Ascending);
+  HipTestHelper<EVP> h(ctx);
+  const unsigned nev = 8, ncv = 256;
+  const double tol = 1e-8;
+  trl::EigensolverParams params{.nev = nev, .ncv = ncv, .max_restarts = 1000, .tolerance = tol};
+  trl::BlockLanczos<EVP> lz(evp, params);
+  h.set_random(lz.initial_block());
+  const std::vector<double> V0 = h.to_host(lz.initial_block());
+  const auto res = lz.solve();
+  h.sync();
+  const auto ev = evp->get_current_eigenvalues();
+
+  std::vector<double> exact;
+  const double pi = std::acos(-1.0);
+  for (int i = 1; i <= side; ++i)
+    for (int j = 1; j <= side; ++j) exact.push_back(4.0 - 2.0 * std::cos(i * pi / (side + 1)) - 2.0 * std::cos(j * pi / (side + 1)));
+  std::sort(exact.begin(), exact.end());
+
+  std::vector<double> ev_or(nev, 0.0);
+  int info[4] = {0, 0, 0, 0};
+  const int rc = oracle_lanczos_solve(N, rp.data(), ci.data(), v.data(), nullptr, nullptr, int(bs), nev, ncv, 1000, tol, 1, V0.data(), ev_or.data(),
+                                      nullptr, info);
+  double err_ex = 0.0, err_or = 0.0;
+  for (unsigned i = 0; i < nev && i < ev.size(); ++i) {
+    err_ex = std::max(err_ex, std::abs(ev[i] - exact[i]) / exact[i]);
+    err_or = std::max(err_or, std::abs(ev[i] - ev_or[i]) / std::abs(ev_or[i]));
+  }
+  const bool ok = res.converged && rc == 0 && info[0] == 1 && ev.size() >= nev && err_ex < 1e-10 && err_or < 1e-10;
+  std::printf("reference driver, 2D Laplacian 128^2 b=4 nev=8 ncv=256: %u restarts (oracle %d), max rel err vs closed form %.2e, vs oracle %.2e -> %s\n",
+              res.iterations, info[1], err_ex, err_or, ok ? "ok" : "FAILED");
+  return ok;
+}
+
 int main()
 {
   trl::hip::Context ctx(0);
@@ -66,6 +139,7 @@ int main()
   }
   failed += !check_cgs2<4>(ctx);
   failed += !check_cgs2<8>(ctx);
+  failed += !check_laplace2d_128(ctx);
   std::printf("reference driver on trl::hip: %s\n", failed ? "FAILED" : "ALL PASSED");
   return failed;
 }

@@ -5,7 +5,11 @@ eigenpairs; the backward is trl_hip_sddmm_adjoint on the device plus the
 chain rule to (theta, w).  Checked against the FP64 closed form
 dloss/dtheta_i = sum_k c_k v_ik^2, dloss/dw_e = sum_k c_k (v_ik - v_jk)^2
 evaluated on the host from the same eigenvectors, and against central finite
-differences (h = 1e-6) of fresh forward solves.  Bar: |adjoint - FD| <= 1e-6.
+differences (h = 1e-6) of fresh forward solves.  Bars: on the 10 theta and
+10 edge-weight coordinates of largest |gradient| at full size,
+|adjoint - FD| / |adjoint| <= 1e-4; on 100 random coordinates of a 64 x 64
+instance (most of whose gradients sit below the FD noise floor)
+|adjoint - FD| <= 1e-6.
 """
 
 import numpy as np
@@ -65,23 +69,37 @@ def test_adjoint_fd_100_coordinates(ctx):
     assert err.max() <= 1e-6
 
 
-def test_adjoint_c5_full_size(ctx):
+@pytest.fixture(scope="module")
+def c5(ctx):
     # C5 at full size: 1000 x 1000 grid (n = 1M), 16 smallest, FP64
     print("adjoint c5 1000x1000", flush=True)
     g, c = config_c5()
     mod = GridSpectralLoss(ctx, g, c, b=8, ncv=256, tol=1e-10)
     _, gt, gw = run(mod, g)
+    return mod, g, gt, gw
+
+
+def test_adjoint_c5_full_size(c5):
+    mod, g, gt, gw = c5
     assert mod.last.resid.max() <= 1e-10
     ct, cw = closed_form(mod)
     rt = np.abs(gt - ct).max() / np.abs(ct).max()
     rw = np.abs(gw - cw).max() / np.abs(cw).max()
     print(f"sddmm vs closed form: theta {rt:.2e}, w {rw:.2e}")
     assert rt <= 1e-12 and rw <= 1e-12
-    # FD on the two largest theta and edge gradients (fresh full solves)
+
+
+# the 10 theta and the 10 edge-weight coordinates of largest |gradient|, in
+# chunks of 5 (each FD value is two fresh full-size solves)
+@pytest.mark.parametrize("kind,chunk", [("theta", 0), ("theta", 1), ("w", 0), ("w", 1)])
+def test_adjoint_c5_top_gradients_relative(c5, kind, chunk):
+    mod, g, gt, gw = c5
+    grad = gt if kind == "theta" else gw
+    idx = np.argsort(-np.abs(grad))[5 * chunk : 5 * chunk + 5]
     worst = 0.0
-    for kind, grad in (("theta", gt), ("w", gw)):
-        for i in np.argsort(-np.abs(grad))[:2]:
-            fd = central_fd(mod, g.theta, g.weights, kind, int(i))
-            print(f"{kind}[{i}]: adjoint {grad[i]:.12e} fd {fd:.12e}", flush=True)
-            worst = max(worst, abs(fd - grad[i]))
-    assert worst <= 1e-6
+    for i in idx:
+        fd = central_fd(mod, g.theta, g.weights, kind, int(i))
+        r = abs(fd - grad[i]) / abs(grad[i])
+        print(f"{kind}[{i}]: adjoint {grad[i]:.12e} fd {fd:.12e} rel {r:.2e}", flush=True)
+        worst = max(worst, r)
+    assert worst <= 1e-4

@@ -45,7 +45,7 @@ def test_batch_matches_single_problem_solves(ctx, tune):
         print(f"problem {p} n={A.n}: cycles {r['iterations']} (single {ref.result['iterations']}), max rel {rel.max():.2e}, "
               f"resid {out.resid[p].max():.2e}")
         assert rel.max() <= 1e-10
-        assert out.resid[p].max() <= 1e-7
+        assert out.resid[p].max() <= 1e-8  # the solve's tolerance
         # Ritz vectors: orthonormal and A x = lambda x
         Xh = np.concatenate(list(out.X[p]), axis=1)[:, :NEV]
         assert np.abs(Xh.T @ Xh - np.eye(NEV)).max() < 1e-10
@@ -90,7 +90,7 @@ def test_batch_adaptive_reorthogonalisation(ctx):
         print(f"problem {p} n={A.n}: cycles {out.results[p]['iterations']} (single {ref.result['iterations']}, two-pass batch "
               f"{two.results[p]['iterations']}), max rel {rel.max():.2e} / {rel2.max():.2e}")
         assert rel.max() <= 1e-10 and rel2.max() <= 1e-10
-        assert out.resid[p].max() <= 1e-7
+        assert out.resid[p].max() <= 1e-8  # the solve's tolerance
         Xh = np.concatenate(list(out.X[p]), axis=1)[:, :NEV]
         assert np.abs(Xh.T @ Xh - np.eye(NEV)).max() < 1e-10
     e = solve_batch(ctx, mats, B, NEV, NCV, V0, tune=False, reorth="dgks", use_graph=False)
@@ -124,7 +124,7 @@ def test_batch_c4_64_uneven_problems(ctx):
           f"p99 {np.percentile(t, 99) * 1e3:.1f} ms, cycles {st['cycles']}, host {st['host_seconds']:.3f} s, tiles {st['ntiles']}, "
           f"format {st['format']}, rows/thread {st['rows_per_thread']}")
     assert all(r["converged"] == 1 for r in out.results)
-    assert out.resid.max() <= 1e-7
+    assert out.resid.max() <= 1e-8  # the solve's tolerance
     # spot checks against single-problem solves: the smallest, the largest and two others
     order = np.argsort([m.n for m in mats])
     for p in (order[0], order[-1], order[20], order[45]):
@@ -132,3 +132,50 @@ def test_batch_c4_64_uneven_problems(ctx):
         rel = np.abs(out.evals[p] - ref.evals) / np.abs(ref.evals)
         print(f"problem {p} n={mats[p].n}: max rel vs single {rel.max():.2e}")
         assert rel.max() <= 1e-10
+
+
+def test_batch_barrier_error_path(ctx):
+    # the barrier error flag raised from the host, as a timed-out device
+    # barrier raises it: trl_hip_batch_extend returns TRL_HIP_EBARRIER (every
+    # step kernel returns at once, so every launch ends); after
+    # trl_hip_batch_reset_flags the same extend runs and gives the
+    # coefficients of an undisturbed run bit for bit
+    import ctypes as C
+
+    from trl_amd import _lib as L
+
+    H = L.hip()
+    mats, V0 = small_batch((18, 23))
+    nprob, m = len(mats), NCV // B
+    keep = [np.ascontiguousarray(a) for A in mats for a in (A.rowptr.astype(np.int32), A.col.astype(np.int32), A.val.astype(np.float64))]
+    n = (C.c_size_t * nprob)(*[A.n for A in mats])
+    rp = (C.c_void_p * nprob)(*[keep[3 * p].ctypes.data for p in range(nprob)])
+    ci = (C.c_void_p * nprob)(*[keep[3 * p + 1].ctypes.data for p in range(nprob)])
+    va = (C.c_void_p * nprob)(*[keep[3 * p + 2].ctypes.data for p in range(nprob)])
+    bt = C.c_void_p()
+    L.check(H.trl_hip_batch_create(ctx.h, nprob, n, rp, ci, va, B, NCV, C.byref(bt)), "batch_create")
+    try:
+        def extend():
+            for p in range(nprob):
+                v = np.ascontiguousarray(V0[p])
+                L.check(H.trl_hip_batch_set_block(bt, p, 0, v.ctypes.data), "set_block")
+            alpha = np.zeros(nprob * m * B * B)
+            beta = np.zeros(nprob * m * B * B)
+            pflag = np.zeros(nprob, dtype=np.int32)
+            rc = H.trl_hip_batch_extend(bt, 0, m, 0, None, alpha.ctypes.data, beta.ctypes.data, pflag.ctypes.data)
+            return rc, alpha, beta, pflag
+
+        rc, a0, b0, f0 = extend()
+        assert rc == 0 and not f0.any()
+        L.check(H.trl_hip_batch_raise_error(bt), "raise_error")
+        rc, _, _, _ = extend()
+        print("extend with the barrier error flag raised:", rc, L.hip().trl_hip_strerror(rc))
+        assert rc == -6
+        rc, _, _, _ = extend()  # the flag is sticky until reset
+        assert rc == -6
+        L.check(H.trl_hip_batch_reset_flags(bt), "reset_flags")
+        rc, a1, b1, f1 = extend()
+        assert rc == 0 and not f1.any()
+        assert np.array_equal(a0, a1) and np.array_equal(b0, b1)
+    finally:
+        H.trl_hip_batch_destroy(bt)

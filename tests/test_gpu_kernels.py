@@ -667,3 +667,62 @@ def test_spmm_axpby(ctx, rng, p, b, variant):
     ref = al * AX + be * X + (0.0 if variant == "noz" else ga * Z)
     assert rel(Yd.download(), ref) < TOL[p]
     assert H().trl_hip_spmm_csr_axpby(ctx.h, L.dtype_code(dt), A.n, b, rp.ptr, ci.ptr, va.ptr, None, Xd.ptr, Xd.ptr, None, 1.0, 0.0, 0.0) == -1
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("fmt,variant", [("csr", 1), ("csr", -2), ("csr", -4), ("csr", 2), ("csr", 8), ("ell", 1), ("ell", -2), ("ell", -4)])
+def test_spmm_offsets64(ctx, rng, p, fmt, variant):
+    # the 64-bit-offset instantiations (taken on their own only for panels or
+    # ELL arrays beyond 4 GiB), forced by the context option: the same
+    # products, bit for bit, as the 32-bit-offset ones, and the oracle's, for
+    # one panel, several panels, the axpby and forced steps, in index and
+    # blocked row order
+    b, dt = 8, DT[p]
+    if variant < 0 and not split_ok(p, b, -variant):
+        pytest.skip("column split needs 8 bytes per lane")
+    print(f"offsets64 {p} {fmt} {variant}", flush=True)
+    A = P.laplacian_3d(11, 9, 7, rng.uniform(0, 1, 693))
+    npan = 3
+    X = rng.standard_normal((npan, A.n, b)).astype(dt)
+    Z = rng.standard_normal((npan, A.n, b)).astype(dt)
+    F = rng.standard_normal((npan, A.n, b)).astype(dt)
+    g = rng.standard_normal((npan, b))
+    d = rng.uniform(0.5, 2, A.n).astype(dt)
+    if fmt == "ell":
+        width, hcol, hval = csr_to_ell(A)
+        M = spmat(1, None, width, dev(ctx, hcol, np.int32), dev(ctx, hval, dt), dev(ctx, d, dt), variant)
+    else:
+        M = spmat(0, dev(ctx, A.rowptr, np.int32), 0, dev(ctx, A.col, np.int32), dev(ctx, A.val, dt), dev(ctx, d, dt), variant)
+    Xd, Fd, gd = dev(ctx, X, dt), dev(ctx, F, dt), dev(ctx, g, np.float64)
+    t = L.dtype_code(dt)
+
+    def run(order):
+        M.plane, M.plane_block = order
+        out = []
+        Y = ctx.empty((npan, A.n, b), dt)
+        L.check(H().trl_hip_spmm(ctx.h, t, A.n, b, C.byref(M), Xd.ptr, Y.ptr), "spmm")
+        out.append(Y.download()[0])
+        L.check(H().trl_hip_spmm_panels(ctx.h, t, A.n, b, C.byref(M), npan, Xd.ptr, Y.ptr), "panels")
+        out.append(Y.download())
+        Zd = dev(ctx, Z, dt)
+        L.check(H().trl_hip_spmm_axpby_panels(ctx.h, t, A.n, b, C.byref(M), npan, Xd.ptr, Zd.ptr, Zd.ptr, 0.5, -1.0, 0.25), "axpby")
+        out.append(Zd.download())
+        L.check(H().trl_hip_spmm_axpby_forced(ctx.h, t, A.n, b, C.byref(M), npan, Xd.ptr, Y.ptr, None, 0.5, -1.0, 0.0, Fd.ptr, gd.ptr), "forced")
+        out.append(Y.download())
+        return out
+
+    AX = np.stack([oracle.spmm(A, X[j].astype(np.float64), d.astype(np.float64)) for j in range(npan)])
+    for order in [(0, 0), (11 * 9, 40)]:
+        ref32 = run(order)
+        ctx.set_option(L.OPT_OFFSETS64, 1)
+        try:
+            got = run(order)
+        finally:
+            ctx.set_option(L.OPT_OFFSETS64, 0)
+        for a, c in zip(got, ref32):
+            assert np.array_equal(a, c), order
+        assert rel(got[0], AX[0]) < TOL[p]
+        assert rel(got[1], AX) < TOL[p]
+        assert rel(got[2], 0.5 * AX - X + 0.25 * Z) < TOL[p]
+        assert rel(got[3], 0.5 * AX - X + F * g[:, None, :]) < TOL[p]
+    assert H().trl_hip_ctx_set_option(ctx.h, 99, 1) == -1

@@ -62,7 +62,11 @@ def test_laplace2d_128_matches_oracle(ctx, reorth):
     print("rel vs oracle", rel_or.max(), "rel vs closed form", rel_ex.max(), "restarts", out.result["iterations"], ref["iterations"])
     assert rel_or.max() <= 1e-10
     assert rel_ex.max() <= 1e-10
-    assert out.resid.max() <= 1e-7
+    # the Ritz estimate ||beta y_last|| / |theta| < tol that stops the solve
+    # (evp_base.hh:125-151) equals ||A x - theta x|| / ||A x|| up to rounding
+    # for a standard problem: the true residuals meet the tolerance
+    print("true residuals", np.array2string(out.resid, precision=2))
+    assert out.resid.max() <= c.tol
     X = out.X.download()  # (nblk, n, b)
     Xh = np.concatenate(list(X), axis=1)[:, : c.nev]
     ang = max_angle(Xh, ref["X"])
@@ -119,7 +123,20 @@ def test_generalized_matches_scipy_golden(ctx, reorth):
     ref = np.array(g["eigenvalues"])
     print("rel vs scipy", (np.abs(out.evals - ref) / ref).max())
     assert np.all(np.abs(out.evals - ref) <= 1e-10 * ref)
-    assert out.resid.max() < 1e-7
+    # The solve runs on M^{-1} K in the M-inner product, and its stopping
+    # estimate ||beta y_last|| / |theta| < tol (evp_base.hh:125-151) bounds
+    # ||M^{-1} K x - theta x||_M / ||M^{-1} K x||_M = ||K x - theta M x||_{M^-1}
+    # / ||K x||_{M^-1}: that residual meets the tolerance.  The 2-norm
+    # residual the library reports is bounded only up to sqrt(cond(M)).
+    X = np.concatenate(list(out.X.download()), axis=1)[:, : c.nev]
+    KX = c.csr.to_scipy() @ X
+    R = KX - (c.mass[:, None] * X) * out.evals
+    w = 1.0 / np.sqrt(c.mass)[:, None]
+    rm = np.linalg.norm(R * w, axis=0) / np.linalg.norm(KX * w, axis=0)
+    print("M^-1-norm residuals", np.array2string(rm, precision=2), "2-norm residuals", np.array2string(out.resid, precision=2),
+          "sqrt(cond(M))", float(np.sqrt(c.mass.max() / c.mass.min())))
+    assert rm.max() <= c.tol
+    assert out.resid.max() <= c.tol * np.sqrt(c.mass.max() / c.mass.min())
 
 
 def test_fp32_and_mixed_laplace2d(ctx):
@@ -132,7 +149,7 @@ def test_fp32_and_mixed_laplace2d(ctx):
     print("mixed", om.result, om.result32, (np.abs(om.evals - c.exact) / c.exact).max())
     assert om.result["converged"] == 1
     assert np.all(np.abs(om.evals - c.exact) <= 1e-10 * c.exact)
-    assert om.resid.max() <= 1e-7
+    assert om.resid.max() <= 1e-8
 
 
 def test_errors(ctx):

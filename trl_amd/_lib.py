@@ -16,6 +16,8 @@ LIB_DIR = Path(__file__).resolve().parent / "lib"
 
 F32 = 0
 F64 = 1
+# trl_hip_ctx_set_option options
+OPT_OFFSETS64 = 1
 
 c_size = C.c_size_t
 c_int = C.c_int
@@ -45,6 +47,7 @@ HIP_API = [
     ("trl_hip_ctx_check", c_int, [c_void_p, c_int_p]),
     ("trl_hip_ctx_stream", c_void_p, [c_void_p]),
     ("trl_hip_ctx_num_cus", c_int, [c_void_p]),
+    ("trl_hip_ctx_set_option", c_int, [c_void_p, c_int, c_int]),
     ("trl_hip_strerror", C.c_char_p, [c_int]),
     ("trl_hip_malloc", c_int, [c_void_p, c_size, C.POINTER(c_void_p)]),
     ("trl_hip_free", c_int, [c_void_p, c_void_p]),
@@ -104,6 +107,7 @@ HIP_API = [
     ("trl_hip_batch_save_blocks", c_int, [c_void_p, c_int, c_int]),
     ("trl_hip_batch_residuals", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_batch_reset_flags", c_int, [c_void_p]),
+    ("trl_hip_batch_raise_error", c_int, [c_void_p]),
     ("trl_hip_sddmm_adjoint", c_int, [c_void_p, c_int, c_size, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 ]
 
@@ -262,6 +266,9 @@ class Context:
         c, s = C.c_longlong(), C.c_longlong()
         check(hip().trl_hip_prof_reorth(self.h, C.byref(c), C.byref(s)), "prof_reorth")
         return {"calls": c.value, "second_passes": s.value}
+
+    def set_option(self, option: int, value: int) -> None:
+        check(hip().trl_hip_ctx_set_option(self.h, int(option), int(value)), "ctx_set_option")
 
     @property
     def num_cus(self) -> int:

@@ -135,15 +135,36 @@ def fd_coordinates(grid: P.WeightedGrid, count: int = 100, seed: int = 42) -> li
     return out
 
 
+def top_coordinates(gt: np.ndarray, gw: np.ndarray, k: int) -> list[tuple[str, int]]:
+    """The k theta and the k edge-weight coordinates of largest |gradient|."""
+    return [("theta", int(i)) for i in np.argsort(-np.abs(gt))[:k]] + [("w", int(i)) for i in np.argsort(-np.abs(gw))[:k]]
+
+
+def central_fd(mod: GridSpectralLoss, theta: np.ndarray, w: np.ndarray, kind: str, i: int, h: float) -> float:
+    """(loss(p + h e_i) - loss(p - h e_i)) / 2h from two fresh forward solves."""
+    th, ww = theta.copy(), w.copy()
+    arr = th if kind == "theta" else ww
+    arr[i] += h
+    lp = float(mod.c @ mod.eigs(th, ww).evals)
+    arr[i] -= 2 * h
+    lm = float(mod.c @ mod.eigs(th, ww).evals)
+    return (lp - lm) / (2 * h)
+
+
 def main(argv=None) -> int:
     """Central-FD check of the adjoint on the C5 workload: prints one line per
-    coordinate (adjoint, FD, difference) and a summary."""
+    coordinate (adjoint, FD, absolute and relative difference) and a summary.
+    --top K checks the K theta and K edge-weight coordinates of largest
+    |gradient| (relative error bar 1e-4); otherwise --count coordinates of
+    stream 7 (absolute bar 1e-6: at most of them the gradient is below the
+    FD noise floor)."""
     import argparse
     import time
 
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--side", type=int, default=1000)
     ap.add_argument("--count", type=int, default=100)
+    ap.add_argument("--top", type=int, default=0)
     ap.add_argument("--h", type=float, default=1e-6)
     ap.add_argument("--ncv", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-10)
@@ -161,22 +182,21 @@ def main(argv=None) -> int:
     gt, gw = th.grad.numpy(), w.grad.numpy()
     print(f"C5 {a.side}x{a.side}: n={g.csr.n} nnz={g.csr.nnz} loss={loss.item():.15e} forward {t1 - t0:.2f}s backward {t2 - t1:.3f}s "
           f"cycles {mod.last.result['iterations']} max resid {mod.last.resid.max():.2e}", flush=True)
+    coords = top_coordinates(gt, gw, a.top) if a.top else fd_coordinates(g, a.count)
     worst, worst_rel = 0.0, 0.0
-    scale = max(np.abs(gt).max(), np.abs(gw).max())
-    for k, (kind, i) in enumerate(fd_coordinates(g, a.count)):
-        th0, w0 = g.theta.copy(), g.weights.copy()
-        arr = th0 if kind == "theta" else w0
-        arr[i] += a.h
-        lp = float(c @ mod.eigs(th0, w0).evals)
-        arr[i] -= 2 * a.h
-        lm = float(c @ mod.eigs(th0, w0).evals)
-        fd = (lp - lm) / (2 * a.h)
+    for k, (kind, i) in enumerate(coords):
+        fd = central_fd(mod, g.theta, g.weights, kind, i, a.h)
         adj = float(gt[i] if kind == "theta" else gw[i])
         d = abs(fd - adj)
+        r = d / abs(adj) if adj != 0.0 else float("inf")
         worst = max(worst, d)
-        worst_rel = max(worst_rel, d / scale)
-        print(f"{k:3d} {kind:5s}[{i:7d}] adjoint {adj: .12e} fd {fd: .12e} |diff| {d:.2e}", flush=True)
-    print(f"max |adjoint - fd| = {worst:.3e} over {a.count} coordinates; relative to max |gradient| ({scale:.3e}): {worst_rel:.3e}", flush=True)
+        worst_rel = max(worst_rel, r)
+        print(f"{k:3d} {kind:5s}[{i:7d}] adjoint {adj: .12e} fd {fd: .12e} |diff| {d:.2e} rel {r:.2e}", flush=True)
+    if a.top:
+        print(f"max relative |adjoint - fd| / |adjoint| = {worst_rel:.3e} over the {a.top} largest-|gradient| theta and w coordinates "
+              f"(max |diff| {worst:.3e})", flush=True)
+        return 0 if worst_rel <= 1e-4 else 1
+    print(f"max |adjoint - fd| = {worst:.3e} over {a.count} coordinates", flush=True)
     return 0 if worst <= 1e-6 else 1
 
 

@@ -211,6 +211,18 @@ void* trl_hip_ctx_stream(trl_hip_ctx* ctx) { return ctx ? static_cast<void*>(ctx
 
 int trl_hip_ctx_num_cus(trl_hip_ctx* ctx) { return ctx ? ctx->num_cus : 0; }
 
+int trl_hip_ctx_set_option(trl_hip_ctx* ctx, int option, int value)
+{
+  if (!ctx) return TRL_HIP_EINVAL;
+  switch (option) {
+  case TRL_HIP_OPT_OFFSETS64:
+    ctx->force_offsets64 = value ? 1 : 0;
+    return 0;
+  default:
+    return TRL_HIP_EINVAL;
+  }
+}
+
 int trl_hip_malloc(trl_hip_ctx* ctx, size_t bytes, void** out)
 {
   if (!ctx || !out) return TRL_HIP_EINVAL;

@@ -17,6 +17,9 @@ struct trl_hip_ctx {
   hipStream_t stream = nullptr;
   int num_cus = 256;
   size_t max_lds = 65536; // dynamic LDS bytes a workgroup may request
+  // TRL_HIP_OPT_OFFSETS64: the operator kernels take their 64-bit-offset
+  // instantiations even where 32-bit lane offsets would do
+  int force_offsets64 = 0;
   // per-workgroup partial sums of the block products (FP64)
   double* partials = nullptr;
   size_t partials_cap = 0; // in doubles

@@ -1140,6 +1140,15 @@ int trl_hip_batch_residuals(trl_hip_batch* bt, int nblk, const double* lambda, d
   return 0;
 }
 
+int trl_hip_batch_raise_error(trl_hip_batch* bt)
+{
+  if (!bt) return TRL_HIP_EINVAL;
+  const int one = 1;
+  TRL_HIP_TRY(hipMemcpyAsync(bt->err, &one, sizeof(int), hipMemcpyHostToDevice, bt->ctx->stream));
+  TRL_HIP_TRY(hipStreamSynchronize(bt->ctx->stream));
+  return 0;
+}
+
 int trl_hip_batch_reset_flags(trl_hip_batch* bt)
 {
   if (!bt) return TRL_HIP_EINVAL;

@@ -509,7 +509,7 @@ int launch_rows_any(trl_hip_ctx* ctx, size_t n, int variant, const int32_t* rowp
                     const Epi& e)
 {
   const size_t lim = size_t(UINT32_MAX);
-  const bool o32 = n * size_t(B) * sizeof(S) <= lim && (rowptr || size_t(width) * n * std::max<size_t>(sizeof(S), 4) <= lim);
+  const bool o32 = !ctx->force_offsets64 && n * size_t(B) * sizeof(S) <= lim && (rowptr || size_t(width) * n * std::max<size_t>(sizeof(S), 4) <= lim);
   if (o32) return launch_panels<S, B, AXPBY, true>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
   return launch_panels<S, B, AXPBY, false>(ctx, n, variant, rowptr, width, col, val, X, Y, e);
 }

@@ -11,6 +11,8 @@ Kernels (--what, comma separated):
   forced32              that FP32 step plus the forcing term F diag(g) (the residual-driven
                         filter of the FP64 sweeps)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
+  rotate32, rotate64    the Ritz rotation of a refinement sweep (m = kc = --panels blocks)
+  restart64             the FP64 Lanczos restart rotation (m = 32 blocks, kc = 5)
   cholqr32, cholqr64    CholQR2 of one FP32 panel, CholQR of one FP64 panel
   copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
 Prints, per kernel, the mean time per launch (one HIP event pair on the
@@ -123,6 +125,19 @@ def main(argv=None) -> int:
             else:
                 Cs = ctx.to_device(np.zeros(k * b * b))  # W unchanged: W -= V 0
                 timed(w, lambda: L.check(H.trl_hip_update(ctx.h, L.F32, n, b, k, V.ptr, Cs.ptr, W), "update"), (k + 2) * P_)
+        elif w in ("rotate32", "rotate64", "restart64"):
+            # the Ritz rotation Out = V[0:m] Y: a refinement sweep's (m = kc =
+            # --panels) in FP32 / FP64, and the FP64 Lanczos restart's (m = 32
+            # blocks, kc = 5)
+            f32 = w == "rotate32"
+            dt = np.float32 if f32 else np.float64
+            s_ = 4 if f32 else 8
+            m, kc = (32, 5) if w == "restart64" else (a.panels, a.panels)
+            V = ctx.to_device(rng.standard_normal((m, n, b)).astype(dt))
+            Yd = ctx.to_device(rng.standard_normal((m * b, kc * b)))
+            Out = ctx.empty((kc, n, b), dt)
+            timed(w, lambda: L.check(H.trl_hip_rotate(ctx.h, L.F32 if f32 else L.F64, n, b, m, V.ptr, Yd.ptr, kc, kc, Out.ptr), "rotate"),
+                  (m + kc) * n * b * s_)
         elif w in ("cholqr32", "cholqr64"):
             f32 = w == "cholqr32"
             Wd = ctx.to_device(rng.standard_normal((n, b)).astype(np.float32 if f32 else np.float64))

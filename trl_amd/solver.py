@@ -18,6 +18,15 @@ from . import _lib as L
 from .problems import CSR
 
 ORDER = {"largest_magnitude": 0, "ascending": 1, "descending": 2}
+
+# The mixed-precision schedule of the 160^3 benchmark (the 3D 7-point
+# Laplacian + diagonal potential, b = 8, 32 smallest, ncv = 256, tol 1e-8):
+# one loose FP32 thick-restart cycle over 192 columns, FP32 Chebyshev /
+# Rayleigh-Ritz sweeps on 5 Ritz blocks to an FP32 residual of 1e-3, then FP64
+# sweeps whose filter runs in FP32 driven by the FP64 residuals, per-block
+# filter degrees (160 at most), adaptive block CGS, autotuned operator.
+MIXED_SCHEDULE = dict(tol_fp32=0.4, cheb_degree=160, refine_blocks=5, max_refine=14, fp32_sweeps=12, tol_fp32_refine=1e-3, degree_opt=1,
+                      residual_filter=True, ncv_fp32=192, reorth="dgks", spmm_tune=True)
 REORTH = {"mgs": 0, "cgs2": 1, "dgks": 2}
 
 
