@@ -113,6 +113,19 @@ int trl_hip_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void*
  * (reorthogonalization.hh:14-21). */
 int trl_hip_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* mdiag, double* C);
 
+/* trl_hip_gram without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and result as trl_hip_gram, with the per-workgroup partial sums in
+ * a fixed 8 MiB device array of the library (one per device, held until the
+ * process ends, its address resolved at trl_hip_ctx_create) instead of the
+ * context's growable buffer.  k*b*b must not exceed
+ * TRL_HIP_NOALLOC_MAX_COEFS (TRL_HIP_EINVAL otherwise).  Every call on one
+ * device shares that array: calls that overlap on one device (several
+ * contexts or host threads on the same GPU) race. */
+#define TRL_HIP_NOALLOC_MAX_COEFS 4096
+int trl_hip_gram_noalloc(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* mdiag, double* C);
+
 /* W -= sum_{j<k} V_j * C_j with C as returned by trl_hip_gram: the fused
  * form of the subtract_product loop of ModifiedGS (reorthogonalization.hh:19). */
 int trl_hip_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W);

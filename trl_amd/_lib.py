@@ -59,6 +59,7 @@ HIP_API = [
     ("trl_hip_panel_sub", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p]),
     ("trl_hip_panel_mult", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
     ("trl_hip_gram", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("trl_hip_gram_noalloc", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_update", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_cholqr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
     ("trl_hip_rotate", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),

@@ -155,10 +155,14 @@ int trl_hip_ctx_create(int device, trl_hip_ctx** out)
     return static_cast<int>(e);
   }
   int rc = ensure_partials(ctx, size_t(1) << 18);
+  // the allocation-free entry point's scratch on this device (current here)
+  void* scr = nullptr;
+  if (!rc) rc = noalloc_scratch(device, &scr);
   if (rc) {
     trl_hip_ctx_destroy(ctx);
     return rc;
   }
+  ctx->noalloc_scratch = static_cast<double*>(scr);
   *out = ctx;
   return 0;
 }
@@ -310,6 +314,15 @@ int trl_hip_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, cons
   if (!ctx || !valid_b(b) || k < 1 || !C || (n && (!V || !W))) return TRL_HIP_EINVAL;
   Prof pr(ctx, TRL_HIP_PROF_GRAM, (k + 1) * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0), t);
   return launch_gram(ctx, t, n, b, k, V, W, mdiag, C);
+}
+
+int trl_hip_gram_noalloc(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* mdiag, double* C)
+{
+  if (!ctx || !valid_b(b) || k < 1 || !C || (n && (!V || !W)) || !ctx->noalloc_scratch) return TRL_HIP_EINVAL;
+  if (size_t(k) * b * b > TRL_HIP_NOALLOC_MAX_COEFS) return TRL_HIP_EINVAL;
+  // the same kernels as trl_hip_gram, their partial sums in the device's
+  // fixed scratch array (no profiling events: those are created on demand)
+  return launch_gram(ctx, t, n, b, k, V, W, mdiag, C, nullptr, ctx->noalloc_scratch, kNoAllocScratchDoubles);
 }
 
 int trl_hip_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W)

@@ -20,6 +20,9 @@ struct trl_hip_ctx {
   // TRL_HIP_OPT_OFFSETS64: the operator kernels take their 64-bit-offset
   // instantiations even where 32-bit lane offsets would do
   int force_offsets64 = 0;
+  // device address of the allocation-free entry point's scratch array
+  // (resolved at creation; see noalloc_scratch)
+  double* noalloc_scratch = nullptr;
   // per-workgroup partial sums of the block products (FP64)
   double* partials = nullptr;
   size_t partials_cap = 0; // in doubles
@@ -70,8 +73,16 @@ int launch_convert(trl_hip_ctx* ctx, trl_hip_dtype st, const void* src, trl_hip_
 int launch_panel_sub(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const void* Y);
 int launch_panel_mult(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* X, const double* W, void* Y, int mode);
 // gate (device int, may be NULL): the kernels return at once when *gate == 0
+// partials/cap: a fixed buffer of cap doubles for the per-workgroup partial
+// sums instead of the context's growable one (the workgroup count is capped
+// to fit; TRL_HIP_EINVAL if one partial row per workgroup does not fit)
 int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* md, double* C,
-                const int* gate = nullptr);
+                const int* gate = nullptr, double* partials = nullptr, size_t cap = 0);
+// Scratch of the allocation-free Gram entry point: kNoAllocScratchDoubles
+// doubles per device, resolved once per device (the device must be current
+// on the first call for it).
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 20;
+int noalloc_scratch(int device, void** out);
 int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W, const int* gate = nullptr);
 int launch_dgks(trl_hip_ctx* ctx, int b, int k, const double* C, double eta2, int* gate);
 int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes);

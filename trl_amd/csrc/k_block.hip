@@ -13,6 +13,8 @@
 #include <cstdlib>
 #include <string>
 
+#include <mutex>
+
 #include "common.hh"
 
 using namespace trl_hip_dev;
@@ -592,8 +594,12 @@ inline int pick_tpw(int per_wave)
 
 // Queue the Gram partials for blocks [0, k) of V against W; returns the
 // number of partial rows written (nwg * RS) in *nw.
+// partials: where the rows go; nullptr = the context's growable buffer
+// (ctx->partials, enlarged here when needed), else a fixed buffer of cap
+// doubles, for which the workgroup count is capped so that the rows fit.
 template <typename S, int B>
-int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* W, const S* md, int* nw, const int* gate)
+int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* W, const S* md, int* nw, const int* gate, double* partials = nullptr,
+                        size_t cap = 0)
 {
   constexpr int BP = pad_b(B), BPT = 16 / BP;
   const int ntile = (k + BPT - 1) / BPT;
@@ -607,37 +613,45 @@ int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* 
   if (g > gmax) g = gmax;
   if (g < 1) g = 1;
   const size_t P = size_t(k) * B * B;
-  int rc = trl_hip_impl::ensure_partials(ctx, g * RS * P);
-  if (rc) return rc;
+  if (partials) {
+    const size_t fit = cap / (size_t(RS) * P);
+    if (fit < 1) return TRL_HIP_EINVAL;
+    if (g > fit) g = fit;
+  }
+  else {
+    int rc = trl_hip_impl::ensure_partials(ctx, g * RS * P);
+    if (rc) return rc;
+    partials = ctx->partials;
+  }
   const size_t ldv = n * B;
   const dim3 grid{unsigned(g)}, blk{unsigned(kThreads)};
   switch (tpw) {
   case 1:
-    hipLaunchKernelGGL((k_gram<S, B, 1>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 1>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   case 2:
-    hipLaunchKernelGGL((k_gram<S, B, 2>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 2>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   case 3:
-    hipLaunchKernelGGL((k_gram<S, B, 3>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 3>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   case 4:
-    hipLaunchKernelGGL((k_gram<S, B, 4>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 4>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   case 5:
-    hipLaunchKernelGGL((k_gram<S, B, 5>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 5>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   case 6:
-    hipLaunchKernelGGL((k_gram<S, B, 6>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 6>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   case 8:
-    hipLaunchKernelGGL((k_gram<S, B, 8>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 8>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   case 12:
-    hipLaunchKernelGGL((k_gram<S, B, 12>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 12>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   default:
-    hipLaunchKernelGGL((k_gram<S, B, 16>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, ctx->partials, gate);
+    hipLaunchKernelGGL((k_gram<S, B, 16>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
     break;
   }
   TRL_HIP_LAUNCH_CHECK();
@@ -714,7 +728,8 @@ int launch_dgks(trl_hip_ctx* ctx, int b, int k, const double* C, double eta2, in
   return 0;
 }
 
-int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* md, double* C, const int* gate)
+int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const void* W, const void* md, double* C, const int* gate,
+                double* partials, size_t cap)
 {
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
@@ -722,7 +737,8 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
       const S* w = static_cast<const S*>(W);
       const S* m = static_cast<const S*>(md);
       if (n == 0) {
-        hipLaunchKernelGGL(k_reduce, dim3(unsigned(size_t(k) * B * B)), dim3(kThreads), 0, ctx->stream, ctx->partials, 0, k * B * B, C, gate);
+        hipLaunchKernelGGL(k_reduce, dim3(unsigned(size_t(k) * B * B)), dim3(kThreads), 0, ctx->stream, partials ? partials : ctx->partials, 0, k * B * B,
+                           C, gate);
         TRL_HIP_LAUNCH_CHECK();
         return 0;
       }
@@ -730,14 +746,48 @@ int launch_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const
       for (int j0 = 0; j0 < k; j0 += kmax) {
         const int kc = (k - j0) < kmax ? (k - j0) : kmax;
         int nw = 0;
-        int rc = queue_gram_partials<S, B>(ctx, n, kc, v + size_t(j0) * n * B, w, m, &nw, gate);
+        int rc = queue_gram_partials<S, B>(ctx, n, kc, v + size_t(j0) * n * B, w, m, &nw, gate, partials, cap);
         if (rc) return rc;
         const int P = kc * B * B;
-        hipLaunchKernelGGL(k_reduce, dim3(unsigned(P)), dim3(kThreads), 0, ctx->stream, ctx->partials, nw, P, C + size_t(j0) * B * B, gate);
+        hipLaunchKernelGGL(k_reduce, dim3(unsigned(P)), dim3(kThreads), 0, ctx->stream, partials ? partials : ctx->partials, nw, P, C + size_t(j0) * B * B,
+                           gate);
         TRL_HIP_LAUNCH_CHECK();
       }
     });
   });
+  return 0;
+}
+
+// ---- the allocation-free Gram entry point's scratch --------------------------
+//
+// A file-scope device array holds the per-workgroup partial sums of
+// trl_hip_gram_noalloc.  Each device holds its own copy from the moment the
+// code object is loaded until the process ends.  In host code `scratch` is
+// only a stand-in for the symbol: its device address is resolved once per
+// device (hipGetSymbolAddress, with that device current) and that address is
+// what the kernels receive.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+} // namespace
+
+int noalloc_scratch(int device, void** out)
+{
+  if (!out || device < 0 || device >= kMaxDevices) return TRL_HIP_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    TRL_HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) return TRL_HIP_EINVAL; // the device must be current
+    void* ptr = nullptr;
+    TRL_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return TRL_HIP_ENOMEM;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
   return 0;
 }
 
