@@ -51,6 +51,8 @@ def lib() -> C.CDLL:
         L.oracle_lanczos_solve.argtypes = [C.c_size_t, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_uint, C.c_uint, C.c_uint, C.c_double, C.c_int, _dp, _dp, _dp,
                                            C.POINTER(C.c_int)]
         L.oracle_lanczos_solve.restype = C.c_int
+        L.oracle_lanczos_solve_timed.argtypes = L.oracle_lanczos_solve.argtypes + [_dp]
+        L.oracle_lanczos_solve_timed.restype = C.c_int
         L.oracle_lanczos_extend.argtypes = [C.c_size_t, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_uint, C.c_uint, _dp, _dp, _dp, _dp]
         L.oracle_lanczos_extend.restype = C.c_int
         L.oracle_time_extend.argtypes = [C.c_size_t, _ip, _ip, _dp, C.c_int, C.c_uint, C.c_uint, _dp, C.c_uint]
@@ -154,16 +156,18 @@ def rotate(V, Yblocks, kc):
 
 
 def lanczos_solve(csr, b, nev, ncv, V0, *, tol=1e-8, order=0, max_restarts=1000, mass=None, ritz_vectors=True):
-    """Thick-restart solve; returns dict(evals, X, converged, iterations, n_op_apply, error)."""
+    """Thick-restart solve; returns dict(evals, X, converged, iterations, n_op_apply, error, seconds); seconds
+    is the wall time of the driver's solve() alone (max_restarts = 1: one full restart cycle)."""
     n = csr.n
     dscale = None if mass is None else 1.0 / np.asarray(mass, dtype=np.float64)
     evals = np.zeros(nev)
     X = np.zeros((n, nev)) if ritz_vectors else None
     info = (C.c_int * 4)()
     V0 = f64(V0).reshape(n, b)
-    lib().oracle_lanczos_solve(n, _i(csr.rowptr), _i(csr.col), _d(f64(csr.val)), _d(f64(dscale)), _d(f64(mass)), b, nev, ncv, max_restarts, tol, order,
-                               _d(V0), _d(evals), _d(X), info)
-    return dict(evals=evals, X=X, converged=bool(info[0]), iterations=info[1], n_op_apply=info[2], error=info[3])
+    sec = C.c_double(0.0)
+    lib().oracle_lanczos_solve_timed(n, _i(csr.rowptr), _i(csr.col), _d(f64(csr.val)), _d(f64(dscale)), _d(f64(mass)), b, nev, ncv, max_restarts, tol,
+                                     order, _d(V0), _d(evals), _d(X), info, C.byref(sec))
+    return dict(evals=evals, X=X, converged=bool(info[0]), iterations=info[1], n_op_apply=info[2], error=info[3], seconds=sec.value)
 
 
 def lanczos_extend(csr, b, nev, ncv, V0, mass=None):

@@ -23,13 +23,13 @@
 // absent from this image, so the dense pieces are this file's own:
 //   cholesky_lower_llt ... Cholesky-Banachiewicz on the lower triangle
 //   invert_upper ......... back substitution
-//   syev ................. Householder tridiagonalisation + implicit QL with
-//                          Wilkinson-type shifts (the tred2/tql2 method of
-//                          Wilkinson & Reinsch, Handbook for Automatic
-//                          Computation vol. II, 1971)
+//   syev ................. Householder tridiagonalisation + implicit
+//                          symmetric QR with the Wilkinson shift (Golub &
+//                          Van Loan, Matrix Computations, Alg. 8.3.1-8.3.3)
 // Each is pinned by tests/test_oracle_*.py against numpy/LAPACK and closed forms.
 // ============================================================================
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -196,129 +196,126 @@ static void invert_upper(const double* R, int b, double* X)
 }
 
 // Symmetric eigen-decomposition, A (n x n row-major, symmetric) -> ascending
-// eigenvalues w and eigenvectors as the columns of Z (row-major n x n).
-// Householder reduction to tridiagonal form with accumulated transform, then
-// the implicit QL iteration with shifts on the tridiagonal.  Returns 0, or -1
-// if an eigenvalue fails to converge in 60 iterations.
+// eigenvalues w and eigenvectors as the columns of Z (row-major n x n), by
+// the textbook two-stage method of Golub & Van Loan, Matrix Computations
+// (4th ed.), written here on the full row-major matrix:
+//  1. Householder tridiagonalisation Q^T A Q = T (Alg. 8.3.1).  Step k takes
+//     the reflector H = I - beta v v^T (v(0) = 1, the stable choice of
+//     Alg. 5.1.1) that maps x = A(k+1:n, k) onto |x| e_1, and updates the
+//     trailing block as A22 -= v w^T + w v^T with p = beta A22 v,
+//     w = p - (beta p^T v / 2) v.  Q = H_0 H_1 ... is formed forward.
+//  2. Implicit symmetric QR steps with the Wilkinson shift on the unreduced
+//     trailing block of T (Alg. 8.3.2), deflating off-diagonal entries below
+//     eps (|d_i| + |d_{i+1}|) (Alg. 8.3.3).  A step starts from the shifted
+//     first column and chases the bulge down with Givens rotations, each of
+//     which is also applied to the columns of Q.
+// Returns 0, or -1 if the QR iteration needs more than 30 n steps.
 static int syev(int n, const double* A, double* w, double* Z)
 {
-  // work in column-major-equivalent "a[i][j]" = Z[i*n + j] on the symmetric copy
-  std::vector<double> a(A, A + size_t(n) * n);
-  std::vector<double> d(n), e(n);
-  auto at = [&](int i, int j) -> double& { return a[size_t(i) * n + j]; };
+  if (n <= 0) return 0;
+  const size_t N = size_t(n);
+  // Qt holds Q transposed (row j = column j of Q), so that both the
+  // reflector and the rotation updates run along contiguous rows
+  std::vector<double> a(A, A + N * N), Qt(N * N, 0.0);
+  auto at = [&](int i, int j) -> double& { return a[size_t(i) * N + j]; };
+  auto qrow = [&](int j) { return Qt.data() + size_t(j) * N; };
+  for (int i = 0; i < n; ++i) qrow(i)[i] = 1.0;
 
-  // --- Householder tridiagonalisation: rows i = n-1 down to 1 -------------
-  for (int i = n - 1; i > 0; --i) {
-    const int l = i - 1;
-    double h = 0.0;
-    if (l > 0) {
-      double scale = 0.0;
-      for (int k = 0; k <= l; ++k) scale += std::fabs(at(i, k));
-      if (scale == 0.0) {
-        e[i] = at(i, l);
-      }
-      else {
-        for (int k = 0; k <= l; ++k) {
-          at(i, k) /= scale;
-          h += at(i, k) * at(i, k);
-        }
-        double f = at(i, l);
-        double g = (f >= 0.0) ? -std::sqrt(h) : std::sqrt(h);
-        e[i] = scale * g;
-        h -= f * g;
-        at(i, l) = f - g;
-        f = 0.0;
-        for (int j = 0; j <= l; ++j) {
-          at(j, i) = at(i, j) / h;
-          g = 0.0;
-          for (int k = 0; k <= j; ++k) g += at(j, k) * at(i, k);
-          for (int k = j + 1; k <= l; ++k) g += at(k, j) * at(i, k);
-          e[j] = g / h;
-          f += e[j] * at(i, j);
-        }
-        const double hh = f / (h + h);
-        for (int j = 0; j <= l; ++j) {
-          f = at(i, j);
-          e[j] = g = e[j] - hh * f;
-          for (int k = 0; k <= j; ++k) at(j, k) -= (f * e[k] + g * at(i, k));
-        }
-      }
+  // ---- 1. tridiagonalisation ------------------------------------------------
+  std::vector<double> v(N), pv(N), wv(N);
+  for (int k = 0; k + 2 < n; ++k) {
+    const double x1 = at(k + 1, k);
+    double sigma = 0.0;
+    for (int i = k + 2; i < n; ++i) sigma += at(i, k) * at(i, k);
+    if (sigma == 0.0) continue; // already in tridiagonal form in this column
+    const double mu = std::sqrt(x1 * x1 + sigma);
+    const double v1 = x1 <= 0.0 ? x1 - mu : -sigma / (x1 + mu);
+    const double beta = 2.0 * v1 * v1 / (sigma + v1 * v1);
+    v[k + 1] = 1.0;
+    for (int i = k + 2; i < n; ++i) v[i] = at(i, k) / v1;
+    double pt_v = 0.0;
+    for (int i = k + 1; i < n; ++i) {
+      double acc = 0.0;
+      for (int j = k + 1; j < n; ++j) acc += at(i, j) * v[j];
+      pv[i] = beta * acc;
+      pt_v += pv[i] * v[i];
     }
-    else {
-      e[i] = at(i, l);
+    const double half = 0.5 * beta * pt_v;
+    for (int i = k + 1; i < n; ++i) wv[i] = pv[i] - half * v[i];
+#pragma omp parallel for if (n - k >= 128)
+    for (int i = k + 1; i < n; ++i)
+      for (int j = k + 1; j < n; ++j) at(i, j) -= v[i] * wv[j] + wv[i] * v[j];
+    at(k + 1, k) = at(k, k + 1) = mu;
+    for (int i = k + 2; i < n; ++i) at(i, k) = at(k, i) = 0.0;
+    // Q <- Q H: rows k+1.. of Qt lose beta v (v^T Qt(k+1:, :))
+    std::fill(pv.begin(), pv.end(), 0.0);
+    for (int j = k + 1; j < n; ++j) {
+      const double* qj = qrow(j);
+      for (int r = 0; r < n; ++r) pv[r] += v[j] * qj[r];
     }
-    d[i] = h;
+    for (int j = k + 1; j < n; ++j) {
+      double* qj = qrow(j);
+      const double f = beta * v[j];
+      for (int r = 0; r < n; ++r) qj[r] -= f * pv[r];
+    }
   }
-  d[0] = 0.0;
-  e[0] = 0.0;
-  // accumulate the transformation
-  for (int i = 0; i < n; ++i) {
-    const int l = i - 1;
-    if (d[i] != 0.0) {
-      for (int j = 0; j <= l; ++j) {
-        double g = 0.0;
-        for (int k = 0; k <= l; ++k) g += at(i, k) * at(k, j);
-        for (int k = 0; k <= l; ++k) at(k, j) -= g * at(k, i);
-      }
-    }
-    d[i] = at(i, i);
-    at(i, i) = 1.0;
-    for (int j = 0; j <= l; ++j) at(j, i) = at(i, j) = 0.0;
-  }
+  std::vector<double> d(N), e(N, 0.0); // e[i] = T(i, i+1)
+  for (int i = 0; i < n; ++i) d[i] = at(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = at(i + 1, i);
 
-  // --- implicit QL on (d, e) --------------------------------------------------
-  for (int i = 1; i < n; ++i) e[i - 1] = e[i];
-  if (n > 0) e[n - 1] = 0.0;
-  for (int l = 0; l < n; ++l) {
-    int iter = 0;
-    int m;
-    do {
-      for (m = l; m < n - 1; ++m) {
-        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-        if (std::fabs(e[m]) <= std::numeric_limits<double>::epsilon() * dd) break;
+  // ---- 2. implicit QR with the Wilkinson shift -------------------------------
+  const double eps = std::numeric_limits<double>::epsilon();
+  auto rotate_q = [&](int k, double c, double s) { // Q <- Q G, G = [c s; -s c] on columns k, k+1
+    double* qa = qrow(k);
+    double* qb = qrow(k + 1);
+    for (int r = 0; r < n; ++r) {
+      const double x0 = qa[r], x1 = qb[r];
+      qa[r] = c * x0 - s * x1;
+      qb[r] = s * x0 + c * x1;
+    }
+  };
+  long steps = 0;
+  int hi = n - 1;
+  while (hi > 0) {
+    if (std::fabs(e[hi - 1]) <= eps * (std::fabs(d[hi - 1]) + std::fabs(d[hi]))) {
+      e[hi - 1] = 0.0;
+      --hi;
+      continue;
+    }
+    int lo = hi - 1;
+    while (lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(d[lo - 1]) + std::fabs(d[lo]))) --lo;
+    if (lo > 0) e[lo - 1] = 0.0;
+    if (++steps > 30L * n) return -1;
+    // Wilkinson shift: the eigenvalue of the trailing 2 x 2 block nearer d[hi]
+    const double delta = 0.5 * (d[hi - 1] - d[hi]);
+    const double eb = e[hi - 1];
+    const double den = delta + (delta >= 0.0 ? 1.0 : -1.0) * std::hypot(delta, eb);
+    const double shift = d[hi] - eb * eb / den;
+    double x = d[lo] - shift, z = e[lo];
+    for (int k = lo; k < hi; ++k) {
+      // rotation on rows/columns k, k+1 zeroing z against x: c x - s z = r, s x + c z = 0
+      const double r = std::hypot(x, z);
+      const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : -z / r;
+      if (k > lo) e[k - 1] = r; // the entry above the bulge
+      const double dk = d[k], dk1 = d[k + 1], ek = e[k];
+      d[k] = c * c * dk - 2.0 * c * s * ek + s * s * dk1;
+      d[k + 1] = s * s * dk + 2.0 * c * s * ek + c * c * dk1;
+      e[k] = c * s * (dk - dk1) + (c * c - s * s) * ek;
+      rotate_q(k, c, s);
+      if (k + 1 < hi) {
+        x = e[k];
+        z = -s * e[k + 1]; // the new bulge T(k, k+2)
+        e[k + 1] *= c;
       }
-      if (m != l) {
-        if (iter++ == 60) return -1;
-        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = std::hypot(g, 1.0);
-        g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
-        double s = 1.0, c = 1.0, p = 0.0;
-        int i;
-        for (i = m - 1; i >= l; --i) {
-          double f = s * e[i];
-          const double bb = c * e[i];
-          e[i + 1] = (r = std::hypot(f, g));
-          if (r == 0.0) {
-            d[i + 1] -= p;
-            e[m] = 0.0;
-            break;
-          }
-          s = f / r;
-          c = g / r;
-          g = d[i + 1] - p;
-          r = (d[i] - g) * s + 2.0 * c * bb;
-          d[i + 1] = g + (p = s * r);
-          g = c * r - bb;
-          for (int k = 0; k < n; ++k) {
-            f = at(k, i + 1);
-            at(k, i + 1) = s * at(k, i) + c * f;
-            at(k, i) = c * at(k, i) - s * f;
-          }
-        }
-        if (r == 0.0 && i >= l) continue;
-        d[l] -= p;
-        e[l] = g;
-        e[m] = 0.0;
-      }
-    } while (m != l);
+    }
   }
-  // sort ascending
-  std::vector<int> idx(n);
+  // ascending order
+  std::vector<int> idx(N);
   std::iota(idx.begin(), idx.end(), 0);
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
   for (int j = 0; j < n; ++j) {
     w[j] = d[idx[j]];
-    for (int i = 0; i < n; ++i) Z[size_t(i) * n + j] = at(i, idx[j]);
+    for (int i = 0; i < n; ++i) Z[size_t(i) * n + j] = qrow(idx[j])[i];
   }
   return 0;
 }
@@ -660,9 +657,12 @@ void oracle_rotate(size_t n, int b, int m, const double* V, const double* Y, int
 // [3] error (0 ok, 1 bad params, 2 Cholesky breakdown, 3 eigensolver failure).
 // evals_out: nev Ritz values in the chosen order; X_out (optional): n x nev
 // row-major Ritz vectors.
-int oracle_lanczos_solve(size_t n, const int32_t* rowptr, const int32_t* col, const double* val, const double* dscale, const double* mdiag, int b,
-                         unsigned nev, unsigned ncv, unsigned max_restarts, double tol, int order, const double* V0, double* evals_out, double* X_out,
-                         int* info)
+// seconds (may be null): wall time of solve() alone, measured around the
+// call as the reference's example does (examples/openmp/csrtest.cc:100-103),
+// without the construction and zero fill of the basis.
+int oracle_lanczos_solve_timed(size_t n, const int32_t* rowptr, const int32_t* col, const double* val, const double* dscale, const double* mdiag, int b,
+                               unsigned nev, unsigned ncv, unsigned max_restarts, double tol, int order, const double* V0, double* evals_out,
+                               double* X_out, int* info, double* seconds)
 {
   oracle::CsrOp A{n, rowptr, col, val, dscale, mdiag};
   oracle::Evp evp(A, b);
@@ -676,7 +676,9 @@ int oracle_lanczos_solve(size_t n, const int32_t* rowptr, const int32_t* col, co
     return lz.error;
   }
   std::memcpy(lz.V.data(), V0, sizeof(double) * n * b);
+  const auto t0 = std::chrono::steady_clock::now();
   lz.solve();
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   info[0] = lz.converged ? 1 : 0;
   info[1] = int(lz.iterations);
   info[2] = int(lz.n_op_apply);
@@ -685,6 +687,13 @@ int oracle_lanczos_solve(size_t n, const int32_t* rowptr, const int32_t* col, co
   for (unsigned i = 0; i < nev && i < evp.evals.size(); ++i) evals_out[i] = evp.evals[i];
   if (X_out) lz.ritz_vectors(nev, X_out);
   return 0;
+}
+
+int oracle_lanczos_solve(size_t n, const int32_t* rowptr, const int32_t* col, const double* val, const double* dscale, const double* mdiag, int b,
+                         unsigned nev, unsigned ncv, unsigned max_restarts, double tol, int order, const double* V0, double* evals_out, double* X_out,
+                         int* info)
+{
+  return oracle_lanczos_solve_timed(n, rowptr, col, val, dscale, mdiag, b, nev, ncv, max_restarts, tol, order, V0, evals_out, X_out, info, nullptr);
 }
 
 // extend(0, m) only (tests/test_lanczos_extend.hh): V_out n x (ncv + b)

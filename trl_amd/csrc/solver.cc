@@ -43,19 +43,25 @@ double now()
 
 // True residuals ||K x - lambda M x|| / ||K x|| of nev Ritz vectors stored as
 // ceil(nev/B) panels at X (device), computed with the undivided operator K.
+// scratch: one n x B panel for K X (allocated here when null).
 template <class S, unsigned B>
-void true_residuals(const Context& c, std::size_t n, const SparseOperator<S>& op, const S* mass, const S* X, unsigned nev, const double* evals, double* out)
+void true_residuals(const Context& c, std::size_t n, const SparseOperator<S>& op, const S* mass, const S* X, unsigned nev, const double* evals, double* out,
+                    S* scratch = nullptr)
 {
   constexpr trl_hip_dtype t = trl::hip::dtype_of<S>();
-  trl::hip::DeviceBuffer<S> KX(c, n * B);
+  trl::hip::DeviceBuffer<S> own;
+  if (!scratch) {
+    own = trl::hip::DeviceBuffer<S>(c, n * B);
+    scratch = own.data();
+  }
   const unsigned nblk = (nev + B - 1) / B;
   for (unsigned j = 0; j < nblk; ++j) {
     const S* Xj = X + std::size_t(j) * n * B;
-    op.apply(int(B), Xj, KX.data(), false);
+    op.apply(int(B), Xj, scratch, false);
     double lam[16] = {0.0};
     for (unsigned q = 0; q < B && j * B + q < nev; ++q) lam[q] = evals[j * B + q];
     double sums[32];
-    throw_on(trl_hip_residual_sq(c.get(), t, n, B, Xj, KX.data(), mass, lam, sums), "residual_sq");
+    throw_on(trl_hip_residual_sq(c.get(), t, n, B, Xj, scratch, mass, lam, sums), "residual_sq");
     for (unsigned q = 0; q < B && j * B + q < nev; ++q) out[j * B + q] = std::sqrt(sums[q]) / std::max(std::sqrt(sums[B + q]), 1e-300);
   }
 }
@@ -175,14 +181,15 @@ struct BlockRefiner {
   unsigned nb;
   bool ascending;
   double eta; // second-pass test of the block Gram-Schmidt (<= 0: always two passes)
-  trl::hip::DeviceBuffer<S> Xa, Xb, AX;
+  // KX: the true-residual check's K X panel, kept across sweeps
+  trl::hip::DeviceBuffer<S> Xa, Xb, AX, KX;
   trl::hip::DeviceBuffer<double> coef;
   S* X; // current Ritz panels (Xa or Xb)
   S* spare;
 
   BlockRefiner(const Context& ctx, std::size_t n_, const SparseOperator<S>& o, const S* m, unsigned nb_, bool asc, double eta_)
       : c(ctx), n(n_), op(o), mass(m), nb(nb_), ascending(asc), eta(eta_), Xa(ctx, std::size_t(nb_) * n_ * B),
-        Xb(ctx, std::size_t(nb_) * n_ * B), AX(ctx, std::size_t(nb_) * n_ * B), coef(ctx, std::size_t(nb_) * nb_ * B * B + 2 * B * B)
+        Xb(ctx, std::size_t(nb_) * n_ * B), AX(ctx, std::size_t(nb_) * n_ * B), KX(ctx, n_ * B), coef(ctx, std::size_t(nb_) * nb_ * B * B + 2 * B * B)
   {
     X = Xa.data();
     spare = Xb.data();
@@ -323,11 +330,13 @@ struct ResidualFilter {
   const SparseOperator<float>& op32;
   trl::hip::DeviceBuffer<float> R, Ea, Eb;
   trl::hip::DeviceBuffer<double> AX, table;
-  unsigned table_steps = 0;
+  unsigned max_degree;
 
-  ResidualFilter(const Context& ctx, std::size_t n_, unsigned nb_, const SparseOperator<float>& o)
+  // max_degree: the largest filter degree of any sweep (sizes the device
+  // coefficient table once)
+  ResidualFilter(const Context& ctx, std::size_t n_, unsigned nb_, const SparseOperator<float>& o, unsigned max_degree_)
       : c(ctx), n(n_), nb(nb_), op32(o), R(ctx, std::size_t(nb_) * n_ * B), Ea(ctx, std::size_t(nb_) * n_ * B), Eb(ctx, std::size_t(nb_) * n_ * B),
-        AX(ctx, std::size_t(nb_) * n_ * B), table(ctx, 1)
+        AX(ctx, std::size_t(nb_) * n_ * B), table(ctx, (std::size_t(max_degree_) + 2) * nb_ * B), max_degree(max_degree_)
   {
   }
 
@@ -376,10 +385,7 @@ struct ResidualFilter {
       }
       host[std::size_t(dmax + 1) * nbB + k] = dj == 0 ? 1.0 : pc;
     }
-    if (table_steps < dmax + 2) {
-      table = trl::hip::DeviceBuffer<double>(c, host.size());
-      table_steps = dmax + 2;
-    }
+    if (dmax > max_degree) throw std::invalid_argument("residual filter: degree above the table size");
     throw_on(trl_hip_memcpy_h2d(c.get(), table.data(), host.data(), host.size() * sizeof(double)), "residual filter table");
     const unsigned np1 = nb - j1;
     // r = A x - theta x in FP64 for the filtered panels; R = r, E_1 = alpha_1 r in FP32
@@ -485,7 +491,7 @@ bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, cons
   for (;;) {
     rf.orthonormalize();
     rf.rayleigh_ritz(theta);
-    true_residuals<S, B>(c, n, op, mass, rf.X, nev, theta.data(), rs.data());
+    true_residuals<S, B>(c, n, op, mass, rf.X, nev, theta.data(), rs.data(), rf.KX.data());
     double worst = 0.0;
     for (double r : rs) worst = std::max(worst, r);
     if (std::getenv("TRL_HIP_VERBOSE")) {
@@ -692,7 +698,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
         // ---- FP64 phase: block refinement of the FP32 Ritz blocks -----------
         if (p->spmm_tune && !rfil) tune_panels_of(*op64, B, nref);
         std::unique_ptr<ResidualFilter<B>> rfilt;
-        if (rfil) rfilt = std::make_unique<ResidualFilter<B>>(c, n, nref, *op32);
+        if (rfil) rfilt = std::make_unique<ResidualFilter<B>>(c, n, nref, *op32, mp->cheb_degree);
         BlockRefiner<double, B> rf(c, n, *op64, mass64, nref, asc, eta);
         if (mp->phase1_fp64)
           throw_on(trl_hip_memcpy_d2d(c.get(), rf.X, X1.data(), std::size_t(nref) * n * B * sizeof(double)), "copy X1");
