@@ -32,10 +32,8 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
-# The OpenMP runtime (the CPU baseline's oracle, the host Rayleigh-Ritz
-# threads) reads its placement when it starts: one thread per core.
-os.environ.setdefault("OMP_PLACES", "cores")
-os.environ.setdefault("OMP_PROC_BIND", "close")
+# host CPUs granted to this process, read before any OpenMP runtime binds a thread
+HOST_CPUS = len(os.sched_getaffinity(0))
 
 METRIC = "time to k eigenpairs @ rel. residual 1e-8: mixed-FP32 vs FP64 GPU; SpMM GB/s"
 HBM_PEAK_GBPS = 8000.0
@@ -78,57 +76,76 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(ctx, c3, fp64_cycles, gpu_fp64_s) -> dict:
-    """The OpenMP CPU oracle (the restated reference driver and OpenMP
-    backend, oracle/trl_oracle.cc) on the host cores granted to this process,
-    OMP_PLACES=cores, timed around solve() as the reference's example does
-    (examples/openmp/csrtest.cc:100-103):
-      - one full FP64 thick-restart cycle of the 160^3 workload (extend over
-        32 blocks with the reference's single-pass block MGS, Rayleigh-Ritz,
-        the k_r x m Ritz rotation, the restart arrow and the quasi-Lanczos
-        step): `value`, seconds per cycle;
-      - whole solves of the 2D Laplacian 128^2 (b = 4, 8 smallest, ncv = 256)
-        and the structural pair (b = 6, 20 smallest, ncv = 120), median of 3,
-        with their eigenvalues against the GPU's FP64 solves of the same
-        problems from the same start blocks.
-    The time to solution of the 160^3 workload is derived (cycle time times
-    the GPU FP64 solve's cycle count) and labelled as such."""
+def cpu_oracle_runs() -> dict:
+    """The CPU side of the baseline, run in a child process whose OpenMP
+    runtime starts with OMP_PLACES=cores, OMP_PROC_BIND=close and one thread
+    per granted host CPU (the parent's host threads stay unbound): the OpenMP
+    oracle (the restated reference driver and OpenMP backend,
+    oracle/trl_oracle.cc) timed around solve() as the reference's example
+    does (examples/openmp/csrtest.cc:100-103).  Whole solves of the 2D
+    Laplacian 128^2 and the structural pair (median of 3), and one full FP64
+    thick-restart cycle of the 160^3 workload (extend over 32 blocks with the
+    reference's single-pass block MGS, Rayleigh-Ritz, the k_r x m Ritz
+    rotation, the restart arrow and the quasi-Lanczos step)."""
     import oracle
     from trl_amd import problems as P
-    from trl_amd.solver import DeviceCSR, EigensolverParams, solve
 
-    cores = len(os.sched_getaffinity(0))
-    oracle.set_threads(cores)
-    out = {"cores": cores, "cpu_model": cpu_model(), "omp_places": os.environ.get("OMP_PLACES"), "omp_threads": oracle.num_threads(), "kind": "port"}
-    # whole solves of the two small configurations, CPU (median of 3) and GPU
+    oracle.set_threads(HOST_CPUS)
+    out = {"omp_threads": oracle.num_threads()}
     for name, cfg in (("laplace2d_128", P.config_c1()), ("stiffness_mass_15360", P.config_c2())):
         V0 = P.initial_block(cfg.csr.n, cfg.b)
         runs, res = [], None
         for _ in range(3):
             res = oracle.lanczos_solve(cfg.csr, cfg.b, cfg.nev, cfg.ncv, V0, tol=cfg.tol, order=1, mass=cfg.mass, ritz_vectors=False)
             runs.append(res["seconds"])
+        out[name] = {"runs": runs, "restarts": res["iterations"], "converged": res["converged"], "evals": res["evals"].tolist()}
+        print(f"[cpu] {name}: {runs}", file=sys.stderr, flush=True)
+    c3 = P.config_c3(160)
+    r = oracle.lanczos_solve(c3.csr, c3.b, c3.nev, c3.ncv, P.initial_block(c3.csr.n, c3.b), tol=c3.tol, order=1, max_restarts=1, ritz_vectors=False)
+    out["laplace3d_160_cycle_s"] = r["seconds"]
+    print(f"[cpu] laplace3d_160 cycle: {r['seconds']:.2f} s", file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_baseline(ctx, c3, fp64_cycles, gpu_fp64_s) -> dict:
+    """The CPU baseline (cpu_oracle_runs in a child process) beside the GPU's
+    FP64 solves of the same small problems from the same start blocks.
+    `value` is the measured seconds of one full FP64 cycle of the 160^3
+    workload; its time to solution is derived (cycle time times the GPU FP64
+    solve's cycle count) and labelled as such."""
+    import subprocess
+
+    from trl_amd import problems as P
+    from trl_amd.solver import DeviceCSR, EigensolverParams, solve
+
+    env = dict(os.environ, OMP_PLACES="cores", OMP_PROC_BIND="close", OMP_NUM_THREADS=str(HOST_CPUS))
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--cpu-oracle-runs"], env=env, stdout=subprocess.PIPE, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU oracle runs failed with status {r.returncode}")
+    cpu = json.loads(r.stdout.strip().splitlines()[-1])
+    out = {"cores": HOST_CPUS, "cpu_model": cpu_model(), "omp_places": "cores", "omp_proc_bind": "close", "omp_threads": cpu["omp_threads"],
+           "kind": "port"}
+    for name, cfg in (("laplace2d_128", P.config_c1()), ("stiffness_mass_15360", P.config_c2())):
+        V0 = P.initial_block(cfg.csr.n, cfg.b)
         dA = DeviceCSR(ctx, cfg.csr, mass=cfg.mass, precisions=("f64",))
         prm = EigensolverParams(cfg.nev, cfg.ncv, tolerance=cfg.tol)
-        solve(dA, cfg.b, prm, ctx.to_device(V0), order="ascending", reorth="mgs")  # first call of the process: untimed
+        solve(dA, cfg.b, prm, ctx.to_device(V0), order="ascending", reorth="mgs")  # untimed first call
         ctx.sync()
         t = time.perf_counter()
         g = solve(dA, cfg.b, prm, ctx.to_device(V0), order="ascending", reorth="mgs")
         ctx.sync()
         tg = time.perf_counter() - t
-        out[name] = {"cpu_seconds_median": float(np.median(runs)), "cpu_seconds_runs": runs, "cpu_restarts": res["iterations"],
-                     "cpu_converged": res["converged"], "gpu_fp64_seconds": tg, "gpu_restarts": g.result["iterations"],
-                     "max_rel_eig_diff_cpu_vs_gpu": float(np.max(np.abs(res["evals"] - g.evals) / np.abs(g.evals)))}
-        log(f"cpu baseline {name}: {out[name]}")
-    # one full FP64 cycle of the 160^3 workload
-    V0 = P.initial_block(c3.csr.n, c3.b)
-    r = oracle.lanczos_solve(c3.csr, c3.b, c3.nev, c3.ncv, V0, tol=c3.tol, order=1, max_restarts=1, ritz_vectors=False)
-    t_cycle = r["seconds"]
+        c = cpu[name]
+        out[name] = {"cpu_seconds_median": float(np.median(c["runs"])), "cpu_seconds_runs": c["runs"], "cpu_restarts": c["restarts"],
+                     "cpu_converged": c["converged"], "gpu_fp64_seconds": tg, "gpu_restarts": g.result["iterations"],
+                     "max_rel_eig_diff_cpu_vs_gpu": float(np.max(np.abs(np.array(c["evals"]) - g.evals) / np.abs(g.evals)))}
+    t_cycle = cpu["laplace3d_160_cycle_s"]
     out.update({
         "value": t_cycle,
         "unit": "s per full FP64 thick-restart cycle (measured)",
         "sample": f"one full FP64 restart cycle of the {c3.name} workload (n={c3.csr.n}, b={c3.b}, ncv={c3.ncv}: extend over {c3.ncv // c3.b} "
-                  f"blocks, Rayleigh-Ritz, rotation, restart step), OpenMP oracle on {cores} host threads; the GPU's plain FP64 solve takes "
-                  f"{gpu_fp64_s / max(fp64_cycles, 1):.3f} s per cycle",
+                  f"blocks, Rayleigh-Ritz, rotation, restart step), OpenMP oracle on {cpu['omp_threads']} host threads; the GPU's plain FP64 "
+                  f"solve takes {gpu_fp64_s / max(fp64_cycles, 1):.3f} s per cycle",
         "gpu_fp64_seconds_per_cycle": gpu_fp64_s / max(fp64_cycles, 1),
         "derived_time_to_solution_s": t_cycle * fp64_cycles,
         "derived_note": f"derived, not measured: cycle time x the {fp64_cycles} cycles of the GPU FP64 solve",
@@ -271,11 +288,15 @@ def main(argv=None) -> int:
     ap.add_argument("--ncv-fp32", type=int, default=S["ncv_fp32"], help="Krylov space of the FP32 Lanczos phase (the FP64 solves keep ncv)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-oracle-runs", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-tune", action="store_true", help="untuned CSR operator kernel instead of the autotuned one")
     ap.add_argument("--reorth", default=S["reorth"], choices=["mgs", "cgs2", "dgks"], help="reorthogonalisation of the mixed and FP64 solves")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-problem workload")
     ap.add_argument("--no-batch-sequential", action="store_true", help="skip the one-by-one comparison of the batched workload")
     a = ap.parse_args(argv)
+    if a.cpu_oracle_runs:  # the child process of the CPU baseline: no GPU use
+        print(json.dumps(cpu_oracle_runs()), flush=True)
+        return 0
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported (see DESIGN.md for how the path would shard)"}))
         return 2
