@@ -383,6 +383,19 @@ def main(argv=None) -> int:
         fp64["same_schedule_phase1_cycles"] = os_.result32["iterations"]
         fp64["same_schedule_sweeps"] = os_.result["iterations"]
         log(f"fp64 same schedule {ts:.2f} s")
+        # the warm start of FP64 thick-restart cycles from the FP32 solve's
+        # restart state (refine = 2), measured beside the shipped schedule
+        ctx.sync()
+        t = time.perf_counter()
+        ow = solve_mixed(dA, cfg.b, params, V0, refine=2, tol_fp32=3e-5, ncv_fp32=192, reorth=a.reorth, spmm_tune=tune)
+        ctx.sync()
+        tw = time.perf_counter() - t
+        fp64["thick_restart_warm_start"] = {
+            "seconds": tw, "fp32_cycles": ow.result32["iterations"], "fp32_seconds": ow.result32["seconds"], "fp64_cycles": ow.result["iterations"],
+            "converged_ritz_estimate": ow.result["converged"], "max_rel_eig_diff_vs_fp64": float(np.max(np.abs(ow.evals - o64.evals) / np.abs(o64.evals))),
+            "max_true_residual": float(ow.resid.max()), "meets_tol": bool(ow.resid.max() <= cfg.tol),
+            "schedule": "FP32 thick-restart block Lanczos to a Ritz estimate of 3e-5 (ncv 192), its restart state carried to FP64, FP64 cycles (ncv 256)"}
+        log(f"thick-restart warm start {tw:.2f} s: {fp64['thick_restart_warm_start']}")
 
     # ---- roofline of the dominant kernel ------------------------------------------------
     busy = {k: v for k, v in fam.items() if v["calls"] and v["bytes"] > 0}
@@ -451,8 +464,10 @@ def main(argv=None) -> int:
         "fp32_phase": last.result32,
         "fp64_phase": last.result,
         "fp64_solve": fp64,
-        "speedup_vs_fp64": (fp64["seconds"] / t_mixed) if fp64 else None,
+        # the precision gain alone: the same two-phase schedule run in FP64
         "speedup_vs_fp64_same_schedule": (fp64["same_schedule_seconds"] / t_mixed) if fp64 else None,
+        # algorithm (filtered subspace iteration after one Lanczos cycle) and precision together
+        "speedup_vs_fp64": (fp64["seconds"] / t_mixed) if fp64 else None,
         "spmm": {"f32": sp32, "f64": sp64},
         "roofline": roof,
         "kernel_families": families,

@@ -83,76 +83,26 @@ public:
   EigensolverResult solve()
   {
     EigensolverResult res{false, 0, 0};
-    const unsigned int m = ncv / blocksize;
+    return run(0, res);
+  }
+
+  // Continues from a thick-restart state set up by the caller, as solve()
+  // leaves it after a restart (lanczos.hh:97-143): V[0:kr] the Ritz blocks,
+  // V[kr] the next Lanczos block, and T = diag(theta) on the blocks (i, i),
+  // i < kr, plus the arrow T(kr, i) = T(i, kr)^T (kr = nev/b + 1, all other
+  // blocks of T[0:kr+1, 0:kr+1] zero).  Runs the quasi-Lanczos step from
+  // block kr (lanczos.hh:150-178), then the solve loop.  With it a solve can
+  // be warm-started from the restart state of another run (for instance the
+  // same problem solved in lower precision).
+  EigensolverResult solve_restarted()
+  {
+    EigensolverResult res{false, 0, 0};
     const unsigned int kr = nev / blocksize + 1;
-    unsigned int k = 0;
-    auto beta = B.block_view(0, 0);
-    auto Z0 = B.block_view(0, 1);
-    auto W0 = W.block_view(0);
-
-    while (res.iterations < max_restarts) {
-      ++res.iterations;
-      res.n_op_apply += extend(k, m);
-      restarted_ = false;
-
-      last_nconv = static_cast<unsigned int>(evp->solve_small_dense(T, beta, nev));
-      if (last_nconv >= nev) {
-        res.converged = true;
-        return res;
-      }
-      if (kr >= m) break; // no room to restart
-
-      // Ritz rotation: W_j = sum_i V_i Y(i, j), j < kr; W_kr = V_m; swap
-      const auto& Y = evp->get_current_eigenvectors();
-      if constexpr (requires { evp->rotate(V, m, Y, kr, W); }) {
-        evp->rotate(V, m, Y, kr, W);
-      }
-      else {
-        for (unsigned int j = 0; j < kr; ++j) {
-          auto Wj = W.block_view(j);
-          Wj.set_zero();
-          for (unsigned int i = 0; i < m; ++i) V.block_view(i).mult_add(Y.block_view(i, j), Wj);
-        }
-      }
-      W.block_view(kr).copy_from(V.block_view(m));
-      std::swap(V, W);
-      W0 = W.block_view(0);
-      restarted_ = true;
-
-      // T <- [diag(theta) arrow; arrow^T ...]
-      for (unsigned int i = 0; i < kr; ++i)
-        for (unsigned int j = 0; j < kr; ++j) {
-          auto Tij = T.block_view(i, j);
-          Tij.set_zero();
-          if (i == j) Tij.set_diagonal(evp->get_eigenvalues_block(i));
-        }
-      for (unsigned int i = 0; i < kr; ++i) {
-        auto Tki = T.block_view(kr, i);
-        beta.mult(Y.block_view(m - 1, i), Tki);
-        T.block_view(i, kr).copy_from_transpose(Tki);
-      }
-
-      // one quasi-Lanczos step from block kr restores the block recurrence
-      k = kr;
-      auto Vk = V.block_view(k);
-      auto Vk1 = V.block_view(k + 1);
-      evp->apply(Vk, Vk1);
-      ++res.n_op_apply;
-      auto Tkk = T.block_view(k, k);
-      evp->dot(Vk, Vk1, Tkk);
-      sub_product(Vk1, Vk, Tkk, W0);
-      if (k > 0) sub_product_transpose(Vk1, V.block_view(k - 1), T.block_view(k, k - 1), W0);
-      reorth_(*evp, V, k + 1, Vk1, Z0);
-      evp->orthonormalize(Vk1, beta);
-      if (k + 1 < T.block_rows()) {
-        T.block_view(k + 1, k).copy_from(beta);
-        T.block_view(k, k + 1).copy_from_transpose(beta);
-        k += 1;
-      }
-      else
-        break;
-    }
-    return res;
+    if (kr >= ncv / blocksize) throw std::invalid_argument("solve_restarted: no room for a thick restart");
+    unsigned int k = kr;
+    restarted_ = true;
+    if (!quasi_step(k, res)) return res;
+    return run(k, res);
   }
 
   // Grow a block-k Lanczos factorisation to block m; returns the number of
@@ -223,6 +173,89 @@ public:
   unsigned int get_nconv() const { return last_nconv; }
 
 private:
+  // The solve loop of lanczos.hh:65-199 from block k: extend to m,
+  // Rayleigh-Ritz, and while not converged the thick restart (Ritz rotation,
+  // arrow rebuild, quasi-Lanczos step).
+  EigensolverResult run(unsigned int k, EigensolverResult res)
+  {
+    const unsigned int m = ncv / blocksize;
+    const unsigned int kr = nev / blocksize + 1;
+    auto beta = B.block_view(0, 0);
+
+    while (res.iterations < max_restarts) {
+      ++res.iterations;
+      res.n_op_apply += extend(k, m);
+      restarted_ = false;
+
+      last_nconv = static_cast<unsigned int>(evp->solve_small_dense(T, beta, nev));
+      if (last_nconv >= nev) {
+        res.converged = true;
+        return res;
+      }
+      if (kr >= m) break; // no room to restart
+
+      // Ritz rotation: W_j = sum_i V_i Y(i, j), j < kr; W_kr = V_m; swap
+      const auto& Y = evp->get_current_eigenvectors();
+      if constexpr (requires { evp->rotate(V, m, Y, kr, W); }) {
+        evp->rotate(V, m, Y, kr, W);
+      }
+      else {
+        for (unsigned int j = 0; j < kr; ++j) {
+          auto Wj = W.block_view(j);
+          Wj.set_zero();
+          for (unsigned int i = 0; i < m; ++i) V.block_view(i).mult_add(Y.block_view(i, j), Wj);
+        }
+      }
+      W.block_view(kr).copy_from(V.block_view(m));
+      std::swap(V, W);
+      restarted_ = true;
+
+      // T <- [diag(theta) arrow; arrow^T ...]
+      for (unsigned int i = 0; i < kr; ++i)
+        for (unsigned int j = 0; j < kr; ++j) {
+          auto Tij = T.block_view(i, j);
+          Tij.set_zero();
+          if (i == j) Tij.set_diagonal(evp->get_eigenvalues_block(i));
+        }
+      for (unsigned int i = 0; i < kr; ++i) {
+        auto Tki = T.block_view(kr, i);
+        beta.mult(Y.block_view(m - 1, i), Tki);
+        T.block_view(i, kr).copy_from_transpose(Tki);
+      }
+
+      k = kr;
+      if (!quasi_step(k, res)) break;
+    }
+    return res;
+  }
+
+  // One quasi-Lanczos step from block k after a restart (lanczos.hh:150-178):
+  // V_{k+1} from A V_k against the arrow and all earlier blocks; k becomes
+  // k + 1.  False when T has no room for the step's block.
+  bool quasi_step(unsigned int& k, EigensolverResult& res)
+  {
+    auto beta = B.block_view(0, 0);
+    auto Z0 = B.block_view(0, 1);
+    auto W0 = W.block_view(0);
+    auto Vk = V.block_view(k);
+    auto Vk1 = V.block_view(k + 1);
+    evp->apply(Vk, Vk1);
+    ++res.n_op_apply;
+    auto Tkk = T.block_view(k, k);
+    evp->dot(Vk, Vk1, Tkk);
+    sub_product(Vk1, Vk, Tkk, W0);
+    if (k > 0) sub_product_transpose(Vk1, V.block_view(k - 1), T.block_view(k, k - 1), W0);
+    reorth_(*evp, V, k + 1, Vk1, Z0);
+    evp->orthonormalize(Vk1, beta);
+    if (k + 1 < T.block_rows()) {
+      T.block_view(k + 1, k).copy_from(beta);
+      T.block_view(k, k + 1).copy_from_transpose(beta);
+      k += 1;
+      return true;
+    }
+    return false;
+  }
+
   // Y -= X B as mult into scratch + -= (lanczos.hh:162-170, 259-265), or in
   // one pass when the view offers subtract_product (every backend of the
   // concepts does, blockview.hh:189-201)

@@ -64,7 +64,9 @@ typedef struct {
   double tol_fp32;           /* stopping tolerance of the FP32 phase                  */
   unsigned max_restarts_fp32;
   unsigned long long seed;   /* refine = 0: mixing of the FP32 Ritz vectors           */
-  int refine;                /* 0: FP64 Lanczos warm start, 1: Chebyshev + Rayleigh-Ritz */
+  int refine;                /* 0: FP64 Lanczos from a mix of the FP32 Ritz vectors,
+                                1: Chebyshev + Rayleigh-Ritz sweeps, 2: FP64
+                                thick-restart cycles from the FP32 restart state */
   unsigned cheb_degree;      /* refine = 1: filter degree per refinement sweep        */
   unsigned max_refine;       /* refine = 1: sweeps before falling back to refine = 0  */
   int phase1_fp64;           /* 1: run phase 1 in FP64 too (the same schedule without
@@ -104,7 +106,15 @@ typedef struct {
  *      of the error of x, so FP32 rounding in E stays below the FP64
  *      tolerance; x <- p(theta) x + E is formed in FP64;
  *   0: an FP64 thick-restart solve started from V0 = X G, G a seeded
- *      Gaussian (nev/b+1)*b x b mixing matrix.
+ *      Gaussian (nev/b+1)*b x b mixing matrix;
+ *   2: FP64 thick-restart cycles continuing the FP32 solve's restart state
+ *      (lanczos.hh:97-143): its k_r = nev/b + 1 Ritz blocks and the block
+ *      after them, converted to FP64 and orthonormalised, their FP64
+ *      Rayleigh quotient diagonalised into theta and the arrow, then the
+ *      quasi-Lanczos step and the driver's cycles.  The FP64 cycles stop on
+ *      the Ritz estimate; the part of A V outside the seeded blocks (the
+ *      FP32 basis' rounding) stays in the true residuals, which resid
+ *      reports.
  * Order must be ascending or descending for refine = 1.
  * val32/mass32/minv32 are FP32 copies of the FP64 arrays.  evals, X, resid
  * refer to the final FP64 result; res->seconds is the wall time of both

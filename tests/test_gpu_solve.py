@@ -281,3 +281,25 @@ def test_mixed_generalized(ctx):
     assert om.result["converged"] == 1
     assert np.all(np.abs(om.evals - ref) <= 1e-10 * ref)
     assert om.resid.max() <= 1e-8
+
+
+@pytest.mark.parametrize("reorth", ["mgs", "dgks"])
+def test_mixed_thick_restart_warm_start(ctx, reorth):
+    # refine = 2: the FP32 solve's thick-restart state (Ritz blocks, the next
+    # block, diag(theta) + arrow) carried to FP64 and continued by the FP64
+    # driver's cycles (BlockLanczos::solve_restarted).  The eigenvalues match
+    # the plain FP64 solve; the true residuals keep the FP32 basis' rounding
+    # (about 1e-5 here), which the FP64 cycles do not see.
+    c = P.config_c3(24)
+    dA = DeviceCSR(ctx, c.csr)
+    V0 = v0(ctx, c.csr.n, c.b)
+    prm = EigensolverParams(c.nev, c.ncv, max_restarts=500)
+    o64 = solve(dA, c.b, prm, V0, reorth=reorth)
+    for tol32 in (1e-3, 3e-5):
+        om = solve_mixed(dA, c.b, prm, V0, refine=2, tol_fp32=tol32, reorth=reorth)
+        rel = np.abs(om.evals - o64.evals) / np.abs(o64.evals)
+        print(f"warm start {reorth} tol32={tol32}: fp32 {om.result32['iterations']} cycles, fp64 {om.result['iterations']} cycles, "
+              f"max rel eig {rel.max():.2e}, true residual {om.resid.max():.2e}")
+        assert om.result["converged"] == 1 and om.result32["converged"] == 1
+        assert rel.max() <= 1e-9
+        assert om.resid.max() <= 1e-4

@@ -531,6 +531,157 @@ bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, cons
 }
 
 
+// FP64 thick-restart Lanczos warm-started from the restart state of an FP32
+// one (mixed mode refine = 2).  The FP32 solve (its basis, SpMM and Gram
+// products in FP32, every coefficient in FP64) runs to tol_fp32; its thick
+// restart state -- the k_r = nev/b + 1 Ritz blocks, the Lanczos block that
+// follows them and T = diag(theta) plus the arrow (lanczos.hh:97-143) -- is
+// carried to FP64: the k_r + 1 blocks are converted and orthonormalised in
+// FP64 (block CGS2 + CholQR2), their Rayleigh quotient H = V^T A V is formed
+// in FP64, the Ritz part H[0:k_r, 0:k_r] is diagonalised (V[0:k_r] rotated by
+// its eigenvectors Z, theta its eigenvalues) and the arrow is
+// H[k_r, 0:k_r] Z.  The FP64 driver then continues with the quasi-Lanczos
+// step from block k_r and its usual cycles (BlockLanczos::solve_restarted).
+// What the FP64 cycles cannot see is the part of A V[0:k_r] outside the
+// span of the k_r + 1 seeded blocks, the FP32 basis' rounding (about
+// eps_32 ||A||): it stays in the true residuals of the final Ritz vectors.
+template <unsigned B, class Reorth>
+void warm_restart_solve(const Context& c, std::size_t n, const int32_t* rowptr, const int32_t* col, const float* val32, const float* mass32,
+                        const float* minv32, const std::shared_ptr<SparseOperator<float>>& op32, const double* val64, const double* mass64,
+                        const double* minv64, const std::shared_ptr<SparseOperator<double>>& op64, const trl_hip_solver_params& p,
+                        const trl_hip_solver_params& p32, const float* V0f, double* evals, double* X, double* resid, trl_hip_solver_result& res,
+                        trl_hip_solver_result& r32)
+{
+  using E32 = trl::hip::CSREVP<float, B>;
+  using E64 = trl::hip::CSREVP<double, B>;
+  const unsigned kr = p.nev / B + 1;
+  const unsigned nblk = (p.nev + B - 1) / B;
+  const std::size_t P = n * B;
+  const trl_hip_dtype f64 = TRL_HIP_F64;
+  // ---- FP32 thick-restart solve ---------------------------------------------
+  auto e32 = std::make_shared<E32>(c, n, rowptr, col, val32, mass32, minv32);
+  e32->set_order(static_cast<trl::hip::EigenvalueOrder>(p.order));
+  e32->set_operator(op32);
+  e32->set_cholqr_passes(2);
+  trl_amd::BlockLanczos<E32, Reorth> lz32(e32, trl_amd::EigensolverParams{p32.nev, p32.ncv, p32.max_restarts, p32.tol});
+  throw_on(trl_hip_memcpy_d2d(c.get(), lz32.initial_block().data_, V0f, P * sizeof(float)), "copy V0");
+  c.sync();
+  const double t32 = now();
+  const auto q32 = lz32.solve();
+  c.sync();
+  r32.seconds = now() - t32;
+  r32.converged = q32.converged ? 1 : 0;
+  r32.iterations = q32.iterations;
+  r32.n_op_apply = q32.n_op_apply;
+  r32.host_seconds = e32->host_seconds();
+  const unsigned m32 = p32.ncv / B;
+  // ---- the FP64 driver and its seeded state ----------------------------------
+  auto e64 = std::make_shared<E64>(c, n, rowptr, col, val64, mass64, minv64);
+  e64->set_order(static_cast<trl::hip::EigenvalueOrder>(p.order));
+  e64->set_operator(op64);
+  if (p.cholqr_passes > 0) e64->set_cholqr_passes(p.cholqr_passes);
+  trl_amd::BlockLanczos<E64, Reorth> lz64(e64, trl_amd::EigensolverParams{p.nev, p.ncv, p.max_restarts, p.tol});
+  auto& V64 = lz64.get_basis();
+  auto& V32 = lz32.get_basis();
+  trl::hip::DeviceBuffer<float> S32(c, std::size_t(kr) * P);
+  std::vector<double> theta32(e32->get_current_eigenvalues().begin(), e32->get_current_eigenvalues().begin() + std::size_t(kr) * B);
+  if (lz32.basis_holds_ritz_blocks()) {
+    // stopped right after a restart: V[0:kr] are the Ritz blocks, V[kr] the block after them
+    throw_on(trl_hip_convert(c.get(), TRL_HIP_F32, V32.data(), f64, V64.data(), std::size_t(kr + 1) * P), "convert restart blocks");
+  }
+  else {
+    // stopped after a Rayleigh-Ritz step: rotate to the Ritz blocks; the next block is V_m
+    const auto& Y = e32->get_current_eigenvectors();
+    throw_on(trl_hip_rotate(c.get(), TRL_HIP_F32, n, B, int(m32), V32.data(), Y.data(), int(Y.block_cols()), int(kr), S32.data()), "Ritz blocks");
+    throw_on(trl_hip_convert(c.get(), TRL_HIP_F32, S32.data(), f64, V64.data(), std::size_t(kr) * P), "convert Ritz blocks");
+    throw_on(trl_hip_convert(c.get(), TRL_HIP_F32, V32.data() + std::size_t(m32) * P, f64, V64.data() + std::size_t(kr) * P, P), "convert V_m");
+  }
+  // FP64 orthonormalisation of the kr + 1 blocks
+  trl::hip::DeviceBuffer<double> coef(c, std::size_t(2 * kr + 3) * B * B), R(c, B * B);
+  for (unsigned j = 0; j <= kr; ++j) {
+    double* Vj = V64.data() + std::size_t(j) * P;
+    if (j > 0) throw_on(trl_hip_reorth(c.get(), f64, n, B, int(j), V64.data(), Vj, mass64, coef.data(), 0.0), "seed reorth");
+    throw_on(trl_hip_cholqr(c.get(), f64, n, B, Vj, mass64, R.data(), 2), "seed cholqr");
+  }
+  // H = V^T M (M^{-1} K) V over the kr + 1 blocks, in FP64
+  const unsigned ns = kr + 1, nt = ns * B;
+  trl::hip::DeviceBuffer<double> AV(c, std::size_t(ns) * P), Hd(c, std::size_t(nt) * nt);
+  for (unsigned j = 0; j < ns; ++j) {
+    op64->apply(int(B), V64.data() + std::size_t(j) * P, AV.data() + std::size_t(j) * P);
+    throw_on(trl_hip_gram(c.get(), f64, n, B, int(ns), V64.data(), AV.data() + std::size_t(j) * P, mass64, Hd.data() + std::size_t(j) * nt * B),
+             "seed projection");
+  }
+  res.n_op_apply = ns;
+  std::vector<double> Hc(std::size_t(nt) * nt), H(std::size_t(nt) * nt);
+  throw_on(trl_hip_memcpy_d2h(c.get(), Hc.data(), Hd.data(), Hc.size() * sizeof(double)), "seed H");
+  // Hd + j*nt*B holds column block j: row r (block r/B, column r%B), column q of block j
+  for (unsigned j = 0; j < ns; ++j)
+    for (unsigned r = 0; r < nt; ++r)
+      for (unsigned q = 0; q < B; ++q) H[std::size_t(r) * nt + j * B + q] = Hc[std::size_t(j) * nt * B + std::size_t(r) * B + q];
+  for (unsigned r = 0; r < nt; ++r)
+    for (unsigned q = r + 1; q < nt; ++q) H[std::size_t(r) * nt + q] = H[std::size_t(q) * nt + r] = 0.5 * (H[std::size_t(r) * nt + q] + H[std::size_t(q) * nt + r]);
+  // Rayleigh-Ritz of the Ritz part
+  const unsigned nr = kr * B;
+  std::vector<double> Hr(std::size_t(nr) * nr), w(nr), Z(std::size_t(nr) * nr);
+  for (unsigned r = 0; r < nr; ++r)
+    for (unsigned q = 0; q < nr; ++q) Hr[std::size_t(r) * nr + q] = H[std::size_t(r) * nt + q];
+  if (trl_amd::dense::syev(int(nr), Hr.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
+  std::vector<unsigned> idx(nr);
+  for (unsigned i = 0; i < nr; ++i) idx[i] = p.order == 2 ? nr - 1 - i : i;
+  if (p.order == 0)
+    std::stable_sort(idx.begin(), idx.end(), [&](unsigned a, unsigned b_) { return std::abs(w[a]) > std::abs(w[b_]); });
+  // V[0:kr] <- V[0:kr] Z (Z in block layout, kr x kr blocks)
+  std::vector<double> Zb(std::size_t(nr) * nr);
+  for (unsigned bi = 0; bi < kr; ++bi)
+    for (unsigned bj = 0; bj < kr; ++bj)
+      for (unsigned ci = 0; ci < B; ++ci)
+        for (unsigned cj = 0; cj < B; ++cj) Zb[((std::size_t(bi) * kr + bj) * B + ci) * B + cj] = Z[std::size_t(bi * B + ci) * nr + idx[bj * B + cj]];
+  trl::hip::DeviceBuffer<double> Zd(c, Zb.size());
+  throw_on(trl_hip_memcpy_h2d(c.get(), Zd.data(), Zb.data(), Zb.size() * sizeof(double)), "seed Z");
+  throw_on(trl_hip_rotate(c.get(), f64, n, B, int(kr), V64.data(), Zd.data(), int(kr), int(kr), AV.data()), "seed rotation");
+  throw_on(trl_hip_memcpy_d2d(c.get(), V64.data(), AV.data(), std::size_t(kr) * P * sizeof(double)), "seed blocks");
+  // T[0:kr+1, 0:kr+1] = diag(theta) + arrow (H[kr, 0:kr] Z) and its transpose
+  auto& T = lz64.get_T();
+  std::vector<double> Th = T.to_host();
+  const std::size_t tbc = T.block_cols();
+  auto tblk = [&](unsigned i, unsigned j) { return Th.data() + (std::size_t(i) * tbc + j) * B * B; };
+  for (unsigned i = 0; i <= kr; ++i)
+    for (unsigned j = 0; j <= kr; ++j) std::fill(tblk(i, j), tblk(i, j) + B * B, 0.0);
+  for (unsigned i = 0; i < kr; ++i)
+    for (unsigned q = 0; q < B; ++q) tblk(i, i)[q * B + q] = w[idx[i * B + q]];
+  for (unsigned r = 0; r < B; ++r)
+    for (unsigned col2 = 0; col2 < nr; ++col2) {
+      double a = 0.0;
+      for (unsigned t = 0; t < nr; ++t) a += H[std::size_t(nr + r) * nt + t] * Z[std::size_t(t) * nr + idx[col2]];
+      const unsigned i = col2 / B, q = col2 % B;
+      tblk(kr, i)[r * B + q] = a;
+      tblk(i, kr)[q * B + r] = a;
+    }
+  T.from_host(Th);
+  // ---- FP64 cycles from the seeded restart state ----------------------------
+  c.sync();
+  const double t64 = now();
+  const auto q64 = lz64.solve_restarted();
+  c.sync();
+  res.converged = q64.converged ? 1 : 0;
+  res.iterations = q64.iterations;
+  res.n_op_apply += q64.n_op_apply;
+  res.host_seconds = e64->host_seconds();
+  (void)t64;
+  const auto& ev = e64->get_current_eigenvalues();
+  for (unsigned i = 0; i < p.nev && i < ev.size(); ++i) evals[i] = ev[i];
+  if (X) {
+    if (lz64.basis_holds_ritz_blocks())
+      throw_on(trl_hip_memcpy_d2d(c.get(), X, V64.data(), std::size_t(nblk) * P * sizeof(double)), "ritz vectors");
+    else {
+      const auto& Y = e64->get_current_eigenvectors();
+      throw_on(trl_hip_rotate(c.get(), f64, n, B, int(p.ncv / B), V64.data(), Y.data(), int(Y.block_cols()), int(nblk), X), "ritz vectors");
+    }
+    if (resid) true_residuals<double, B>(c, n, *op64, mass64, X, p.nev, evals, resid, AV.data());
+  }
+  c.sync();
+}
+
 template <class F>
 int dispatch_b(int b, F&& f)
 {
@@ -645,6 +796,26 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       throw_on(trl_hip_convert(c.get(), TRL_HIP_F64, V0, TRL_HIP_F32, V0f.data(), n * B), "convert V0");
       // phase 1's Krylov space: ncv_fp32 columns (0: the solve's ncv)
       const unsigned ncv1 = mp->ncv_fp32 ? mp->ncv_fp32 : p->ncv;
+      if (mp->refine == 2) {
+        // FP64 thick-restart cycles warm-started from the FP32 restart state
+        auto op32w = make_operator<float, B>(c, n, rowptr, col, val32, minv32, *p);
+        trl_hip_solver_params p32 = *p;
+        p32.ncv = ncv1;
+        p32.tol = mp->tol_fp32;
+        p32.max_restarts = mp->max_restarts_fp32;
+        if (p->reorth == 2)
+          warm_restart_solve<B, trl::hip::BlockCGSAdaptive>(c, n, rowptr, col, val32, mass32, minv32, op32w, val64, mass64, minv64, op64, *p, p32, V0f.data(),
+                                                            evals, X, resid, *res, r32);
+        else if (p->reorth == 1)
+          warm_restart_solve<B, trl::hip::BlockCGS2>(c, n, rowptr, col, val32, mass32, minv32, op32w, val64, mass64, minv64, op64, *p, p32, V0f.data(),
+                                                     evals, X, resid, *res, r32);
+        else
+          warm_restart_solve<B, trl_amd::ModifiedGS>(c, n, rowptr, col, val32, mass32, minv32, op32w, val64, mass64, minv64, op64, *p, p32, V0f.data(),
+                                                     evals, X, resid, *res, r32);
+        c.sync();
+        res->seconds = now() - t0;
+        return;
+      }
       const unsigned m = ncv1 / B;
       unsigned nref = mp->refine == 1 ? (mp->refine_blocks ? mp->refine_blocks : kr + 6) : kr;
       if (nref < kr) nref = kr;
