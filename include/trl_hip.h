@@ -188,6 +188,20 @@ typedef struct {
   int variant;
   int plane;
   int plane_block;
+  /* The several-panel ELL step only (trl_hip_spmm_panels and its axpby and
+   * forced forms on ELL rows of at most 8 slots; ignored elsewhere):
+   *   share = 1: a gathered column that is a row of the same wave (the +-1
+   *     neighbours of a stencil row, the diagonal, padding) is taken from the
+   *     lane that loads that row as its own, by a cross-lane permute, instead
+   *     of a load of its own;
+   *   line > 0, tile_lines > 0 (with plane > 0): inside each plane block a
+   *     workgroup visits tile_lines consecutive lines of `line` rows instead
+   *     of consecutive rows, so that the rows gathered +-line away are mostly
+   *     its own; used only where such tiles cover every plane block exactly
+   *     (index order otherwise).  The product is the same. */
+  int share;
+  int line;
+  int tile_lines;
 } trl_hip_spmat;
 
 /* Y = diag(dscale) A X: EVP::apply of CSREVP (examples/openmp/csrevp.hh:36-44;

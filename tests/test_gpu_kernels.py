@@ -726,3 +726,70 @@ def test_spmm_offsets64(ctx, rng, p, fmt, variant):
         assert rel(got[2], 0.5 * AX - X + 0.25 * Z) < TOL[p]
         assert rel(got[3], 0.5 * AX - X + F * g[:, None, :]) < TOL[p]
     assert H().trl_hip_ctx_set_option(ctx.h, 99, 1) == -1
+
+
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b,variant", [(8, 1), (8, -2), (8, -4), (4, 1), (1, 1), (6, 1)])
+def test_spmm_panels_shared_gathers_and_tiles(ctx, rng, p, b, variant):
+    # the several-panel ELL step with in-wave gathers taken from the owning
+    # lane (share) and with workgroups visiting tiles of lines (tile_lines x
+    # line): the plain product bit for bit the same as without them (the same
+    # products in the same order), the axpby and forced epilogues to the
+    # rounding of the compiler's fused multiply-adds, and the oracle's; in
+    # index order, blocked orders the tiles cover, blocked orders they do not
+    # (the step then visits rows in block order) and orders whose segments
+    # end inside waves; a ragged matrix (a stencil with boundary rows and an
+    # empty row)
+    dt = DT[p]
+    if variant < 0 and not split_ok(p, b, -variant):
+        pytest.skip("column split needs 8 bytes per lane")
+    print(f"shared gathers and tiles {p} b={b} variant={variant}", flush=True)
+    nx, ny, nz = 64, 8, 3
+    A = P.laplacian_3d(nx, ny, nz, rng.uniform(0, 1, nx * ny * nz))
+    lo, hi = A.rowptr[100], A.rowptr[101]
+    keep = np.ones(A.nnz, bool)
+    keep[lo:hi] = False
+    rp = A.rowptr.copy()
+    rp[101:] -= hi - lo
+    A = P.CSR(A.n, rp, A.col[keep], A.val[keep])
+    npan = 3
+    X = rng.standard_normal((npan, A.n, b)).astype(dt)
+    Z = rng.standard_normal((npan, A.n, b)).astype(dt)
+    F = rng.standard_normal((npan, A.n, b)).astype(dt)
+    g = rng.standard_normal((npan, b))
+    d = rng.uniform(0.5, 2, A.n).astype(dt)
+    width, hcol, hval = csr_to_ell(A)
+    M = spmat(1, None, width, dev(ctx, hcol, np.int32), dev(ctx, hval, dt), dev(ctx, d, dt), variant)
+    Xd, Fd, gd = dev(ctx, X, dt), dev(ctx, F, dt), dev(ctx, g, np.float64)
+    t = L.dtype_code(dt)
+
+    def run():
+        out = []
+        Y = ctx.empty((npan, A.n, b), dt)
+        L.check(H().trl_hip_spmm_panels(ctx.h, t, A.n, b, C.byref(M), npan, Xd.ptr, Y.ptr), "panels")
+        out.append(Y.download())
+        Zd = dev(ctx, Z, dt)
+        L.check(H().trl_hip_spmm_axpby_panels(ctx.h, t, A.n, b, C.byref(M), npan, Xd.ptr, Zd.ptr, Zd.ptr, 0.5, -1.0, 0.25), "axpby")
+        out.append(Zd.download())
+        L.check(H().trl_hip_spmm_axpby_forced(ctx.h, t, A.n, b, C.byref(M), npan, Xd.ptr, Y.ptr, None, 0.5, -1.0, 0.0, Fd.ptr, gd.ptr), "forced")
+        out.append(Y.download())
+        return out
+
+    AX = np.stack([oracle.spmm(A, X[j].astype(np.float64), d.astype(np.float64)) for j in range(npan)])
+    plane = nx * ny
+    for order in [(0, 0), (plane, 4 * nx), (plane, plane), (plane, 2 * nx), (200, 37)]:
+        M.plane, M.plane_block = order
+        M.share, M.line, M.tile_lines = 0, 0, 0
+        ref = run()
+        for share, tile in [(1, 0), (0, 4), (1, 4), (1, 2)]:
+            M.share, M.line, M.tile_lines = share, (nx if tile else 0), tile
+            got = run()
+            assert np.array_equal(got[0], ref[0]), (order, share, tile)
+            for a, c in zip(got[1:], ref[1:]):
+                assert rel(a, c) <= 4 * np.finfo(dt).eps, (order, share, tile)
+            assert rel(got[0], AX) < TOL[p]
+            assert rel(got[1], 0.5 * AX - X + 0.25 * Z) < TOL[p]
+            assert rel(got[2], 0.5 * AX - X + F * g[:, None, :]) < TOL[p]
+    # invalid option values are refused
+    M.share = 2
+    assert H().trl_hip_spmm_panels(ctx.h, t, A.n, b, C.byref(M), npan, Xd.ptr, Xd.offset(0)) == -1

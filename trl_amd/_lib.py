@@ -31,10 +31,11 @@ class SpMat(C.Structure):
     """trl_hip_spmat: a device-resident sparse operator and its kernel variant.
 
     format 0 = CSR (rowptr, col, val), 1 = slot-major ELL (width, col, val);
-    plane / plane_block: blocked row order (0 = index order)."""
+    plane / plane_block: blocked row order (0 = index order); share, line,
+    tile_lines: options of the several-panel ELL step."""
 
     _fields_ = [("format", c_int), ("rowptr", c_void_p), ("width", c_int), ("col", c_void_p), ("val", c_void_p), ("dscale", c_void_p),
-                ("variant", c_int), ("plane", c_int), ("plane_block", c_int)]
+                ("variant", c_int), ("plane", c_int), ("plane_block", c_int), ("share", c_int), ("line", c_int), ("tile_lines", c_int)]
 
 
 c_spmat_p = C.POINTER(SpMat)

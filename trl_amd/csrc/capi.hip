@@ -222,6 +222,7 @@ int trl_hip_ctx_set_option(trl_hip_ctx* ctx, int option, int value)
   case TRL_HIP_OPT_OFFSETS64:
     ctx->force_offsets64 = value ? 1 : 0;
     return 0;
+
   default:
     return TRL_HIP_EINVAL;
   }
@@ -425,6 +426,7 @@ int check_spmat(size_t n, const trl_hip_spmat* A)
   if (n && (A->format == 0 || A->width > 0) && (!A->col || !A->val)) return TRL_HIP_EINVAL;
   if (n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
   if (A->plane < 0 || (A->plane > 0 && (A->plane_block < 1 || A->plane_block > A->plane))) return TRL_HIP_EINVAL;
+  if ((A->share != 0 && A->share != 1) || A->line < 0 || A->tile_lines < 0) return TRL_HIP_EINVAL;
   return 0;
 }
 
@@ -600,8 +602,8 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
   auto consider = [&](const trl_hip_spmat& A) -> double {
     const double us = time_one(A);
     if (verbose)
-      std::fprintf(stderr, "[tune] %s b=%d %s x%d %s/v%d/plane%d:%d %.1f us\n", t == TRL_HIP_F32 ? "f32" : "f64", b, axpby ? "axpby" : "apply", npanels,
-                   A.format ? "ell" : "csr", A.variant, A.plane, A.plane_block, us);
+      std::fprintf(stderr, "[tune] %s b=%d %s x%d %s/v%d/plane%d:%d share%d tile%dx%d %.1f us\n", t == TRL_HIP_F32 ? "f32" : "f64", b,
+                   axpby ? "axpby" : "apply", npanels, A.format ? "ell" : "csr", A.variant, A.plane, A.plane_block, A.share, A.tile_lines, A.line, us);
     if (us > 0.0 && us < best_t) {
       best_t = us;
       *best = A;
@@ -642,12 +644,36 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
     // panels gathered between two uses of a row two planes apart do not
     // fit in L2 (measured on the 160^3 stencil, 7 panels: 1.14x to 1.36x
     // slower than blocks of 1/8 of a plane for every variant)
-    if (ell)
+    if (ell) {
       for (int v : {-2, -4, 1}) {
         if (v == 1 && !narrow) continue;
         const int pb = block_of(v, 8);
         consider(pb ? with(*ell, v, bw, pb) : with(*ell, v, 0, 0));
       }
+      // the fastest of those with in-wave gathers shared between lanes, and
+      // in tiles of 4 lines (the line length: the largest offset below half
+      // the bandwidth) where the tiles cover the plane blocks exactly
+      if (best_t < 1e300 && best->format == 1 && ell->width <= 8) {
+        int line = 0;
+        if (best->plane > 0) {
+          int* d = ctx->flags + 16;
+          if (hipMemsetAsync(d, 0, sizeof(int), ctx->stream) != hipSuccess || launch_csr_bandwidth_below(ctx, n, csr->rowptr, csr->col, bw / 2, d) ||
+              hipMemcpyAsync(&line, d, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+            line = 0;
+        }
+        const trl_hip_spmat base = *best;
+        for (int sh : {0, 1})
+          for (int ty : {0, 4}) {
+            if (sh == 0 && ty == 0) continue;
+            if (ty && !line) continue;
+            trl_hip_spmat A = base;
+            A.share = sh;
+            A.line = ty ? line : 0;
+            A.tile_lines = ty;
+            consider(A);
+          }
+      }
+    }
     else
       consider(with(*csr, 0, 0, 0));
   }

@@ -96,6 +96,8 @@ int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hi
 // product kernel of this variant
 int spmm_rows_per_block(trl_hip_dtype t, int b, int variant, bool panels = false);
 int launch_csr_bandwidth(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, int* dev_out);
+// max |col - row| over the entries with |col - row| < limit
+int launch_csr_bandwidth_below(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, int limit, int* dev_out);
 int launch_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* dev_out);
 int launch_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ecol,
                       void* eval);

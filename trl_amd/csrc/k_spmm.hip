@@ -71,6 +71,13 @@ struct Epi {
   // out as X, g[p * B + c] the coefficient of column c of panel p
   const void* F;
   const double* g;
+  // tiled order inside the plane blocks (tly > 0): a plane holds lines of
+  // `line` rows, and a workgroup takes tly consecutive lines of
+  // rows_per_wg / tly rows each (the launcher checks that the tiles cover
+  // every plane block exactly)
+  unsigned line = 0, tly = 0;
+  // several-panel ELL step: in-wave gathers from the owning lane
+  unsigned share = 0;
 };
 
 // Row handled by local row slot `slot` (0 .. rows_per_wg-1) of row-range
@@ -85,6 +92,16 @@ __device__ inline int row_of(unsigned wrow, unsigned slot, unsigned rows_per_wg,
     return r < unsigned(n) ? int(r) : -1;
   }
   const unsigned wps = (e.pblock + rows_per_wg - 1) / rows_per_wg;
+  if (e.tly) {
+    const unsigned tx = rows_per_wg / e.tly, xt = e.line / tx;
+    const unsigned seg = wrow / wps, w_in = wrow % wps;
+    const unsigned pb = seg / e.nz, z = seg % e.nz;
+    const unsigned y = (w_in / xt) * e.tly + slot / tx, x = (w_in % xt) * tx + slot % tx;
+    const unsigned p = pb * e.pblock + y * e.line + x;
+    if (p >= e.plane) return -1;
+    const size_t r = size_t(z) * e.plane + p;
+    return r < size_t(n) ? int(r) : -1;
+  }
   const unsigned seg = wrow / wps, in_seg = (wrow % wps) * rows_per_wg + slot;
   const unsigned pb = seg / e.nz, z = seg % e.nz;
   const unsigned p = pb * e.pblock + in_seg;
@@ -320,6 +337,130 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
   }
 }
 
+// A value of another lane of the wave: byte address srcaddr = 4 * source lane.
+template <typename S>
+__device__ inline S lane_value(int srcaddr, S v)
+{
+  if constexpr (sizeof(S) == 4)
+    return __builtin_bit_cast(S, __builtin_amdgcn_ds_bpermute(srcaddr, __builtin_bit_cast(int, v)));
+  else {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_ds_bpermute(srcaddr, int(u & 0xffffffff));
+    const int hi = __builtin_amdgcn_ds_bpermute(srcaddr, int(u >> 32));
+    return __builtin_bit_cast(S, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+  }
+}
+
+// The several-panel ELL step with gathers served inside the wave: the rows
+// of a wave are consecutive (64 / CS of them, lanes in row order), so a
+// column that falls on one of them (the +-1 neighbours of a stencil row, the
+// diagonal, padding) is the X row another lane of the wave loads as its own.
+// Each lane loads its own X row and only the columns outside the wave; the
+// others come from the owning lane by a cross-lane permute (ds_bpermute,
+// no LDS allocation), which takes those gathers off the vector memory path.
+template <typename S, int B, int CS, bool AXPBY, bool O32, bool FORCE>
+__global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panels_sh(int width, const int32_t* __restrict__ col,
+                                                                                   const S* __restrict__ val, const S* __restrict__ X, S* Y, Epi e,
+                                                                                   int n)
+{
+  using I = Idx<O32>;
+  constexpr int BC = B / CS;
+  __shared__ double gsh[FORCE ? kForceCoefs : 1];
+  if constexpr (FORCE) {
+    for (unsigned i = threadIdx.x; i < e.npan * B; i += blockDim.x) gsh[i] = e.g[i];
+    __syncthreads();
+  }
+  const int row = row_of(xcd_block(blockIdx.x, gridDim.x), threadIdx.x / CS, blockDim.x / CS, n, e);
+  const int part = int(threadIdx.x % CS);
+  const int off = part * BC;
+  const bool live = row >= 0;
+  // live lanes form a prefix of the wave (rows ascend with the lane until a
+  // segment or the matrix ends)
+  const unsigned long long liveb = __ballot(live);
+  if (liveb == 0) return;
+  const int nrows = int(__popcll(liveb)) / CS;
+  const int r0w = __shfl(row, 0);
+  const I ro = I(live ? row : r0w) * I(B) + I(off);
+  const S* Z = static_cast<const S*>(e.Z);
+  const S* F = FORCE ? static_cast<const S*>(e.F) : nullptr;
+  int cq[kChunk];
+  S vi[kChunk];
+  unsigned inw = 0;  // bit q: column of slot q is a row of this wave
+  unsigned anyin = 0; // bit q: some lane of the wave takes slot q from a lane
+#pragma unroll
+  for (int q = 0; q < kChunk; ++q) {
+    int c = live ? row : r0w;
+    vi[q] = S(0);
+    if (live && q < width) {
+      const size_t base = size_t(q) * size_t(n);
+      c = ld<true>(elem<O32>(col + base, I(row)));
+      vi[q] = ld<true>(elem<O32>(val + base, I(row)));
+    }
+    cq[q] = c;
+    const bool in = live && unsigned(c - r0w) < unsigned(nrows);
+    if (in) inw |= 1u << q;
+    if (__ballot(in) != 0) anyin |= 1u << q;
+  }
+  const S* ds = static_cast<const S*>(e.dscale);
+  const double arow = (AXPBY && ds && live) ? e.alpha * double(ld<true>(elem<O32>(ds, I(row)))) : e.alpha;
+  for (unsigned p = 0; p < e.npan; ++p) {
+    const size_t po = size_t(p) * e.pstride;
+    const S* Xp = X + po;
+    S zr[BC] = {}, fr[BC] = {}, xo[BC];
+    if constexpr (AXPBY) {
+      if (Z && live) RowIO<S, BC>::template load<true>(elem<O32>(Z + po, ro), zr);
+      if constexpr (FORCE)
+        if (live) RowIO<S, BC>::template load<true>(elem<O32>(F + po, ro), fr);
+    }
+    RowIO<S, BC>::load(elem<O32>(Xp, ro), xo);
+    S x[kChunk][BC];
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      if (live && !(inw >> q & 1u)) {
+        I o = I(cq[q]) * I(B) + I(off);
+        if constexpr (O32) asm volatile("" : "+v"(o));
+        RowIO<S, BC>::load(elem<O32>(Xp, o), x[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      if (anyin >> q & 1u) { // wave-uniform
+        const int src = ((cq[q] - r0w) * CS + part) * 4;
+        const bool mine = inw >> q & 1u;
+#pragma unroll
+        for (int c = 0; c < BC; ++c) {
+          const S t = lane_value(src, xo[c]);
+          if (mine) x[q][c] = t;
+        }
+      }
+    }
+    if (!live) continue;
+    S acc[BC];
+#pragma unroll
+    for (int c = 0; c < BC; ++c) acc[c] = S(0);
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q)
+#pragma unroll
+      for (int c = 0; c < BC; ++c) acc[c] = fma(vi[q], x[q][c], acc[c]);
+    S out[BC];
+    if constexpr (AXPBY) {
+#pragma unroll
+      for (int c = 0; c < BC; ++c) {
+        double v = arow * double(acc[c]) + e.beta * double(xo[c]);
+        if (Z) v += e.gamma * double(zr[c]);
+        if constexpr (FORCE) v += gsh[p * B + off + c] * double(fr[c]);
+        out[c] = S(v);
+      }
+    }
+    else {
+      const S d = ds ? ld<true>(elem<O32>(ds, I(row))) : S(1);
+#pragma unroll
+      for (int c = 0; c < BC; ++c) out[c] = ds ? acc[c] * d : acc[c];
+    }
+    RowIO<S, BC>::template store<true>(elem<O32>(Y + po, ro), out);
+  }
+}
+
 template <typename S, int B, int L, bool AXPBY, bool O32>
 __global__ __launch_bounds__((spmm_threads<S, B>())) void k_spmm_lanes(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                                        const S* __restrict__ val, const S* __restrict__ X, S* Y, Epi e, int n)
@@ -439,7 +580,13 @@ int launch_ell_panels(trl_hip_ctx* ctx, size_t n, int variant, int width, const 
   do {                                                                                                                                             \
     constexpr unsigned T = panel_threads<S, B / CS_>();                                                                                            \
     const dim3 g{unsigned(row_ranges(n, T / CS_, e))}, blk{T};                                                                                     \
-    if (AXPBY && e.F)                                                                                                                              \
+    if (e.share) {                                                                                                                                 \
+      if (AXPBY && e.F)                                                                                                                            \
+        hipLaunchKernelGGL((k_spmm_ell_panels_sh<S, B, CS_, AXPBY, O32, AXPBY>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));        \
+      else                                                                                                                                         \
+        hipLaunchKernelGGL((k_spmm_ell_panels_sh<S, B, CS_, AXPBY, O32, false>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));        \
+    }                                                                                                                                              \
+    else if (AXPBY && e.F)                                                                                                                         \
       hipLaunchKernelGGL((k_spmm_ell_panels<S, B, CS_, AXPBY, O32, AXPBY>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));             \
     else                                                                                                                                           \
       hipLaunchKernelGGL((k_spmm_ell_panels<S, B, CS_, AXPBY, O32, false>), g, blk, 0, ctx->stream, width, col, val, X, Y, e, int(n));             \
@@ -569,9 +716,38 @@ __global__ __launch_bounds__(kThreads) void k_csr_to_ell(const int32_t* __restri
   }
 }
 
+// max |col - row| over the entries with |col - row| < limit (the next
+// bandwidth below the far off-diagonals: the line length of a grid stencil)
+__global__ __launch_bounds__(kThreads) void k_csr_bandwidth_below(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int n, int limit,
+                                                                  int* __restrict__ out)
+{
+  __shared__ int red[kThreads];
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const int d = abs(col[p] - i);
+      if (d < limit) m = max(m, d);
+    }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < unsigned(s)) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
+
 } // namespace
 
 namespace trl_hip_impl {
+
+int launch_csr_bandwidth_below(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, const int32_t* col, int limit, int* dev_out)
+{
+  hipLaunchKernelGGL(k_csr_bandwidth_below, dim3(unsigned(std::min<size_t>((n + kThreads - 1) / kThreads, 1024))), dim3(kThreads), 0, ctx->stream,
+                     rowptr, col, int(n), limit, dev_out);
+  TRL_HIP_LAUNCH_CHECK();
+  return 0;
+}
 
 int launch_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* dev_out)
 {
@@ -600,6 +776,18 @@ int launch_spmm(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hi
   if ((F != nullptr) != (g != nullptr) || (F && !axpby)) return TRL_HIP_EINVAL;
   Epi e{A.dscale, Z, alpha, beta, gamma, unsigned(npanels), n * size_t(b), unsigned(A.plane), unsigned(A.plane_block), 0, F, g};
   if (A.plane > 0) e.nz = unsigned((n + size_t(A.plane) - 1) / size_t(A.plane));
+  if (npanels > 1 && A.format == 1 && A.width <= kChunk) {
+    e.share = unsigned(A.share);
+    if (A.plane > 0 && A.tile_lines > 0 && A.line > 0) {
+      // tiles of tile_lines lines: only where they cover every plane block exactly
+      const int R = spmm_rows_per_block(t, b, A.variant, true);
+      const int ty = A.tile_lines, ln = A.line;
+      if (R > 0 && R % ty == 0 && ln % (R / ty) == 0 && A.plane_block % ln == 0 && (A.plane_block / ln) % ty == 0 && A.plane % A.plane_block == 0) {
+        e.line = unsigned(ln);
+        e.tly = unsigned(ty);
+      }
+    }
+  }
   const int variant = A.variant;
   // workgroup count bound: at least 256 lanes per workgroup
   const unsigned lanes = unsigned(variant < 0 ? -variant : (variant > 1 ? variant : 1));

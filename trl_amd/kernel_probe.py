@@ -40,6 +40,9 @@ def main(argv=None) -> int:
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10, help="untimed launches before each kernel's timed ones")
     ap.add_argument("--panels", type=int, default=5, help="panels per launch of panels32/panels64 (the refinement's blocks)")
+    ap.add_argument("--share", type=int, default=-1, help="several-panel step: 1 / 0 forces in-wave gathers shared between lanes on / off "
+                    "(-1: as tuned)")
+    ap.add_argument("--tile", type=int, default=-1, help="several-panel step: lines per workgroup tile, 0 off (-1: as tuned)")
     ap.add_argument("--what", default="spmm32,spmm64,spmm_axpby32,spmm_axpby64,panels32,panels64,forced32,gram32,update32,cholqr32,cholqr64,copy")
     a = ap.parse_args(argv)
     what = a.what.split(",")
@@ -89,6 +92,10 @@ def main(argv=None) -> int:
             npan = a.panels
             op.tune_panels(npan)
             d = op.desc_panels
+            if a.share >= 0:
+                d.share = a.share
+            if a.tile >= 0:
+                d.tile_lines, d.line = a.tile, (a.side if a.tile else 0)
             X = ctx.to_device(rng.standard_normal((npan, n, b)).astype(dt))
             Y = ctx.to_device(rng.standard_normal((npan, n, b)).astype(dt))
             mat = (d.width * n * (4 + s)) if d.format == 1 else (A.nnz * (4 + s) + (n + 1) * 4)
@@ -103,6 +110,10 @@ def main(argv=None) -> int:
             npan = a.panels
             op.tune_panels(npan)
             d = op.desc_panels
+            if a.share >= 0:
+                d.share = a.share
+            if a.tile >= 0:
+                d.tile_lines, d.line = a.tile, (a.side if a.tile else 0)
             X = ctx.to_device(rng.standard_normal((npan, n, b)).astype(np.float32))
             Y = ctx.to_device(rng.standard_normal((npan, n, b)).astype(np.float32))
             F = ctx.to_device(rng.standard_normal((npan, n, b)).astype(np.float32))

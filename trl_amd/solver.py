@@ -119,7 +119,8 @@ class DeviceOperator:
     @staticmethod
     def _name(d) -> str:
         order = f"/plane{d.plane}:{d.plane_block}" if d.plane else ""
-        return f"{'ell' if d.format else 'csr'}/v{d.variant}{order}"
+        extra = ("/share" if d.share else "") + (f"/tile{d.tile_lines}x{d.line}" if d.tile_lines else "")
+        return f"{'ell' if d.format else 'csr'}/v{d.variant}{order}{extra}"
 
     @property
     def name(self) -> str:
