@@ -195,7 +195,7 @@ def dominant_kernel_rate(ctx, dA, b, dtype, npanels, reps=200) -> dict:
     o32 = "true" if dA.n * b * sz <= 0xFFFFFFFF and (d.format == 0 or d.width * dA.n * max(sz, 4) <= 0xFFFFFFFF) else "false"
     cs = max(1, -d.variant)
     if multi and d.format == 1 and d.width <= 8 and d.variant <= 1:
-        name = f"k_spmm_ell_panels<{S}, {b}, {cs}, true, {o32}, false>"
+        name = f"k_spmm_ell_panels{'_sh' if d.share else ''}<{S}, {b}, {cs}, true, {o32}, false>"
     elif d.variant <= 1:
         name = f"k_spmm_rows<{S}, {b}, {cs}, {'true' if d.format else 'false'}, true, {o32}>"
     else:

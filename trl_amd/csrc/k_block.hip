@@ -14,6 +14,7 @@
 #include <string>
 
 #include <mutex>
+#include <type_traits>
 
 #include "common.hh"
 
@@ -44,6 +45,39 @@ template <typename S>
 __device__ constexpr int mfma_row(int q, int e)
 {
   return sizeof(S) == 8 ? 4 * e + q : 4 * q + e;
+}
+
+// Quad-lane exchanges (DPP quad permutes, no memory): the value of lane
+// (lane ^ 1) or (lane ^ 2) within each group of four lanes.
+template <int CTRL>
+__device__ inline float dpp_f(float x)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline double dpp_f(double x)
+{
+  const long long u = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_mov_dpp(int(u & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(int(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+constexpr int kDppXor1 = 0xB1; // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E; // quad_perm [2, 3, 0, 1]
+
+// 2 x 2 transposes of registers (e0, e0 + D) across lanes (lane, lane ^ L)
+// in place: afterwards the lane with bit L clear holds what lane and its
+// partner held in register e0, the other what they held in e0 + D.
+template <int CTRL, int L, int D, typename A>
+__device__ inline void pair_transpose(A& r, int e0, int lane)
+{
+  const bool hi = (lane & L) != 0;
+  const auto send = hi ? r[e0] : r[e0 + D];
+  const auto recv = dpp_f<CTRL>(send);
+  if (hi)
+    r[e0] = recv;
+  else
+    r[e0 + D] = recv;
 }
 
 constexpr int kThreads = 256;
@@ -513,14 +547,34 @@ __global__ __launch_bounds__(kThreads) void k_update(const S* __restrict__ V, si
 // The needed part of Y (m*B x nb*B, padded to NT*16 columns) is staged in
 // LDS in the panel type; each wave computes 16 rows x NT*16 columns with K
 // running over the m*B basis columns.
+// Rotation operand loads: VEC consecutive columns of a basis row per lane
+// and load (16 bytes where the block size allows), fed to VEC successive
+// MFMA steps.  The MFMA's k-slot q of lane group q then stands for k =
+// kb + VEC q + v at step v: a permutation of the contraction index, applied
+// to the LDS-held coefficient operand alike.
+template <typename S, int B>
+constexpr int rot_vec()
+{
+  constexpr int v = 16 / int(sizeof(S));
+  return (B % v == 0) ? v : (B % 2 == 0 ? 2 : 1);
+}
+// k-steps whose operand loads are issued together, ahead of their products
 constexpr int kRotUnroll = 8;
 
-template <typename S, int B, int NT>
+template <typename S, int VEC>
+struct RotVec {
+  using T = std::conditional_t<VEC == 1, S, S __attribute__((ext_vector_type(VEC)))>;
+};
+
+template <typename S, int B, int NT, int VEC>
 __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
                                                 S* __restrict__ Out, size_t n)
 {
   using M = Mfma<S>;
   using acc_t = typename M::acc_t;
+  constexpr int KS = 4 * VEC;                                     // k per super-step (VEC MFMA steps)
+  constexpr int U = (kRotUnroll + VEC - 1) / VEC;                 // super-steps loaded together
+  using VT = typename RotVec<S, VEC>::T;
   extern __shared__ unsigned char smem_raw[];
   S* Ys = reinterpret_cast<S*>(smem_raw);
   const int K = m * B, NC = nb * B;
@@ -538,7 +592,7 @@ __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int i16 = lane & 15, q4 = lane >> 4;
   const size_t ntiles = (n + 15) / 16;
-  const int ksteps = (K + 3) / 4;
+  const int nsup = (K + KS - 1) / KS;
   for (size_t rt = size_t(blockIdx.x) * nwave + wave; rt < ntiles; rt += size_t(gridDim.x) * nwave) {
     const size_t r0 = rt * 16;
     const size_t ra = r0 + i16;
@@ -546,39 +600,83 @@ __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t 
     acc_t acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = acc_t{0, 0, 0, 0};
-    // the basis loads of kRotUnroll k-steps are issued together, ahead of
-    // their products: one memory round trip per chunk, not per k-step
-    for (int s0 = 0; s0 < ksteps; s0 += kRotUnroll) {
-      S a[kRotUnroll];
+    for (int s0 = 0; s0 < nsup; s0 += U) {
+      VT a[U];
 #pragma unroll
-      for (int u = 0; u < kRotUnroll; ++u) {
-        const int kk = 4 * (s0 + u) + q4;
-        a[u] = S(0);
-        if (rok && kk < K) {
-          const int j = kk / B, c = kk % B;
-          a[u] = V[size_t(j) * ldv + ra * B + c];
-        }
+      for (int u = 0; u < U; ++u) {
+        const int k0 = (s0 + u) * KS + VEC * q4; // VEC columns of one block row (VEC divides B)
+        a[u] = VT{};
+        if (rok && k0 < K) a[u] = *reinterpret_cast<const VT*>(V + size_t(k0 / B) * ldv + ra * B + (k0 % B));
       }
 #pragma unroll
-      for (int u = 0; u < kRotUnroll; ++u) {
-        if (s0 + u >= ksteps) break; // uniform over the wave
-        const int kk = 4 * (s0 + u) + q4;
-        const bool kok = kk < K;
+      for (int u = 0; u < U; ++u) {
+        if (s0 + u >= nsup) break; // uniform over the wave
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const S bv = kok ? Ys[kk * ldy + t * 16 + i16] : S(0);
-          acc[t] = M::op(a[u], bv, acc[t]);
+        for (int v = 0; v < VEC; ++v) {
+          const int kk = (s0 + u) * KS + VEC * q4 + v;
+          const bool kok = kk < K;
+          S av;
+          if constexpr (VEC == 1)
+            av = a[u];
+          else
+            av = a[u][v];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const S bv = kok ? Ys[kk * ldy + t * 16 + i16] : S(0);
+            acc[t] = M::op(av, bv, acc[t]);
+          }
         }
       }
     }
+    // stores: element e of lane (i16, q4) is row mfma_row(q4, e), column
+    // t*16 + i16 of the output.  Transposed across neighbouring lanes so
+    // that each lane writes 16 contiguous bytes of one row where the block
+    // size allows.
+    if constexpr (sizeof(S) == 4 && B % 4 == 0) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int jn = t * 16 + i16;
-      if (jn >= NC) continue;
+      for (int t = 0; t < NT; ++t) {
+        acc_t r = acc[t];
+        pair_transpose<kDppXor1, 1, 1>(r, 0, lane);
+        pair_transpose<kDppXor1, 1, 1>(r, 2, lane);
+        pair_transpose<kDppXor2, 2, 2>(r, 0, lane);
+        pair_transpose<kDppXor2, 2, 2>(r, 1, lane);
+        // r[j]: row 4 q4 + (i16 & 3), column t*16 + (i16 & ~3) + j
+        const int jn = t * 16 + (i16 & ~3);
+        const size_t row = r0 + 4 * q4 + (i16 & 3);
+        if (jn < NC && row < n) {
+          using V4 = S __attribute__((ext_vector_type(4)));
+          *reinterpret_cast<V4*>(Out + size_t(jn / B) * ldv + row * B + (jn % B)) = V4{r[0], r[1], r[2], r[3]};
+        }
+      }
+    }
+    else if constexpr (sizeof(S) == 8 && B % 2 == 0) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const size_t row = r0 + mfma_row<S>(q4, e);
-        if (row < n) Out[size_t(jn / B) * ldv + row * B + (jn % B)] = static_cast<S>(acc[t][e]);
+      for (int t = 0; t < NT; ++t) {
+        acc_t r = acc[t];
+        pair_transpose<kDppXor1, 1, 1>(r, 0, lane);
+        pair_transpose<kDppXor1, 1, 1>(r, 2, lane);
+        // (r[e0], r[e0 + 1]), e0 = 0, 2: row mfma_row(q4, e0 + (i16 & 1)), columns t*16 + (i16 & ~1) + 0, 1
+        const int jn = t * 16 + (i16 & ~1);
+        if (jn < NC) {
+          using V2 = S __attribute__((ext_vector_type(2)));
+#pragma unroll
+          for (int e0 = 0; e0 < 4; e0 += 2) {
+            const size_t row = r0 + mfma_row<S>(q4, e0 + (i16 & 1));
+            if (row < n) *reinterpret_cast<V2*>(Out + size_t(jn / B) * ldv + row * B + (jn % B)) = V2{r[e0], r[e0 + 1]};
+          }
+        }
+      }
+    }
+    else {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int jn = t * 16 + i16;
+        if (jn >= NC) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const size_t row = r0 + mfma_row<S>(q4, e);
+          if (row < n) Out[size_t(jn / B) * ldv + row * B + (jn % B)] = static_cast<S>(acc[t][e]);
+        }
       }
     }
   }
@@ -828,11 +926,165 @@ __global__ void k_chol_small(const double* __restrict__ G, const double* __restr
   }
 }
 
+// Gram matrix of one panel, one row per lane per step, in FP64:
+// G = Y^T diag(m) Y with Y = W, or (TRANSFORM) Y = W R1^{-1} rounded to the
+// panel type exactly as k_scale stores it -- the Gram that CholQR2's second
+// pass needs, formed without writing Y.  Each lane keeps the B(B+1)/2
+// entries of the upper triangle; the workgroup sums them (butterfly per
+// wave, waves in order) and writes both triangles of its partial Gram,
+// entry-major (entry e of workgroup w at e*G + w) for k_reduce.
+template <typename S, int B, bool TRANSFORM>
+__global__ __launch_bounds__(kThreads) void k_gram_rows(const S* __restrict__ W, const S* __restrict__ md, size_t n, const double* __restrict__ Rinv,
+                                                        double* __restrict__ partials)
+{
+  constexpr int NP = B * (B + 1) / 2;
+  __shared__ double Rs[TRANSFORM ? B * B : 1];
+  __shared__ double red[kThreads / 64][NP];
+  if constexpr (TRANSFORM) {
+    for (int i = threadIdx.x; i < B * B; i += blockDim.x) Rs[i] = Rinv[i];
+    __syncthreads();
+  }
+  double g[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) g[i] = 0.0;
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t row = size_t(blockIdx.x) * blockDim.x + threadIdx.x; row < n; row += stride) {
+    S w[B];
+    RowIO<S, B>::template load<true>(W + row * B, w);
+    double y[B];
+    if constexpr (TRANSFORM) {
+      double acc[B];
+#pragma unroll
+      for (int c = 0; c < B; ++c) acc[c] = 0.0;
+#pragma unroll
+      for (int cp = 0; cp < B; ++cp) {
+        const double x = static_cast<double>(w[cp]);
+#pragma unroll
+        for (int c = cp; c < B; ++c) acc[c] = fma(x, Rs[cp * B + c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < B; ++c) y[c] = static_cast<double>(static_cast<S>(acc[c]));
+    }
+    else {
+#pragma unroll
+      for (int c = 0; c < B; ++c) y[c] = static_cast<double>(w[c]);
+    }
+    const double m = md ? static_cast<double>(ld<true>(md + row)) : 1.0;
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const double yi = md ? y[i] * m : y[i];
+#pragma unroll
+      for (int j = i; j < B; ++j) {
+        g[idx] = fma(yi, y[j], g[idx]);
+        ++idx;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    double v = g[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  const unsigned G = gridDim.x;
+  for (int e = threadIdx.x; e < B * B; e += blockDim.x) {
+    const int i = e / B, j = e % B;
+    const int a = i < j ? i : j, b2 = i < j ? j : i;
+    const int idx = a * B - a * (a - 1) / 2 + (b2 - a); // packed upper-triangle index of (a, b2)
+    double t = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q][idx];
+    partials[size_t(e) * G + blockIdx.x] = t;
+  }
+}
+
+// W <- (W R1^{-1}) R2^{-1} with the intermediate rounded to the panel type:
+// the two scale passes of CholQR2 in one read and one write of W.
+template <typename S, int B>
+__global__ __launch_bounds__(kThreads) void k_scale2(S* __restrict__ W, const double* __restrict__ R1inv, const double* __restrict__ R2inv, size_t n)
+{
+  __shared__ double Rs[2][B * B];
+  for (int i = threadIdx.x; i < B * B; i += blockDim.x) {
+    Rs[0][i] = R1inv[i];
+    Rs[1][i] = R2inv[i];
+  }
+  __syncthreads();
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t row = size_t(blockIdx.x) * blockDim.x + threadIdx.x; row < n; row += stride) {
+    S w[B];
+    RowIO<S, B>::template load<true>(W + row * B, w);
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      double acc[B];
+#pragma unroll
+      for (int c = 0; c < B; ++c) acc[c] = 0.0;
+#pragma unroll
+      for (int cp = 0; cp < B; ++cp) {
+        const double x = static_cast<double>(w[cp]);
+#pragma unroll
+        for (int c = cp; c < B; ++c) acc[c] = fma(x, Rs[pass][cp * B + c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < B; ++c) w[c] = static_cast<S>(acc[c]);
+    }
+    RowIO<S, B>::template store<true>(W + row * B, w);
+  }
+}
+
 // Cholesky-QR with the Gram matrix on MFMA (the block-product kernel on W
 // against itself), a one-thread FP64 Cholesky and the FP64 scale; one panel
 // read for the Gram and one read + write for the scale per pass.
+// CholQR2 in four panel passes: G1 = W^T M W (read W), R1; G2 = Y^T M Y for
+// Y = W R1^{-1} formed on the fly (read W), R2 and R = R2 R1; W <- Y R2^{-1}
+// (read and write W) -- where the two-pass form below reads W three times
+// and writes it twice.  Thread-per-row FP64 Gram kernels; per-workgroup
+// partials summed by k_reduce; the b x b Cholesky in FP64 on the device.
+int launch_cholqr2_rows(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R)
+{
+  double* R1 = ctx->small;
+  double* Rinv1 = ctx->small + 256;
+  double* Rinv2 = ctx->small + 512;
+  double* G = ctx->small + 768;
+  TRL_DISPATCH_T(t, {
+    TRL_DISPATCH_B(b, {
+      S* w = static_cast<S*>(W);
+      const S* m = static_cast<const S*>(md);
+      size_t g = (n + kThreads - 1) / kThreads;
+      const size_t gmax = size_t(4) * ctx->num_cus;
+      if (g > gmax) g = gmax;
+      if (g < 1) g = 1;
+      int rc = ensure_partials(ctx, g * B * B);
+      if (rc) return rc;
+      const dim3 grid{unsigned(g)}, blk{unsigned(kThreads)};
+      hipLaunchKernelGGL((k_gram_rows<S, B, false>), grid, blk, 0, ctx->stream, w, m, n, nullptr, ctx->partials);
+      TRL_HIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_reduce, dim3(unsigned(B * B)), dim3(kThreads), 0, ctx->stream, ctx->partials, int(g), B * B, G, nullptr);
+      TRL_HIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_chol_small<B>), dim3(1), dim3(64), 0, ctx->stream, G, nullptr, R1, Rinv1, ctx->flags);
+      TRL_HIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_gram_rows<S, B, true>), grid, blk, 0, ctx->stream, w, m, n, Rinv1, ctx->partials);
+      TRL_HIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_reduce, dim3(unsigned(B * B)), dim3(kThreads), 0, ctx->stream, ctx->partials, int(g), B * B, G, nullptr);
+      TRL_HIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_chol_small<B>), dim3(1), dim3(64), 0, ctx->stream, G, R1, R, Rinv2, ctx->flags);
+      TRL_HIP_LAUNCH_CHECK();
+      if (n == 0) return 0;
+      size_t gs = (n + kThreads - 1) / kThreads;
+      const size_t gsmax = size_t(8) * ctx->num_cus;
+      if (gs > gsmax) gs = gsmax;
+      hipLaunchKernelGGL((k_scale2<S, B>), dim3(unsigned(gs)), dim3(kThreads), 0, ctx->stream, w, Rinv1, Rinv2, n);
+      TRL_HIP_LAUNCH_CHECK();
+    });
+  });
+  return 0;
+}
+
 int launch_cholqr_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes)
 {
+  if (passes == 2 && b <= 8) return launch_cholqr2_rows(ctx, t, n, b, W, md, R);
   double* R1 = ctx->small;
   double* Rinv = ctx->small + 256;
   double* G = ctx->small + 768;
@@ -917,7 +1169,13 @@ int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, con
       int nb_max = (nt_fit * 16) / B;
       if (nb_max < 1) return TRL_HIP_EINVAL;
       size_t g = (n + 127) / 128;
-      const size_t gmax = size_t(2) * ctx->num_cus;
+      // four 512-lane workgroups per CU (two: 3.60 TB/s instead of 3.98 for the
+      // FP32 sweep rotation, m = kc = 5, on the 160^3 stencil)
+      const size_t gmax = size_t(4) * ctx->num_cus;
+      // FP64 with many basis blocks (the Lanczos restart, m = 32) takes one
+      // element per load: 2.74 ms against 3.16 ms for two (their register
+      // cost lowers the occupancy of the long contraction loop)
+      const bool v1 = sizeof(S) == 8 && m >= 8;
       if (g > gmax) g = gmax;
       for (int j0 = 0; j0 < kc; j0 += nb_max) {
         const int nb = (kc - j0) < nb_max ? (kc - j0) : nb_max;
@@ -927,16 +1185,28 @@ int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, con
         S* o = static_cast<S*>(Out) + size_t(j0) * n * B;
         switch (nt) {
         case 1:
-          hipLaunchKernelGGL((k_rotate<S, B, 1>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+          if (v1)
+            hipLaunchKernelGGL((k_rotate<S, B, 1, 1>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+          else
+            hipLaunchKernelGGL((k_rotate<S, B, 1, rot_vec<S, B>()>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
           break;
         case 2:
-          hipLaunchKernelGGL((k_rotate<S, B, 2>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+          if (v1)
+            hipLaunchKernelGGL((k_rotate<S, B, 2, 1>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+          else
+            hipLaunchKernelGGL((k_rotate<S, B, 2, rot_vec<S, B>()>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
           break;
         case 3:
-          hipLaunchKernelGGL((k_rotate<S, B, 3>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+          if (v1)
+            hipLaunchKernelGGL((k_rotate<S, B, 3, 1>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+          else
+            hipLaunchKernelGGL((k_rotate<S, B, 3, rot_vec<S, B>()>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
           break;
         default:
-          hipLaunchKernelGGL((k_rotate<S, B, 4>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+          if (v1)
+            hipLaunchKernelGGL((k_rotate<S, B, 4, 1>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+          else
+            hipLaunchKernelGGL((k_rotate<S, B, 4, rot_vec<S, B>()>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
           break;
         }
         TRL_HIP_LAUNCH_CHECK();
