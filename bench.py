@@ -287,6 +287,8 @@ def main(argv=None) -> int:
                     help="1: the FP64 sweeps' filter runs in FP32, driven by the FP64 residuals of the Ritz pairs")
     ap.add_argument("--ncv-fp32", type=int, default=S["ncv_fp32"], help="Krylov space of the FP32 Lanczos phase (the FP64 solves keep ncv)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--rr-compare", action="store_true", help="also run the plain FP64 solve with the dense Householder Rayleigh-Ritz eigensolver "
+                    "in every step (the structured one is the default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-oracle-runs", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-tune", action="store_true", help="untuned CSR operator kernel instead of the autotuned one")
@@ -372,8 +374,21 @@ def main(argv=None) -> int:
                 "mixed_vs_fp64_rel_eig_diff_per_pair": [float(f"{x:.3e}") for x in per_pair],
                 "mixed_true_residual_per_pair": [float(f"{x:.3e}") for x in last.resid],
                 "fp64_true_residual_per_pair": [float(f"{x:.3e}") for x in o64.resid],
-                "mixed_vs_fp64_max_principal_angle": float(np.arcsin(min(1.0, sin_max)))}
-        log(f"fp64 solve {t64:.2f} s, cycles {o64.result['iterations']}, eig agreement {agree:.2e}")
+                "mixed_vs_fp64_max_principal_angle": float(np.arcsin(min(1.0, sin_max))),
+                # the host Rayleigh-Ritz steps of the solve, and how many took
+                # the structured (arrow + band) eigensolver
+                "host_rr_s": o64.result["host_seconds"], "rr_steps": o64.result["rr_steps"], "rr_structured": o64.result["rr_structured"]}
+        log(f"fp64 solve {t64:.2f} s, cycles {o64.result['iterations']}, eig agreement {agree:.2e}, host Rayleigh-Ritz "
+            f"{o64.result['host_seconds']:.3f} s in {o64.result['rr_steps']} steps ({o64.result['rr_structured']} structured)")
+        if a.rr_compare:
+            ctx.sync()
+            t = time.perf_counter()
+            od = solve(dA, cfg.b, params, V0, dtype="f64", order="ascending", reorth=a.reorth, spmm_tune=tune, rr_dense=True)
+            ctx.sync()
+            fp64["dense_rr"] = {"seconds": time.perf_counter() - t, "host_rr_s": od.result["host_seconds"], "rr_steps": od.result["rr_steps"],
+                                "rr_structured": od.result["rr_structured"], "cycles": od.result["iterations"],
+                                "max_rel_eig_diff_vs_structured": float(np.max(np.abs(od.evals - o64.evals) / np.abs(o64.evals)))}
+            log(f"fp64 solve with the dense Rayleigh-Ritz: {fp64['dense_rr']}")
         ctx.sync()
         t = time.perf_counter()
         os_ = solve_mixed(dA, cfg.b, params, V0, phase1_fp64=True, **mixed_kw)

@@ -90,7 +90,9 @@ public:
     // may ask for more (set_ritz_blocks)
     const std::size_t blocks = std::max<std::size_t>(nev / blocksize + 1, ritz_blocks_);
     trl_amd::rayleigh_ritz(blocksize, nb, T.block_cols(), tb.data(), bh.data(), nev, static_cast<int>(order_), tolerance_, rr_, dense_,
-                           blocks * blocksize);
+                           blocks * blocksize, !dense_rr_);
+    ++rr_steps_;
+    if (rr_.structured >= 0) ++rr_structured_;
     eigenvalues = rr_.theta;
     evec_host_ = rr_.Y;
     last_residuals_ = rr_.residual;
@@ -125,6 +127,12 @@ public:
   // At least this many leading blocks of Ritz vectors are computed by every
   // Rayleigh-Ritz step (default: the nev/b + 1 the driver uses).
   void set_ritz_blocks(std::size_t blocks) { ritz_blocks_ = blocks; }
+  // true: the dense Householder eigensolver for every Rayleigh-Ritz step
+  // (the structured one is used otherwise whenever T has the restart form)
+  void set_dense_rayleigh_ritz(bool on) { dense_rr_ = on; }
+  // Rayleigh-Ritz steps so far, and of them those by the structured solver
+  unsigned rr_steps() const { return rr_steps_; }
+  unsigned rr_structured() const { return rr_structured_; }
   int get_cholqr_passes() const { return cholqr_passes_; }
 
   // Optional diagonal inner-product weight M (device, panel type, n entries).
@@ -179,6 +187,8 @@ protected:
   const PanelScalar* mass_ = nullptr;
   int cholqr_passes_ = std::is_same_v<PanelScalar, float> ? 2 : 1;
   std::size_t ritz_blocks_ = 0;
+  bool dense_rr_ = false;
+  unsigned rr_steps_ = 0, rr_structured_ = 0;
 
   std::vector<double> eigenvalues;
   std::unique_ptr<BlockMatrix> eigenvectors;

@@ -27,7 +27,7 @@ public:
   // Borrows rowptr[n+1], col[nnz], val[nnz] (device) and the optional row
   // scaling dscale[n]; variant as in trl_hip_spmat (0 = untuned default).
   SparseOperator(const Context& ctx, std::size_t n, const int32_t* rowptr, const int32_t* col, const S* val, const S* dscale = nullptr, int variant = 0)
-      : ctx_(ctx), n_(n), csr_{0, rowptr, 0, col, val, dscale, variant, 0, 0}, op_(csr_), op_axpby_(csr_), op_panels_(csr_)
+      : ctx_(ctx), n_(n), csr_{0, rowptr, 0, col, val, dscale, variant, 0, 0, 0, 0, 0}, op_(csr_), op_axpby_(csr_), op_panels_(csr_)
   {
   }
 
@@ -158,7 +158,7 @@ private:
       ell_ = std::make_shared<Ell>(ctx_, n_, width);
       throw_on(trl_hip_csr_to_ell(ctx_.get(), dtype, n_, csr_.rowptr, csr_.col, csr_.val, width, ell_->col.data(), ell_->val.data()), "csr_to_ell");
     }
-    ell_desc_ = trl_hip_spmat{1, nullptr, width, ell_->col.data(), ell_->val.data(), csr_.dscale, 0, 0, 0};
+    ell_desc_ = trl_hip_spmat{1, nullptr, width, ell_->col.data(), ell_->val.data(), csr_.dscale, 0, 0, 0, 0, 0, 0};
   }
 
   Context ctx_;

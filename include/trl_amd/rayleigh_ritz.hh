@@ -24,6 +24,7 @@ struct RitzResult {
   std::vector<double> Y;         // nb x bc blocks of b x b: block (i, j) at ((i*bc + j)*b*b), row-major
   std::vector<double> residual;  // estimates of the first min(nev, nt) pairs
   std::size_t nconv = 0;
+  int structured = -1;           // the Ritz part p of the structured eigensolver; -1: the dense one
 };
 
 // T: nb x bc blocks of b x b in BlockMatrix layout (block (i, j) at
@@ -33,9 +34,12 @@ struct RitzResult {
 // ncols: the Ritz vectors needed, in the chosen order (the driver uses the
 // first k_r*b = (nev/b + 1)*b: the restart rotation and the arrow,
 // lanczos.hh:105-143); Y's other columns are left zero.  Fewer columns than
-// nb*b take the eigensolver that computes only those eigenvectors.
+// nb*b take the eigensolver that computes only those eigenvectors: the
+// structured one (dense::ArrowBandEigensolver) when T has the thick-restart
+// form (unless allow_structured is false), the dense Householder one
+// otherwise; out.structured tells which.
 inline void rayleigh_ritz(unsigned b, std::size_t nb, std::size_t bc, const double* T, const double* beta, std::size_t nev, int order, double tol,
-                          RitzResult& out, std::vector<double>& dense, std::size_t ncols = std::size_t(-1))
+                          RitzResult& out, std::vector<double>& dense, std::size_t ncols = std::size_t(-1), bool allow_structured = true)
 {
   const std::size_t nt = nb * b;
   ncols = std::min(ncols, nt);
@@ -50,12 +54,20 @@ inline void rayleigh_ritz(unsigned b, std::size_t nb, std::size_t bc, const doub
   // zcol[j]: the column of Z holding the eigenvector of ascending eigenvalue j
   std::vector<std::size_t> zcol(nt, 0);
   std::size_t zld = nt;
+  out.structured = -1;
   if (ncols == nt) {
     Z.resize(nt * nt);
     if (dense::syev(int(nt), dense.data(), w.data(), Z.data()) != 0) throw std::runtime_error("Eigendecomposition failed");
     std::iota(zcol.begin(), zcol.end(), std::size_t(0));
   }
   else {
+    // T of the thick restart is a diagonal Ritz part, its arrow to block
+    // k_r, and a band of half-bandwidth b (block tridiagonal with the
+    // upper-triangular R factors below the diagonal): the structured solver
+    // reduces it in O(nt^2 b); any other T takes the dense one
+    const int p = allow_structured ? dense::arrow_band_structure(int(nt), int(b), dense.data()) : -1;
+    dense::ArrowBandEigensolver ab;
+    const bool structured = p >= 0 && ab.reduce(int(nt), int(b), p, dense.data(), w.data()) == 0;
     // the order depends only on the eigenvalues: find them first (the
     // selection below then asks for the eigenvectors of the leading ncols)
     std::vector<int> sel;
@@ -64,16 +76,20 @@ inline void rayleigh_ritz(unsigned b, std::size_t nb, std::size_t bc, const doub
     else if (order == 2)
       for (std::size_t j = nt - ncols; j < nt; ++j) sel.push_back(int(j));
     else {
-      if (dense::syev_select(int(nt), dense.data(), w.data(), 0, nullptr, nullptr) != 0) throw std::runtime_error("Eigendecomposition failed");
+      if (!structured && dense::syev_select(int(nt), dense.data(), w.data(), 0, nullptr, nullptr) != 0)
+        throw std::runtime_error("Eigendecomposition failed");
       std::iota(idx.begin(), idx.end(), 0u);
       std::stable_sort(idx.begin(), idx.end(), [&](unsigned x, unsigned y) { return std::abs(w[x]) > std::abs(w[y]); });
       for (std::size_t j = 0; j < ncols; ++j) sel.push_back(int(idx[j]));
       std::sort(sel.begin(), sel.end());
     }
     Z.resize(nt * ncols);
-    if (dense::syev_select(int(nt), dense.data(), w.data(), int(ncols), sel.data(), Z.data()) == 0) {
+    const int rc = structured ? ab.vectors(int(ncols), sel.data(), Z.data())
+                              : dense::syev_select(int(nt), dense.data(), w.data(), int(ncols), sel.data(), Z.data());
+    if (rc == 0) {
       for (std::size_t c = 0; c < ncols; ++c) zcol[std::size_t(sel[c])] = c;
       zld = ncols;
+      if (structured) out.structured = p;
     }
     else {
       // inverse iteration gave up (a non-finite iterate): the full solver

@@ -39,6 +39,10 @@ typedef struct {
   int spmm_tune;         /* 1: time the operator kernels (CSR variants, ELL) for
                             b at the start of the solve and use the fastest;
                             lanes_per_row is then ignored                        */
+  int rr_dense;          /* 1: the Rayleigh-Ritz step always takes the dense
+                            Householder eigensolver (for comparison); 0: the
+                            structured one whenever T has the thick-restart
+                            form (diagonal Ritz part, arrow, band)              */
 } trl_hip_solver_params;
 
 typedef struct {
@@ -47,6 +51,8 @@ typedef struct {
   unsigned n_op_apply;     /* EigensolverResult::n_op_apply (params.hh:15)   */
   double seconds;          /* wall time of solve(), device work included     */
   double host_seconds;     /* of which the host Rayleigh-Ritz steps          */
+  unsigned rr_steps;       /* Rayleigh-Ritz steps                            */
+  unsigned rr_structured;  /* of which by the structured eigensolver         */
 } trl_hip_solver_result;
 
 /* Standard (mass = mass_inv = NULL) or generalized problem with a diagonal
@@ -176,6 +182,18 @@ int trl_hip_dense_syev(int n, const double* A, double* w, double* Z);
  * The Rayleigh-Ritz step uses it for the k_r*b Ritz vectors the driver
  * needs. */
 int trl_hip_dense_syev_select(int n, const double* A, double* w, int k, const int* sel, double* Z);
+/* trl_hip_dense_syev_select for the projected matrix of the thick-restart
+ * block Lanczos (block size b): A[0:p, 0:p] diagonal (the kept Ritz values),
+ * those rows coupled only to the b coordinates p..p+b-1 (the arrow), and
+ * A[p:, p:] of half-bandwidth b (block tridiagonal with upper-triangular
+ * sub-diagonal blocks).  *p_out receives p (0 before the first restart), or
+ * -1 with TRL_HIP_EINVAL when A lacks that structure.  The Ritz part is
+ * reduced to band form by Householder QL steps, the band to tridiagonal form
+ * by Householder bulge chasing: O(n^2 (b + k)) work in place
+ * of the O(n^3) Householder tridiagonalisation.  The step it accelerates is
+ * the dense eigensolve of impl/openmp/evp_base.hh:79-154; the reference has
+ * no separate structured solver. */
+int trl_hip_dense_syev_arrow_band(int n, int b, const double* A, double* w, int k, const int* sel, double* Z, int* p_out);
 
 /* ---- Matrix Market coordinate files (host) -------------------------------
  * Replaces load_matrix_market of examples/sycl/csrevp.hh:26-124: '%' comment

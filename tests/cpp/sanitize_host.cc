@@ -1,5 +1,5 @@
 // Host code under AddressSanitizer and UndefinedBehaviorSanitizer: the dense
-// eigensolver (both paths, serial and threaded reductions), the Rayleigh-Ritz
+// eigensolver (all three paths, serial and threaded reductions), the Rayleigh-Ritz
 // step in its three orders, the Matrix Market reader on good and refused
 // files, and the oracle's Lanczos solve on a small 1D Laplacian.  Built with
 // g++ only (no device code) by tests/test_host_sanitizers.py; prints OK and
@@ -73,7 +73,10 @@ void dense_cases()
     check(trl_amd::dense::syev_select(n, A.data(), ws.data(), k, sel.data(), Zs.data()) == 0, "syev_select n=" + std::to_string(n));
     for (int c = 0; c < k; ++c) wk[c] = ws[sel[c]];
     check(eigen_residual(n, A, wk.data(), Zs.data(), k, k) < 1e-10 * n, "syev_select residual n=" + std::to_string(n));
-    for (int j = 0; j < n; ++j) check(ws[j] == w[j], "syev_select eigenvalues n=" + std::to_string(n));
+    // the eigenvalue-only QR (root-free) against the one that accumulates vectors
+    double wmax = 0.0;
+    for (int j = 0; j < n; ++j) wmax = std::max(wmax, std::fabs(w[j]));
+    for (int j = 0; j < n; ++j) check(std::fabs(ws[j] - w[j]) <= 1e-13 * std::max(wmax, 1.0) * n, "syev_select eigenvalues n=" + std::to_string(n));
   }
   // a repeated eigenvalue (the 2D Laplacian's symmetric pairs), threaded size
   const int s = 12, n = s * s;
@@ -93,6 +96,38 @@ void dense_cases()
   check(trl_amd::dense::syev_select(n, L.data(), w.data(), 10, sel.data(), Z.data()) == 0, "syev_select laplace2d");
   for (int c = 0; c < 10; ++c) wk[c] = w[c];
   check(eigen_residual(n, L, wk.data(), Z.data(), 10, 10) < 1e-11, "syev_select laplace2d residual");
+  // the structured path on the thick-restart form: 5 Ritz blocks (diagonal),
+  // the arrow to block 5, then block tridiagonal (b = 8, 20 blocks: threaded
+  // back-transformation), against the dense path
+  {
+    const int b = 8, kr = 5, nb = 20, nt = nb * b, p = kr * b;
+    std::mt19937_64 g(9);
+    std::normal_distribution<double> nd;
+    std::vector<double> T(std::size_t(nt) * nt, 0.0);
+    auto at = [&](int i, int j) -> double& { return T[std::size_t(i) * nt + j]; };
+    for (int i = 0; i < p; ++i) at(i, i) = nd(g);
+    for (int r = 0; r < b; ++r)
+      for (int c = 0; c < p; ++c) at(p + r, c) = at(c, p + r) = 0.1 * nd(g);
+    for (int blk = kr; blk < nb; ++blk) {
+      const int o = blk * b;
+      for (int r = 0; r < b; ++r)
+        for (int c = 0; c <= r; ++c) at(o + r, o + c) = at(o + c, o + r) = nd(g);
+      if (blk + 1 < nb)
+        for (int r = 0; r < b; ++r)
+          for (int c = r; c < b; ++c) at(o + b + r, o + c) = at(o + c, o + b + r) = nd(g);
+    }
+    check(trl_amd::dense::arrow_band_structure(nt, b, T.data()) == p, "arrow_band_structure finds the Ritz part");
+    std::vector<int> sb(48);
+    for (int j = 0; j < 48; ++j) sb[j] = j;
+    std::vector<double> wb(nt), Zb(std::size_t(nt) * 48), wd(nt), Zd(std::size_t(nt) * 48), wk2(48);
+    check(trl_amd::dense::syev_arrow_band_select(nt, b, p, T.data(), wb.data(), 48, sb.data(), Zb.data()) == 0, "syev_arrow_band_select");
+    check(trl_amd::dense::syev_select(nt, T.data(), wd.data(), 48, sb.data(), Zd.data()) == 0, "syev_select on the restart form");
+    for (int j = 0; j < nt; ++j) check(std::fabs(wb[j] - wd[j]) < 1e-12, "syev_arrow_band_select eigenvalues");
+    for (int c = 0; c < 48; ++c) wk2[c] = wb[c];
+    check(eigen_residual(nt, T, wk2.data(), Zb.data(), 48, 48) < 1e-11, "syev_arrow_band_select residual");
+    T[std::size_t(3) * nt + p + b + 1] = T[std::size_t(p + b + 1) * nt + 3] = 1.0;
+    check(trl_amd::dense::arrow_band_structure(nt, b, T.data()) == -1, "arrow_band_structure rejects a coupling beyond block kr");
+  }
   // argument checks
   int bad[2] = {1, 1};
   check(trl_amd::dense::syev_select(4, L.data(), w.data(), 2, bad, Z.data()) == -2, "syev_select rejects a non-increasing selection");

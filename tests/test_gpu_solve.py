@@ -74,6 +74,42 @@ def test_laplace2d_128_matches_oracle(ctx, reorth):
     assert ang < 1e-6
 
 
+@pytest.mark.parametrize("order", ["ascending", "largest_magnitude"])
+def test_structured_rayleigh_ritz_matches_dense(ctx, order):
+    # the same 128^2 solve (nt = 256) with the structured Rayleigh-Ritz
+    # eigensolver (arrow + band, the default) and with the dense Householder
+    # one in every step: every step of the default run takes the structured
+    # path, the dense run none; eigenvalues agree to 1e-12 relative, both meet
+    # the tolerance, and the host time per step is printed
+    c = P.config_c1()
+    print(f"structured RR laplace2d_128 order={order}", flush=True)
+    dA = DeviceCSR(ctx, c.csr, precisions=("f64",))
+    prm = EigensolverParams(c.nev, c.ncv, tolerance=c.tol)
+    # one untimed solve first: the first one pays for starting the host
+    # thread pool and first touches
+    solve(dA, c.b, prm, v0(ctx, c.csr.n, c.b), order=order, reorth="cgs2")
+    runs = {}
+    for dense in (False, True):
+        out = solve(dA, c.b, prm, v0(ctx, c.csr.n, c.b), order=order, reorth="cgs2", rr_dense=dense)
+        r = out.result
+        print(f"  rr_dense={dense}: {r['iterations']} restarts, {r['rr_steps']} RR steps ({r['rr_structured']} structured), "
+              f"host {r['host_seconds'] * 1e3 / max(r['rr_steps'], 1):.2f} ms per step, solve {r['seconds']:.3f} s", flush=True)
+        assert r["converged"] == 1
+        assert out.resid.max() <= c.tol
+        runs[dense] = out
+    s_, d_ = runs[False].result, runs[True].result
+    assert s_["rr_structured"] == s_["rr_steps"] > 0
+    assert d_["rr_structured"] == 0
+    # both FP64 eigensolvers are accurate to eps ||T|| in absolute terms
+    # (||T|| <= ||A|| = 8 here), which for the smallest Ritz value (1.2e-3)
+    # is about 1e-12 relative; the two runs agree to that through the solve
+    diff = np.abs(runs[False].evals - runs[True].evals)
+    print("  max abs eigenvalue difference", diff.max(), "relative", (diff / np.abs(runs[True].evals)).max())
+    assert diff.max() <= 1e-12 * 8
+    if order == "ascending":
+        assert np.max(np.abs(runs[False].evals - c.exact) / c.exact) <= 1e-10
+
+
 def test_matrix_market_file_solve(ctx, tmp_path):
     # the 128^2 Laplacian written as a symmetric Matrix Market file (lower
     # triangle, 1-based) and read back by the library's loader: the solve

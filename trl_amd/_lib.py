@@ -292,11 +292,12 @@ class Context:
 class SolverParams(C.Structure):
     _fields_ = [("b", c_int), ("nev", C.c_uint), ("ncv", C.c_uint), ("max_restarts", C.c_uint), ("tol", C.c_double), ("order", c_int),
                 ("reorth", c_int), ("cholqr_passes", c_int), ("lanes_per_row", c_int),
-                ("spmm_tune", c_int)]
+                ("spmm_tune", c_int), ("rr_dense", c_int)]
 
 
 class SolverResult(C.Structure):
-    _fields_ = [("converged", c_int), ("iterations", C.c_uint), ("n_op_apply", C.c_uint), ("seconds", C.c_double), ("host_seconds", C.c_double)]
+    _fields_ = [("converged", c_int), ("iterations", C.c_uint), ("n_op_apply", C.c_uint), ("seconds", C.c_double), ("host_seconds", C.c_double),
+                ("rr_steps", C.c_uint), ("rr_structured", C.c_uint)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -330,6 +331,7 @@ SOLVER_API = [
                                         C.POINTER(SolverResult)]),
     ("trl_hip_dense_syev", c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_dense_syev_select", c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    ("trl_hip_dense_syev_arrow_band", c_int, [c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_batch_solve_csr", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(BatchParams), c_void_p, c_void_p,
                                         c_void_p, c_void_p, C.POINTER(BatchStats)]),
     ("trl_hip_mm_read", c_int, [C.c_char_p, C.POINTER(c_void_p)]),

@@ -115,6 +115,7 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
   evp->set_operator(op);
   if (p.cholqr_passes > 0) evp->set_cholqr_passes(p.cholqr_passes);
   if (X_out) evp->set_ritz_blocks(nblk_out); // the mixed solve keeps more Ritz blocks than the driver needs
+  evp->set_dense_rayleigh_ritz(p.rr_dense != 0);
   trl_amd::EigensolverParams ep{p.nev, p.ncv, p.max_restarts, p.tol};
 
   auto run = [&](auto& lz) {
@@ -129,6 +130,8 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
     res.iterations = r.iterations;
     res.n_op_apply = r.n_op_apply;
     res.host_seconds = evp->host_seconds();
+    res.rr_steps = evp->rr_steps();
+    res.rr_structured = evp->rr_structured();
     const auto& ev = evp->get_current_eigenvalues();
     for (unsigned i = 0; i < p.nev && i < ev.size(); ++i) evals[i] = ev[i];
     if (all_evals) *all_evals = ev;
@@ -563,6 +566,7 @@ void warm_restart_solve(const Context& c, std::size_t n, const int32_t* rowptr, 
   e32->set_order(static_cast<trl::hip::EigenvalueOrder>(p.order));
   e32->set_operator(op32);
   e32->set_cholqr_passes(2);
+  e32->set_dense_rayleigh_ritz(p.rr_dense != 0);
   trl_amd::BlockLanczos<E32, Reorth> lz32(e32, trl_amd::EigensolverParams{p32.nev, p32.ncv, p32.max_restarts, p32.tol});
   throw_on(trl_hip_memcpy_d2d(c.get(), lz32.initial_block().data_, V0f, P * sizeof(float)), "copy V0");
   c.sync();
@@ -574,12 +578,15 @@ void warm_restart_solve(const Context& c, std::size_t n, const int32_t* rowptr, 
   r32.iterations = q32.iterations;
   r32.n_op_apply = q32.n_op_apply;
   r32.host_seconds = e32->host_seconds();
+  r32.rr_steps = e32->rr_steps();
+  r32.rr_structured = e32->rr_structured();
   const unsigned m32 = p32.ncv / B;
   // ---- the FP64 driver and its seeded state ----------------------------------
   auto e64 = std::make_shared<E64>(c, n, rowptr, col, val64, mass64, minv64);
   e64->set_order(static_cast<trl::hip::EigenvalueOrder>(p.order));
   e64->set_operator(op64);
   if (p.cholqr_passes > 0) e64->set_cholqr_passes(p.cholqr_passes);
+  e64->set_dense_rayleigh_ritz(p.rr_dense != 0);
   trl_amd::BlockLanczos<E64, Reorth> lz64(e64, trl_amd::EigensolverParams{p.nev, p.ncv, p.max_restarts, p.tol});
   auto& V64 = lz64.get_basis();
   auto& V32 = lz32.get_basis();
@@ -667,6 +674,8 @@ void warm_restart_solve(const Context& c, std::size_t n, const int32_t* rowptr, 
   res.iterations = q64.iterations;
   res.n_op_apply += q64.n_op_apply;
   res.host_seconds = e64->host_seconds();
+  res.rr_steps = e64->rr_steps();
+  res.rr_structured = e64->rr_structured();
   (void)t64;
   const auto& ev = e64->get_current_eigenvalues();
   for (unsigned i = 0; i < p.nev && i < ev.size(); ++i) evals[i] = ev[i];
@@ -723,6 +732,16 @@ int trl_hip_dense_syev_select(int n, const double* A, double* w, int k, const in
 {
   if (n < 0 || k < 0 || k > n || (n > 0 && (!A || !w)) || (k > 0 && (!sel || !Z))) return TRL_HIP_EINVAL;
   const int rc = trl_amd::dense::syev_select(n, A, w, k, sel, Z);
+  return rc == 0 ? 0 : (rc == -2 ? TRL_HIP_EINVAL : TRL_HIP_EEIG);
+}
+
+int trl_hip_dense_syev_arrow_band(int n, int b, const double* A, double* w, int k, const int* sel, double* Z, int* p_out)
+{
+  if (n < 0 || b < 1 || k < 0 || k > n || (n > 0 && (!A || !w)) || (k > 0 && (!sel || !Z))) return TRL_HIP_EINVAL;
+  const int p = n > 0 ? trl_amd::dense::arrow_band_structure(n, b, A) : 0;
+  if (p_out) *p_out = p;
+  if (p < 0) return TRL_HIP_EINVAL;
+  const int rc = trl_amd::dense::syev_arrow_band_select(n, b, p, A, w, k, sel, Z);
   return rc == 0 ? 0 : (rc == -2 ? TRL_HIP_EINVAL : TRL_HIP_EEIG);
 }
 

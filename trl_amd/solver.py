@@ -151,13 +151,13 @@ class SolveOutput:
     result32: dict | None = None
 
 
-def _params(b, params: EigensolverParams, order, reorth, cholqr_passes, lanes, tune):
+def _params(b, params: EigensolverParams, order, reorth, cholqr_passes, lanes, tune, rr_dense=False):
     return L.SolverParams(b, params.nev, params.ncv, params.max_restarts, params.tolerance, ORDER.get(order, order), REORTH.get(reorth, reorth),
-                          cholqr_passes, lanes, 1 if tune else 0)
+                          cholqr_passes, lanes, 1 if tune else 0, 1 if rr_dense else 0)
 
 
 def solve(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *, dtype="f64", order="ascending", reorth="cgs2", cholqr_passes=0,
-          lanes_per_row=0, spmm_tune=False, ritz_vectors=True) -> SolveOutput:
+          lanes_per_row=0, spmm_tune=False, ritz_vectors=True, rr_dense=False) -> SolveOutput:
     ctx = A.ctx
     dt = np.float64 if dtype == "f64" else np.float32
     assert V0.dtype == dt and V0.nbytes == A.n * b * np.dtype(dt).itemsize
@@ -166,7 +166,7 @@ def solve(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *,
     evals = np.zeros(params.nev)
     resid = np.zeros(params.nev)
     res = L.SolverResult()
-    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row, spmm_tune)
+    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row, spmm_tune, rr_dense)
     val, mass, minv = A.ptrs(dtype)
     rc = L.solver().trl_hip_solve_csr(ctx.h, L.dtype_code(dt), A.n, A.rowptr.ptr, A.col.ptr, val, mass, minv, C.byref(p), V0.ptr,
                                       evals.ctypes.data, X.ptr if X else None, resid.ctypes.data if X else None, C.byref(res))
@@ -176,7 +176,8 @@ def solve(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *,
 
 def solve_mixed(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArray, *, tol_fp32=1e-5, max_restarts_fp32=1000, seed=7, order="ascending",
                 reorth="cgs2", cholqr_passes=0, lanes_per_row=0, spmm_tune=False, refine=1, cheb_degree=80, max_refine=8, refine_blocks=0,
-                phase1_fp64=False, fp32_sweeps=0, tol_fp32_refine=1e-4, degree_opt=False, residual_filter=False, ncv_fp32=0) -> SolveOutput:
+                phase1_fp64=False, fp32_sweeps=0, tol_fp32_refine=1e-4, degree_opt=False, residual_filter=False, ncv_fp32=0,
+                rr_dense=False) -> SolveOutput:
     ctx = A.ctx
     assert V0.dtype == np.float64
     nblk = (params.nev + b - 1) // b
@@ -185,7 +186,7 @@ def solve_mixed(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArr
     resid = np.zeros(params.nev)
     res = L.SolverResult()
     res32 = L.SolverResult()
-    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row, spmm_tune)
+    p = _params(b, params, order, reorth, cholqr_passes, lanes_per_row, spmm_tune, rr_dense)
     mp = L.MixedParams(tol_fp32, max_restarts_fp32, seed, refine, cheb_degree, max_refine, 1 if phase1_fp64 else 0, refine_blocks, fp32_sweeps,
                        tol_fp32_refine, int(degree_opt), 1 if residual_filter else 0, ncv_fp32)
     v64, m64, mi64 = A.ptrs("f64")
@@ -216,3 +217,19 @@ def dense_syev_select(A: np.ndarray, sel):
     L.check(L.solver().trl_hip_dense_syev_select(n, A.ctypes.data, w.ctypes.data, len(sel), sel.ctypes.data, Z.ctypes.data),
             "trl_hip_dense_syev_select")
     return w, Z
+
+
+def dense_syev_arrow_band(A: np.ndarray, b: int, sel):
+    """The structured eigensolver of the projected matrix (for tests): all
+    eigenvalues, the selected eigenvectors and p, the size of the diagonal
+    Ritz part it found.  Raises TrlHipError when A lacks the arrow-plus-band
+    structure."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    sel = np.ascontiguousarray(sel, dtype=np.int32)
+    w = np.empty(n)
+    Z = np.empty((n, len(sel)))
+    p = C.c_int(-1)
+    L.check(L.solver().trl_hip_dense_syev_arrow_band(n, b, A.ctypes.data, w.ctypes.data, len(sel), sel.ctypes.data, Z.ctypes.data, C.byref(p)),
+            "trl_hip_dense_syev_arrow_band")
+    return w, Z, p.value
