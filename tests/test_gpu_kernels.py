@@ -483,6 +483,66 @@ def test_cholqr2_contract(ctx, rng, p, b):
     assert rel(Xn @ R, X) < (1e-13 if p == "f64" else 1e-6)
 
 
+@pytest.mark.parametrize("n", [1, 100, 3001, 600001])
+@pytest.mark.parametrize("b", [1, 4, 6, 8])
+@pytest.mark.parametrize("mass", [False, True])
+def test_cholqr2_fp64_matches_oracle_twice(ctx, rng, n, b, mass):
+    # FP64 CholQR2 (two Gram-by-rows passes, each factored in its last
+    # workgroup, then the two-step scale) equals the oracle's CholQR applied
+    # twice: R = R2 R1 and X R^{-1}; n = 1 and 100 run on one workgroup,
+    # 600001 on the capped grid.  Two calls on the same input are bitwise equal.
+    print(f"cholqr2 f64 n={n} b={b} mass={mass}", flush=True)
+    X = rng.standard_normal((n, b)) * np.logspace(0, 1, b)
+    m = rng.uniform(0.5, 2, n) if mass else None
+    md = dev(ctx, m, np.float64) if mass else None
+    Rd = ctx.empty((b, b))
+    Xd = dev(ctx, X, np.float64)
+    rc = H().trl_hip_cholqr(ctx.h, L.F64, n, b, Xd.ptr, md.ptr if md else None, Rd.ptr, 2)
+    if n < b:
+        # rank-deficient: the first pass breaks down
+        L.check(rc, "cholqr2")
+        assert ctx.check() > 0
+        return
+    L.check(rc, "cholqr2")
+    assert ctx.check() == 0
+    Xr = X.copy()
+    rc1, R1 = oracle.cholqr(Xr, m)
+    rc2, R2 = oracle.cholqr(Xr, m)
+    assert rc1 == 0 and rc2 == 0
+    assert rel(Rd.download(), R2 @ R1) < 1e-12
+    assert rel(Xd.download(), Xr) < 1e-11
+    X2 = dev(ctx, X, np.float64)
+    R2d = ctx.empty((b, b))
+    L.check(H().trl_hip_cholqr(ctx.h, L.F64, n, b, X2.ptr, md.ptr if md else None, R2d.ptr, 2), "cholqr2 again")
+    assert np.array_equal(X2.download(), Xd.download()) and np.array_equal(R2d.download(), Rd.download())
+
+
+@pytest.mark.parametrize("b", [4, 8])
+def test_cholqr2_fp32_mass_and_breakdown(ctx, rng, b):
+    # FP32 CholQR2 with a mass diagonal: X^T M X = I to FP32 accuracy and
+    # X_old = X_new R; then a zero column: the first pass's pivot is flagged
+    # and X is left unchanged
+    n = 50000
+    X = rng.standard_normal((n, b)).astype(np.float32)
+    m = rng.uniform(0.5, 2, n).astype(np.float32)
+    Xd = dev(ctx, X, np.float32)
+    md = dev(ctx, m, np.float32)
+    Rd = ctx.empty((b, b))
+    L.check(H().trl_hip_cholqr(ctx.h, L.F32, n, b, Xd.ptr, md.ptr, Rd.ptr, 2), "cholqr2")
+    assert ctx.check() == 0
+    Xn = Xd.download().astype(np.float64)
+    R = Rd.download()
+    assert np.abs((Xn * m[:, None]).T @ Xn - np.eye(b)).max() < 1e-6
+    assert rel(Xn @ R, X.astype(np.float64)) < 1e-6
+    Xz = X.copy()
+    Xz[:, 1] = 0.0
+    Xzd = dev(ctx, Xz, np.float32)
+    L.check(H().trl_hip_cholqr(ctx.h, L.F32, n, b, Xzd.ptr, None, Rd.ptr, 2), "cholqr2 breakdown")
+    assert ctx.check() == 2
+    assert np.array_equal(Xzd.download(), Xz)
+    assert ctx.check() == 0
+
+
 def test_cholqr_breakdown_flag(ctx, rng):
     n, b = 500, 4
     X = rng.standard_normal((n, b))

@@ -229,72 +229,65 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ 
   }
 }
 
-// The same factorisation for a compile-time B with every array in
-// registers (fully unrolled): the one-thread runtime-b version below is
-// bound by LDS latency (tens of microseconds at b = 8).
+// Factor G = R^T R (G symmetric B x B, B <= 8, its lower triangle read as
+// an LLT reads it), set Rout = R Rprev (or R) and Rinv = R^{-1}, one thread
+// per column: row k of R is R[k][j] = (G[j][k] - sum_{p<k} R[p][k] R[p][j]) /
+// R[k][k] (the terms subtracted in p order) for all j >= k at once, R kept
+// in LDS (Rs, B x B) between the steps, T (B) the step's partial sums;
+// thread j then forms column j of R^{-1} by back substitution and of
+// R Rprev.  Every thread of the workgroup must call it (it holds workgroup
+// barriers); threads j >= B only take part in them.  On breakdown (a pivot
+// that is not positive) the first failing 1-based pivot goes to flags[0] if
+// it is clear, and Rout and Rinv are set to the identity.
 template <int B>
-__device__ void chol_reg(const double* G, const double* Rprev, double* Rout, double* Rinv, int* flags)
+__device__ void chol_lanes(const double* G, const double* __restrict__ Rprev, double* __restrict__ Rout, double* __restrict__ Rinv, int* __restrict__ flags,
+                           double* Rs, double* T)
 {
-  double L[B][B], X[B][B], g[B][B];
-#pragma unroll
-  for (int i = 0; i < B; ++i)
-#pragma unroll
-    for (int j = 0; j < B; ++j) {
-      L[i][j] = X[i][j] = 0.0;
-      g[i][j] = G[i * B + j];
-    }
+  static_assert(B <= 8, "one thread per column");
+  const int j = threadIdx.x;
+  const bool mine = j < B;
   int fail = 0;
 #pragma unroll
-  for (int j = 0; j < B; ++j) {
-    double d = g[j][j];
+  for (int k = 0; k < B; ++k) {
+    if (mine && j >= k) {
+      double t = G[j * B + k];
 #pragma unroll
-    for (int p = 0; p < j; ++p) d -= L[p][j] * L[p][j];
-    if (!(d > 0.0) && !fail) fail = j + 1;
-    const double rjj = sqrt(d > 0.0 ? d : 1.0);
-    L[j][j] = rjj;
-#pragma unroll
-    for (int i = j + 1; i < B; ++i) {
-      double t = g[i][j];
-#pragma unroll
-      for (int p = 0; p < j; ++p) t -= L[p][j] * L[p][i];
-      L[j][i] = t / rjj;
+      for (int p = 0; p < k; ++p) t -= Rs[p * B + k] * Rs[p * B + j];
+      T[j] = t;
     }
+    __syncthreads();
+    const double d = T[k];
+    if (!(d > 0.0) && !fail) fail = k + 1;
+    const double rkk = sqrt(d > 0.0 ? d : 1.0);
+    if (mine) Rs[k * B + j] = j == k ? rkk : (j > k ? T[j] / rkk : 0.0);
+    __syncthreads();
   }
+  if (!mine) return;
   if (fail) {
-    if (flags[0] == 0) flags[0] = fail;
+    if (j == 0 && flags[0] == 0) flags[0] = fail;
 #pragma unroll
-    for (int i = 0; i < B; ++i)
-#pragma unroll
-      for (int j = 0; j < B; ++j) Rout[i * B + j] = Rinv[i * B + j] = (i == j) ? 1.0 : 0.0;
+    for (int i = 0; i < B; ++i) Rout[i * B + j] = Rinv[i * B + j] = (i == j) ? 1.0 : 0.0;
     return;
   }
+  double xcol[B];
 #pragma unroll
-  for (int j = 0; j < B; ++j) {
-    X[j][j] = 1.0 / L[j][j];
+  for (int i = B - 1; i >= 0; --i) {
+    double t = 0.0;
 #pragma unroll
-    for (int i = j - 1; i >= 0; --i) {
-      double t = 0.0;
-#pragma unroll
-      for (int p = i + 1; p <= j; ++p) t += L[i][p] * X[p][j];
-      X[i][j] = -t / L[i][i];
-    }
+    for (int p = i + 1; p < B; ++p) t += Rs[i * B + p] * xcol[p]; // X[p][j] = 0 for p > j
+    xcol[i] = i > j ? 0.0 : (i == j ? 1.0 / Rs[i * B + i] : -t / Rs[i * B + i]);
   }
 #pragma unroll
-  for (int i = 0; i < B; ++i)
+  for (int i = 0; i < B; ++i) {
+    double v = Rs[i * B + j];
+    if (Rprev) {
+      v = 0.0;
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-      Rinv[i * B + j] = X[i][j];
-      double v = 0.0;
-      if (j >= i) {
-        if (Rprev) {
-#pragma unroll
-          for (int p = i; p <= j; ++p) v += L[i][p] * Rprev[p * B + j];
-        }
-        else
-          v = L[i][j];
-      }
-      Rout[i * B + j] = v;
+      for (int p = i; p < B; ++p) v += Rs[i * B + p] * Rprev[p * B + j];
     }
+    Rinv[i * B + j] = xcol[i];
+    Rout[i * B + j] = i > j ? 0.0 : v;
+  }
 }
 
 // Factor G = R^T R (G symmetric b x b in LDS, lower triangle read as an LLT
@@ -358,6 +351,80 @@ __device__ void chol_bxb(const double* G, int b, const double* Rprev, double* Ro
 //   triangle, B(B+1)/2 entries), reduced per workgroup (shuffles, then LDS);
 //   the last workgroup to finish sums the workgroup partials in fixed order
 //   and factors G on the device (chol_bxb), then re-arms the counter.
+// The end of a one-pass Gram + Cholesky kernel.  g: this thread's packed
+// upper triangle of the Gram contributions (NP = B(B+1)/2 entries).  The
+// workgroup sums them (butterfly inside each wave, then the waves in order)
+// into partials, stored entry-major (entry i of workgroup w at i*G + w); the
+// workgroup that takes the last ticket on *counter then sums every
+// workgroup's partials in a fixed order (thread t: workgroups t, t + 256,
+// ...; then the same butterfly and wave order: deterministic for a given
+// grid), factors G = R^T R in FP64 (R times Rprev into Rout when Rprev is
+// given, this pass's R^{-1} into Rinv_out, a breakdown into flags) and
+// re-arms the counter.  red, Gs, Rl, Xl and ticket are the caller's LDS.
+template <int B>
+__device__ void finish_gram_chol(double (&g)[B * (B + 1) / 2], double* __restrict__ partials, unsigned* __restrict__ counter,
+                                 const double* __restrict__ Rprev, double* __restrict__ Rout, double* __restrict__ Rinv_out, int* __restrict__ flags,
+                                 double (*red)[B * (B + 1) / 2], double* Gs, double (*Rl)[16], double (*Xl)[16], unsigned& ticket)
+{
+  constexpr int NP = B * (B + 1) / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned G = gridDim.x;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    double v = g[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
+    double t = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q][i];
+    partials[size_t(i) * G + blockIdx.x] = t;
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
+  __syncthreads();
+  if (ticket != G - 1) return;
+  __threadfence();
+  // last workgroup: every thread sums the workgroups w = t, t + 256, ... of
+  // all entries (independent loads), then the same butterfly and wave order
+  // as above (fixed order: deterministic for a given grid)
+#pragma unroll
+  for (int i = 0; i < NP; ++i) g[i] = 0.0;
+  for (unsigned w = threadIdx.x; w < G; w += kThreads)
+#pragma unroll
+    for (int i = 0; i < NP; ++i) g[i] += partials[size_t(i) * G + w];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    double v = g[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
+    double t = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q][i];
+    // unpack the upper-triangle index i into (r, c) and mirror
+    int r = 0, rem = i;
+    while (rem >= B - r) {
+      rem -= B - r;
+      ++r;
+    }
+    const int c = r + rem;
+    Gs[r * B + c] = t;
+    Gs[c * B + r] = t;
+  }
+  __syncthreads();
+  if constexpr (B <= 8)
+    chol_lanes<B>(Gs, Rprev, Rout, Rinv_out, flags, &Rl[0][0], &Xl[0][0]);
+  else if (threadIdx.x == 0)
+    chol_bxb(Gs, B, Rprev, Rout, Rinv_out, flags, Rl, Xl);
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
 template <typename S, int B, bool SCALE>
 __global__ __launch_bounds__(kThreads) void k_cholqr_pass(S* __restrict__ W, const S* __restrict__ md, size_t n, const double* __restrict__ Rinv_prev,
                                                           double* __restrict__ partials, unsigned* __restrict__ counter, const double* __restrict__ Rprev,
@@ -426,66 +493,7 @@ __global__ __launch_bounds__(kThreads) void k_cholqr_pass(S* __restrict__ W, con
       }
     }
   }
-  // workgroup reduction: butterfly inside each wave, then waves in order;
-  // partials are stored entry-major (entry i of workgroup w at i*G + w)
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned G = gridDim.x;
-#pragma unroll
-  for (int i = 0; i < NP; ++i) {
-    double v = g[i];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[wave][i] = v;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
-    double t = 0.0;
-    for (int q = 0; q < kThreads / 64; ++q) t += red[q][i];
-    partials[size_t(i) * G + blockIdx.x] = t;
-  }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
-  __syncthreads();
-  if (ticket != G - 1) return;
-  __threadfence();
-  // last workgroup: every thread sums the workgroups w = t, t + 256, ... of
-  // all entries (independent loads), then the same butterfly and wave order
-  // as above (fixed order: deterministic for a given grid)
-#pragma unroll
-  for (int i = 0; i < NP; ++i) g[i] = 0.0;
-  for (unsigned w = threadIdx.x; w < G; w += kThreads)
-#pragma unroll
-    for (int i = 0; i < NP; ++i) g[i] += partials[size_t(i) * G + w];
-#pragma unroll
-  for (int i = 0; i < NP; ++i) {
-    double v = g[i];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[wave][i] = v;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
-    double t = 0.0;
-    for (int q = 0; q < kThreads / 64; ++q) t += red[q][i];
-    // unpack the upper-triangle index i into (r, c) and mirror
-    int r = 0, rem = i;
-    while (rem >= B - r) {
-      rem -= B - r;
-      ++r;
-    }
-    const int c = r + rem;
-    Gs[r * B + c] = t;
-    Gs[c * B + r] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if constexpr (B <= 8)
-      chol_reg<B>(Gs, Rprev, Rout, Rinv_out, flags);
-    else
-      chol_bxb(Gs, B, Rprev, Rout, Rinv_out, flags, Rl, Xl);
-    *counter = 0u;
-  }
+  finish_gram_chol<B>(g, partials, counter, Rprev, Rout, Rinv_out, flags, red, Gs, Rl, Xl, ticket);
 }
 
 // W <- W Rinv (Rinv upper triangular), applied in FP64.
@@ -918,26 +926,59 @@ __global__ void k_chol_small(const double* __restrict__ G, const double* __restr
   __shared__ double Rl[16][16], Xl[16][16];
   for (int i = threadIdx.x; i < B * B; i += blockDim.x) Gs[i] = G[i];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if constexpr (B <= 8)
-      chol_reg<B>(Gs, Rprev, Rout, Rinv, flags);
-    else
-      chol_bxb(Gs, B, Rprev, Rout, Rinv, flags, Rl, Xl);
+  if constexpr (B <= 8)
+    chol_lanes<B>(Gs, Rprev, Rout, Rinv, flags, &Rl[0][0], &Xl[0][0]);
+  else if (threadIdx.x == 0)
+    chol_bxb(Gs, B, Rprev, Rout, Rinv, flags, Rl, Xl);
+}
+
+// Rows of one Gram pass of CholQR2: G = Y^T M Y in FP64 products, Y = W
+// (TRANSFORM false) or Y = W R1^{-1} rounded to the panel type and formed
+// on the fly from W (TRANSFORM: the second pass, Rinv = R1^{-1}).  Each
+// thread loads four rows (grid-stride) before their arithmetic, so four
+// row loads per lane are in flight; it keeps the B(B+1)/2 entries of the
+// upper triangle, and the workgroup sums them (butterfly per wave, waves in
+// order) into its packed partial Gram, entry-major (entry i of workgroup w
+// at i*G + w) for k_chol_partials.
+template <typename S, int B, bool TRANSFORM>
+__device__ inline void gram_row_add(const S (&w)[B], double m, const double* Rs, double (&g)[B * (B + 1) / 2])
+{
+  double y[B];
+  if constexpr (TRANSFORM) {
+    double acc[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) acc[c] = 0.0;
+#pragma unroll
+    for (int cp = 0; cp < B; ++cp) {
+      const double x = static_cast<double>(w[cp]);
+#pragma unroll
+      for (int c = cp; c < B; ++c) acc[c] = fma(x, Rs[cp * B + c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < B; ++c) y[c] = static_cast<double>(static_cast<S>(acc[c]));
+  }
+  else {
+#pragma unroll
+    for (int c = 0; c < B; ++c) y[c] = static_cast<double>(w[c]);
+  }
+  int idx = 0;
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    const double yi = y[i] * m;
+#pragma unroll
+    for (int jj = i; jj < B; ++jj) {
+      g[idx] = fma(yi, y[jj], g[idx]);
+      ++idx;
+    }
   }
 }
 
-// Gram matrix of one panel, one row per lane per step, in FP64:
-// G = Y^T diag(m) Y with Y = W, or (TRANSFORM) Y = W R1^{-1} rounded to the
-// panel type exactly as k_scale stores it -- the Gram that CholQR2's second
-// pass needs, formed without writing Y.  Each lane keeps the B(B+1)/2
-// entries of the upper triangle; the workgroup sums them (butterfly per
-// wave, waves in order) and writes both triangles of its partial Gram,
-// entry-major (entry e of workgroup w at e*G + w) for k_reduce.
 template <typename S, int B, bool TRANSFORM>
 __global__ __launch_bounds__(kThreads) void k_gram_rows(const S* __restrict__ W, const S* __restrict__ md, size_t n, const double* __restrict__ Rinv,
                                                         double* __restrict__ partials)
 {
   constexpr int NP = B * (B + 1) / 2;
+  constexpr int U = 4;
   __shared__ double Rs[TRANSFORM ? B * B : 1];
   __shared__ double red[kThreads / 64][NP];
   if constexpr (TRANSFORM) {
@@ -948,38 +989,21 @@ __global__ __launch_bounds__(kThreads) void k_gram_rows(const S* __restrict__ W,
 #pragma unroll
   for (int i = 0; i < NP; ++i) g[i] = 0.0;
   const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t row = size_t(blockIdx.x) * blockDim.x + threadIdx.x; row < n; row += stride) {
+  size_t row = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; row + (U - 1) * stride < n; row += U * stride) {
+    S w[U][B];
+#pragma unroll
+    for (int u = 0; u < U; ++u) RowIO<S, B>::template load<false>(W + (row + u * stride) * B, w[u]);
+    double mm[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) mm[u] = md ? static_cast<double>(ld<false>(md + row + u * stride)) : 1.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) gram_row_add<S, B, TRANSFORM>(w[u], mm[u], Rs, g);
+  }
+  for (; row < n; row += stride) {
     S w[B];
-    RowIO<S, B>::template load<true>(W + row * B, w);
-    double y[B];
-    if constexpr (TRANSFORM) {
-      double acc[B];
-#pragma unroll
-      for (int c = 0; c < B; ++c) acc[c] = 0.0;
-#pragma unroll
-      for (int cp = 0; cp < B; ++cp) {
-        const double x = static_cast<double>(w[cp]);
-#pragma unroll
-        for (int c = cp; c < B; ++c) acc[c] = fma(x, Rs[cp * B + c], acc[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < B; ++c) y[c] = static_cast<double>(static_cast<S>(acc[c]));
-    }
-    else {
-#pragma unroll
-      for (int c = 0; c < B; ++c) y[c] = static_cast<double>(w[c]);
-    }
-    const double m = md ? static_cast<double>(ld<true>(md + row)) : 1.0;
-    int idx = 0;
-#pragma unroll
-    for (int i = 0; i < B; ++i) {
-      const double yi = md ? y[i] * m : y[i];
-#pragma unroll
-      for (int j = i; j < B; ++j) {
-        g[idx] = fma(yi, y[j], g[idx]);
-        ++idx;
-      }
-    }
+    RowIO<S, B>::template load<false>(W + row * B, w);
+    gram_row_add<S, B, TRANSFORM>(w, md ? static_cast<double>(ld<false>(md + row)) : 1.0, Rs, g);
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -991,14 +1015,55 @@ __global__ __launch_bounds__(kThreads) void k_gram_rows(const S* __restrict__ W,
   }
   __syncthreads();
   const unsigned G = gridDim.x;
-  for (int e = threadIdx.x; e < B * B; e += blockDim.x) {
-    const int i = e / B, j = e % B;
-    const int a = i < j ? i : j, b2 = i < j ? j : i;
-    const int idx = a * B - a * (a - 1) / 2 + (b2 - a); // packed upper-triangle index of (a, b2)
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) {
     double t = 0.0;
-    for (int q = 0; q < kThreads / 64; ++q) t += red[q][idx];
-    partials[size_t(e) * G + blockIdx.x] = t;
+    for (int q = 0; q < kThreads / 64; ++q) t += red[q][i];
+    partials[size_t(i) * G + blockIdx.x] = t;
   }
+}
+
+// The end of a Gram-by-rows pass: one workgroup of kReduceThreads sums the
+// G workgroups' packed partials -- wave q takes the entries i = q, q + 16,
+// ..., each lane the workgroups w = lane, lane + 64, ... of an entry, then a
+// butterfly (a fixed order) --, mirrors the sums into the b x b Gram and
+// factors it (chol_lanes): Rout = R Rprev (or R), Rinv = R^{-1}, a
+// breakdown into flags.
+constexpr int kReduceThreads = 1024;
+
+template <int B>
+__global__ __launch_bounds__(kReduceThreads) void k_chol_partials(const double* __restrict__ partials, int G, const double* __restrict__ Rprev,
+                                                                  double* __restrict__ Rout, double* __restrict__ Rinv, int* __restrict__ flags)
+{
+  static_assert(B <= 8, "chol_lanes");
+  constexpr int NP = B * (B + 1) / 2;
+  constexpr int NW = kReduceThreads / 64;
+  __shared__ double Gs[B * B], Rsh[B * B], T[B];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = wave; i < NP; i += NW) {
+    const double* src = partials + size_t(i) * G;
+    double a0 = 0.0, a1 = 0.0;
+    int w = lane;
+    for (; w + 64 < G; w += 128) {
+      a0 += src[w];
+      a1 += src[w + 64];
+    }
+    if (w < G) a0 += src[w];
+    double x = a0 + a1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) {
+      int r = 0, rem = i;
+      while (rem >= B - r) {
+        rem -= B - r;
+        ++r;
+      }
+      const int c = r + rem;
+      Gs[r * B + c] = x;
+      Gs[c * B + r] = x;
+    }
+  }
+  __syncthreads();
+  chol_lanes<B>(Gs, Rprev, Rout, Rinv, flags, Rsh, T);
 }
 
 // W <- (W R1^{-1}) R2^{-1} with the intermediate rounded to the panel type:
@@ -1034,57 +1099,66 @@ __global__ __launch_bounds__(kThreads) void k_scale2(S* __restrict__ W, const do
   }
 }
 
-// Cholesky-QR with the Gram matrix on MFMA (the block-product kernel on W
-// against itself), a one-thread FP64 Cholesky and the FP64 scale; one panel
-// read for the Gram and one read + write for the scale per pass.
-// CholQR2 in four panel passes: G1 = W^T M W (read W), R1; G2 = Y^T M Y for
-// Y = W R1^{-1} formed on the fly (read W), R2 and R = R2 R1; W <- Y R2^{-1}
-// (read and write W) -- where the two-pass form below reads W three times
-// and writes it twice.  Thread-per-row FP64 Gram kernels; per-workgroup
-// partials summed by k_reduce; the b x b Cholesky in FP64 on the device.
-int launch_cholqr2_rows(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R)
+// Cholesky-QR for b <= 8 by rows.  CholQR (passes = 1) in two panel
+// passes: G = W^T M W (read W), R; W <- W R^{-1} (read and write W).
+// CholQR2 in four: G1 = W^T M W (read W), R1; G2 = Y^T M Y for Y = W R1^{-1}
+// formed on the fly (read W), R2 and R = R2 R1; W <- Y R2^{-1} (read and
+// write W) -- where two CholQR passes read W three times and write it
+// twice.  Per Gram a thread-per-row FP64 kernel and one workgroup that sums
+// its partials and factors the b x b Gram (k_chol_partials).
+int launch_cholqr_rows(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes)
 {
   double* R1 = ctx->small;
   double* Rinv1 = ctx->small + 256;
   double* Rinv2 = ctx->small + 512;
-  double* G = ctx->small + 768;
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
-      S* w = static_cast<S*>(W);
-      const S* m = static_cast<const S*>(md);
-      size_t g = (n + kThreads - 1) / kThreads;
-      const size_t gmax = size_t(4) * ctx->num_cus;
-      if (g > gmax) g = gmax;
-      if (g < 1) g = 1;
-      int rc = ensure_partials(ctx, g * B * B);
-      if (rc) return rc;
-      const dim3 grid{unsigned(g)}, blk{unsigned(kThreads)};
-      hipLaunchKernelGGL((k_gram_rows<S, B, false>), grid, blk, 0, ctx->stream, w, m, n, nullptr, ctx->partials);
-      TRL_HIP_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_reduce, dim3(unsigned(B * B)), dim3(kThreads), 0, ctx->stream, ctx->partials, int(g), B * B, G, nullptr);
-      TRL_HIP_LAUNCH_CHECK();
-      hipLaunchKernelGGL((k_chol_small<B>), dim3(1), dim3(64), 0, ctx->stream, G, nullptr, R1, Rinv1, ctx->flags);
-      TRL_HIP_LAUNCH_CHECK();
-      hipLaunchKernelGGL((k_gram_rows<S, B, true>), grid, blk, 0, ctx->stream, w, m, n, Rinv1, ctx->partials);
-      TRL_HIP_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_reduce, dim3(unsigned(B * B)), dim3(kThreads), 0, ctx->stream, ctx->partials, int(g), B * B, G, nullptr);
-      TRL_HIP_LAUNCH_CHECK();
-      hipLaunchKernelGGL((k_chol_small<B>), dim3(1), dim3(64), 0, ctx->stream, G, R1, R, Rinv2, ctx->flags);
-      TRL_HIP_LAUNCH_CHECK();
-      if (n == 0) return 0;
-      size_t gs = (n + kThreads - 1) / kThreads;
-      const size_t gsmax = size_t(8) * ctx->num_cus;
-      if (gs > gsmax) gs = gsmax;
-      hipLaunchKernelGGL((k_scale2<S, B>), dim3(unsigned(gs)), dim3(kThreads), 0, ctx->stream, w, Rinv1, Rinv2, n);
-      TRL_HIP_LAUNCH_CHECK();
+      if constexpr (B <= 8) {
+        S* w = static_cast<S*>(W);
+        const S* m = static_cast<const S*>(md);
+        size_t g = (n + kThreads - 1) / kThreads;
+        const size_t gmax = size_t(2) * ctx->num_cus;
+        if (g > gmax) g = gmax;
+        if (g < 1) g = 1;
+        constexpr int NP = B * (B + 1) / 2;
+        int rc = ensure_partials(ctx, g * NP);
+        if (rc) return rc;
+        const dim3 grid{unsigned(g)}, blk{unsigned(kThreads)};
+        hipLaunchKernelGGL((k_gram_rows<S, B, false>), grid, blk, 0, ctx->stream, w, m, n, nullptr, ctx->partials);
+        TRL_HIP_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_chol_partials<B>), dim3(1), dim3(kReduceThreads), 0, ctx->stream, ctx->partials, int(g), nullptr, passes == 1 ? R : R1,
+                           Rinv1, ctx->flags);
+        TRL_HIP_LAUNCH_CHECK();
+        if (passes == 2) {
+          hipLaunchKernelGGL((k_gram_rows<S, B, true>), grid, blk, 0, ctx->stream, w, m, n, Rinv1, ctx->partials);
+          TRL_HIP_LAUNCH_CHECK();
+          hipLaunchKernelGGL((k_chol_partials<B>), dim3(1), dim3(kReduceThreads), 0, ctx->stream, ctx->partials, int(g), R1, R, Rinv2, ctx->flags);
+          TRL_HIP_LAUNCH_CHECK();
+        }
+        if (n == 0) return 0;
+        size_t gs = (n + kThreads - 1) / kThreads;
+        const size_t gsmax = size_t(8) * ctx->num_cus;
+        if (gs > gsmax) gs = gsmax;
+        if (passes == 2)
+          hipLaunchKernelGGL((k_scale2<S, B>), dim3(unsigned(gs)), dim3(kThreads), 0, ctx->stream, w, Rinv1, Rinv2, n);
+        else
+          hipLaunchKernelGGL((k_scale<S, B>), dim3(unsigned(gs)), dim3(kThreads), 0, ctx->stream, w, Rinv1, n);
+        TRL_HIP_LAUNCH_CHECK();
+      }
+      else
+        return TRL_HIP_EINVAL;
     });
   });
   return 0;
 }
 
+// Cholesky-QR with the Gram matrix on MFMA (the block-product kernel on W
+// against itself), a one-thread FP64 Cholesky and the FP64 scale; one panel
+// read for the Gram and one read + write for the scale per pass.  Blocks of
+// b <= 8 take the rows form above.
 int launch_cholqr_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes)
 {
-  if (passes == 2 && b <= 8) return launch_cholqr2_rows(ctx, t, n, b, W, md, R);
+  if (b <= 8 && (passes == 1 || passes == 2)) return launch_cholqr_rows(ctx, t, n, b, W, md, R, passes);
   double* R1 = ctx->small;
   double* Rinv = ctx->small + 256;
   double* G = ctx->small + 768;
