@@ -146,7 +146,7 @@ public:
   // ---- fused block products (used by BlockCGS2 and trl_amd::BlockLanczos) --
 
   // C = V[0:count]^T M W, C a device (count*b x b) FP64 array owned by this EVP.
-  double* gram_blocks(const BlockMultivector& V, unsigned count, const BlockView& W)
+  virtual double* gram_blocks(const BlockMultivector& V, unsigned count, const BlockView& W)
   {
     double* C = coef(std::size_t(count) * blocksize * blocksize);
     throw_on(trl_hip_gram(ctx_.get(), dtype, N, blocksize, int(count), V.data(), W.data_, mass_, C), "gram_blocks");
@@ -157,7 +157,7 @@ public:
   // classical Gram-Schmidt: two passes (eta <= 0), or a second pass only where
   // the first one cancelled (the DGKS test with threshold eta), decided on the
   // device.
-  void reorth_blocks(const BlockMultivector& V, unsigned count, const BlockView& W, double eta)
+  virtual void reorth_blocks(const BlockMultivector& V, unsigned count, const BlockView& W, double eta)
   {
     double* C = coef(std::size_t(2 * count + 1) * blocksize * blocksize);
     throw_on(trl_hip_reorth(ctx_.get(), dtype, N, blocksize, int(count), V.data(), W.data_, mass_, C, eta), "reorth_blocks");

@@ -43,6 +43,8 @@ extern "C" {
 #define TRL_HIP_ERANGE (-5) /* block index out of range (multivector.hh:87)           */
 #define TRL_HIP_EBARRIER (-6) /* a batched step's device barrier timed out             */
 #define TRL_HIP_EIO (-7)      /* a file cannot be opened or parsed                      */
+#define TRL_HIP_ENOCONV (-8)  /* an inner iterative solve missed its tolerance          */
+#define TRL_HIP_EINDEF (-9)   /* an operator that must be SPD gave p^T K p <= 0         */
 
 typedef enum { TRL_HIP_F32 = 0, TRL_HIP_F64 = 1 } trl_hip_dtype;
 
@@ -159,6 +161,14 @@ int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, co
  * also yields W^T M W in the same sweep.  C: device scratch of
  * (2k+1)*b*b doubles. */
 int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, void* W, const void* mdiag, double* C, double eta);
+
+/* CholQR in the inner product of a sparse SPD matrix B: the generalized
+ * problem's orthonormalize (EVPBase::orthonormalize, evp_base.hh:39-71, with
+ * the B-dot of examples/openmp/csrevp.hh:89-97), given BW = B W (n x b,
+ * device, not overlapping W).  Per pass G = W^T BW, G = R^T R (FP64,
+ * breakdown as trl_hip_cholqr), W <- W R^{-1} and BW <- BW R^{-1}; with
+ * passes = 2 the returned R is R2 R1. */
+int trl_hip_cholqr_bprod(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, void* BW, double* R, int passes);
 
 /* ---- operator ------------------------------------------------------------ */
 
@@ -277,6 +287,24 @@ int trl_hip_spmm_tune_axpby(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, 
  * (trl_hip_spmm_axpby_panels with Z = Y); *best_us is per launch. */
 int trl_hip_spmm_tune_panels(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int npanels, const trl_hip_spmat* csr, const trl_hip_spmat* ell,
                              trl_hip_spmat* best, double* best_us);
+
+/* ---- shift-invert inner solve ---------------------------------------------- */
+
+/* X = K^{-1} rhs for b right-hand sides (n x b panels, FP64) with K = A -
+ * sigma B symmetric positive definite: the solve of the reference's
+ * shift-invert apply (examples/openmp/csrevp.hh:76-84, Eigen SparseLU there)
+ * by the Jacobi-preconditioned conjugate gradient method, one recurrence per
+ * column, on the device (K any trl_hip_spmat form, dinv = 1/diag(K), n
+ * doubles).  Stops when every column's recurrence residual satisfies
+ * |r| <= tol |rhs|, or after maxit iterations (TRL_HIP_ENOCONV).
+ * TRL_HIP_EINDEF when a search direction gives p^T K p <= 0.  work: device
+ * scratch of trl_hip_pcg_workspace(n, b) bytes.  Synchronous (the host reads
+ * the residuals every 8 iterations); *iters and *relres (max over columns of
+ * |r| / |rhs|) may be NULL.  No reference counterpart for the iteration
+ * itself (the reference factorises). */
+size_t trl_hip_pcg_workspace(size_t n, int b);
+int trl_hip_pcg(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* K, const void* dinv, const void* rhs, void* X, double tol,
+                int maxit, void* work, int* iters, double* relres);
 
 /* ---- small b x b blocks (MatrixBlockViewConcept, concepts.hh:26-39) ----- */
 

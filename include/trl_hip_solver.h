@@ -131,6 +131,34 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
                             const trl_hip_mixed_params* mp, const double* V0, double* evals, double* X, double* resid, trl_hip_solver_result* res,
                             trl_hip_solver_result* res32);
 
+/* ---- generalized problems by shift-invert ----------------------------------- */
+
+typedef struct {
+  unsigned solves;              /* inner solves (one per operator apply)            */
+  unsigned long long iterations; /* conjugate gradient iterations over all solves   */
+  unsigned max_iterations;      /* the most any one solve took                      */
+} trl_hip_inner_stats;
+
+/* A x = lambda B x with sparse symmetric A and sparse SPD B by shift-invert:
+ * the reference's CSRGeneralizedEVP driven by BlockLanczos
+ * (examples/openmp/csrevp.hh:51-104, examples/openmp/gevp_test.cc:62-116).
+ * The driver runs on (A - shift B)^{-1} B in the B-inner product; its Ritz
+ * values mu = 1/(lambda - shift) (order as p->order, |mu| descending by
+ * default: the lambda nearest the shift) are returned mapped back,
+ * evals[i] = shift + 1/mu[i] (gevp_test.cc:135-141), mu optional.
+ * A, B: host CSR arrays (full symmetric storage, columns ascending per row),
+ * FP64.  The inner solve is the device PCG of trl_hip_pcg at relative
+ * tolerance inner_tol (0: 1e-12), so A - shift B must be positive definite
+ * (TRL_HIP_EINDEF otherwise; TRL_HIP_ENOCONV if an inner solve stalls).
+ * p->reorth 0 = ModifiedGS through the B-dot, 1 or 2 = block CGS2 in the
+ * B-inner product.  V0: device n x b.  X (optional, device): ceil(nev/b)
+ * panels of B-orthonormal Ritz vectors.  resid (optional, host): nev true
+ * residuals |A x - lambda B x| / |A x|.  inner (optional): PCG counts. */
+int trl_hip_solve_csr_shift_invert(trl_hip_ctx* ctx, size_t n, const int32_t* a_rowptr, const int32_t* a_col, const double* a_val,
+                                   const int32_t* b_rowptr, const int32_t* b_col, const double* b_val, double shift, double inner_tol,
+                                   const trl_hip_solver_params* p, const double* V0, double* evals, double* mu, double* X, double* resid,
+                                   trl_hip_solver_result* res, trl_hip_inner_stats* inner);
+
 /* ---- many independent problems in lock-step (no reference counterpart) ---- */
 
 typedef struct {

@@ -9,7 +9,10 @@
 //    DRIVER = trl::BlockLanczos (include/trl/eigensolvers/lanczos.hh), with
 //    the default trl::ModifiedGS and with trl::hip::BlockCGS2, and the 2D
 //    Laplacian 128^2 (8 smallest, ncv = 256) against its closed form and the
-//    CPU oracle.
+//    CPU oracle; and the generalized problem with a sparse mass matrix by
+//    shift-invert (trl::hip::CSRGeneralizedEVP, the reference's
+//    examples/openmp/csrevp.hh:51-104) against the closed form of the
+//    bilinear finite-element Laplacian pair.
 #include <algorithm>
 #include <cstdint>
 #include <vector>
@@ -31,6 +34,9 @@ static_assert(trl::BlockMultiVector<E64::BlockMultivector>);
 static_assert(trl::MatrixBlockViewConcept<E64::BlockMatrixBlockView>);
 static_assert(trl::ReorthogonalizationStrategy<trl::ModifiedGS, E64, E64::BlockMultivector>);
 static_assert(trl::ReorthogonalizationStrategy<trl::hip::BlockCGS2, E64, E64::BlockMultivector>);
+using EG = trl::hip::CSRGeneralizedEVP<4>;
+static_assert(trl::Eigenproblem<EG>);
+static_assert(trl::ReorthogonalizationStrategy<trl::ModifiedGS, EG, EG::BlockMultivector>);
 
 // the reference driver with the fused block-CGS2 strategy
 template <unsigned bs>
@@ -123,6 +129,62 @@ bool check_laplace2d_128(const trl::hip::Context& ctx)
   return ok;
 }
 
+// K x = lambda M x for bilinear finite elements on the unit square (nx x ny
+// interior nodes, Dirichlet): K = K1x (x) M1y + M1x (x) K1y, M = M1x (x) M1y,
+// K1 = tridiag(-1, 2, -1) / h, M1 = h tridiag(1, 4, 1) / 6.  Shift 0, the 8
+// eigenvalues nearest it (|mu| largest, the reference's order) by the
+// UNCHANGED trl::BlockLanczos on (K - 0 M)^{-1} M in the M-inner product,
+// mapped back by lambda = 1/mu (examples/openmp/gevp_test.cc:135-141),
+// against lambda_k + lambda_l, lambda_k = (6/h^2)(1 - cos t_k)/(2 + cos t_k).
+bool check_shift_invert(const trl::hip::Context& ctx)
+{
+  constexpr unsigned bs = 4;
+  const int nx = 48, ny = 40;
+  const std::size_t N = std::size_t(nx) * ny;
+  const double hx = 1.0 / (nx + 1), hy = 1.0 / (ny + 1);
+  auto k1 = [](int d, double h) { return (d == 0 ? 2.0 : -1.0) / h; };
+  auto m1 = [](int d, double h) { return h * (d == 0 ? 4.0 : 1.0) / 6.0; };
+  std::vector<int32_t> rp{0}, ci;
+  std::vector<double> kv, mv;
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (x + dx < 0 || x + dx >= nx || y + dy < 0 || y + dy >= ny) continue;
+          ci.push_back((y + dy) * nx + x + dx);
+          kv.push_back(k1(dx, hx) * m1(dy, hy) + m1(dx, hx) * k1(dy, hy));
+          mv.push_back(m1(dx, hx) * m1(dy, hy));
+        }
+      rp.push_back(int32_t(ci.size()));
+    }
+  auto evp = std::make_shared<trl::hip::CSRGeneralizedEVP<bs>>(ctx, N, rp, ci, kv, rp, ci, mv, 0.0);
+  HipTestHelper<trl::hip::CSRGeneralizedEVP<bs>> h(ctx);
+  const unsigned nev = 8;
+  trl::EigensolverParams params{.nev = nev, .ncv = 64, .max_restarts = 1000, .tolerance = 1e-10};
+  trl::BlockLanczos<trl::hip::CSRGeneralizedEVP<bs>> lz(evp, params);
+  h.set_random(lz.initial_block());
+  const auto res = lz.solve();
+  h.sync();
+  std::vector<double> lam = evp->back_transformed();
+  lam.resize(nev);
+  std::sort(lam.begin(), lam.end());
+  std::vector<double> exact;
+  const double pi = std::acos(-1.0);
+  auto one = [&](int k, int m, double hh) {
+    const double t = k * pi / (m + 1);
+    return 6.0 / (hh * hh) * (1.0 - std::cos(t)) / (2.0 + std::cos(t));
+  };
+  for (int k = 1; k <= nx; ++k)
+    for (int l = 1; l <= ny; ++l) exact.push_back(one(k, nx, hx) + one(l, ny, hy));
+  std::sort(exact.begin(), exact.end());
+  double worst = 0.0;
+  for (unsigned i = 0; i < nev; ++i) worst = std::max(worst, std::abs(lam[i] - exact[i]) / exact[i]);
+  const bool ok = res.converged && worst < 1e-10;
+  std::printf("reference driver, shift-invert FE pair 48x40 (sparse mass) b=4 nev=8: %u restarts, %u inner solves, max rel err vs closed form %.2e -> %s\n",
+              res.iterations, evp->inner_solves(), worst, ok ? "ok" : "FAILED");
+  return ok;
+}
+
 int main()
 {
   trl::hip::Context ctx(0);
@@ -140,6 +202,7 @@ int main()
   failed += !check_cgs2<4>(ctx);
   failed += !check_cgs2<8>(ctx);
   failed += !check_laplace2d_128(ctx);
+  failed += !check_shift_invert(ctx);
   std::printf("reference driver on trl::hip: %s\n", failed ? "FAILED" : "ALL PASSED");
   return failed;
 }

@@ -16,7 +16,7 @@ ROOT = Path(__file__).resolve().parent.parent
 def declared(header: str) -> list[str]:
     text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*|void|const char\s*\*)\s*(trl_hip_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*|void|const char\s*\*|size_t)\s*(trl_hip_\w+)\s*\(", text, flags=re.M)))
 
 
 @pytest.mark.parametrize("header,loader", [("trl_hip.h", L.hip), ("trl_hip_solver.h", L.solver)])

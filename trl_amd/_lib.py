@@ -63,6 +63,9 @@ HIP_API = [
     ("trl_hip_gram_noalloc", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_update", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_cholqr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
+    ("trl_hip_cholqr_bprod", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
+    ("trl_hip_pcg_workspace", c_size, [c_size, c_int]),
+    ("trl_hip_pcg", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_void_p, c_void_p, c_void_p, C.c_double, c_int, c_void_p, c_int_p, c_double_p]),
     ("trl_hip_rotate", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("trl_hip_spmm", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_void_p, c_void_p]),
     ("trl_hip_spmm_axpby", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_void_p, c_void_p, c_void_p, C.c_double, C.c_double, C.c_double]),
@@ -309,6 +312,15 @@ class MixedParams(C.Structure):
                 ("degree_opt", c_int), ("residual_filter", c_int), ("ncv_fp32", C.c_uint)]
 
 
+class InnerStats(C.Structure):
+    """trl_hip_inner_stats: the shift-invert solve's conjugate-gradient counts."""
+
+    _fields_ = [("solves", C.c_uint), ("iterations", C.c_ulonglong), ("max_iterations", C.c_uint)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class BatchParams(C.Structure):
     _fields_ = [("b", c_int), ("nev", C.c_uint), ("ncv", C.c_uint), ("max_restarts", C.c_uint), ("tol", C.c_double), ("order", c_int),
                 ("cholqr_passes", c_int), ("tune", c_int), ("use_graph", c_int), ("reorth", c_int)]
@@ -329,6 +341,9 @@ SOLVER_API = [
     ("trl_hip_solve_csr_mixed", c_int, [c_void_p, c_size, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         C.POINTER(SolverParams), C.POINTER(MixedParams), c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(SolverResult),
                                         C.POINTER(SolverResult)]),
+    ("trl_hip_solve_csr_shift_invert", c_int, [c_void_p, c_size, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.c_double, C.c_double,
+                                               C.POINTER(SolverParams), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(SolverResult),
+                                               C.POINTER(InnerStats)]),
     ("trl_hip_dense_syev", c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_dense_syev_select", c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("trl_hip_dense_syev_arrow_band", c_int, [c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),

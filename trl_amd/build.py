@@ -50,7 +50,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++20", "-Wall", "-Wextra", f"-I{INCLUDE}", f"-I{CSRC}", "-fopenmp"]
 
-KERNEL_SOURCES = ["capi.hip", "k_panel.hip", "k_block.hip", "k_spmm.hip", "k_extra.hip", "k_batch.hip"]
+KERNEL_SOURCES = ["capi.hip", "k_panel.hip", "k_block.hip", "k_spmm.hip", "k_extra.hip", "k_batch.hip", "k_solve.hip"]
 SOLVER_SOURCES = ["solver.cc", "dense_eigen.cc", "batch_solver.cc", "mmio.cc"]
 
 

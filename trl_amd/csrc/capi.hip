@@ -118,6 +118,10 @@ const char* trl_hip_strerror(int code)
     return "block index out of range";
   case TRL_HIP_EBARRIER:
     return "device barrier of a batched step timed out";
+  case TRL_HIP_ENOCONV:
+    return "inner solve did not converge";
+  case TRL_HIP_EINDEF:
+    return "shifted operator is not positive definite";
   case TRL_HIP_EIO:
     return "cannot open or parse the file";
   default:
@@ -397,6 +401,13 @@ int trl_hip_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, 
   if (!ctx || !valid_b(b) || !R || passes < 1 || passes > 2 || (n && !W)) return TRL_HIP_EINVAL;
   Prof pr(ctx, TRL_HIP_PROF_CHOLQR, passes * (3 * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0)), t);
   return launch_cholqr(ctx, t, n, b, W, mdiag, R, passes);
+}
+
+int trl_hip_cholqr_bprod(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, void* BW, double* R, int passes)
+{
+  if (!ctx || !valid_b(b) || !R || passes < 1 || passes > 2 || (n && (!W || !BW)) || (n && W == BW)) return TRL_HIP_EINVAL;
+  Prof pr(ctx, TRL_HIP_PROF_CHOLQR, passes * 6 * panel_bytes(t, n, b), t);
+  return launch_cholqr_bprod(ctx, t, n, b, W, BW, R, passes);
 }
 
 int trl_hip_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int y_block_cols, int kc, void* Out)
@@ -709,6 +720,20 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
 } // namespace
 
 extern "C" {
+
+size_t trl_hip_pcg_workspace(size_t n, int b) { return valid_b(b) ? pcg_workspace_bytes(n, b) : 0; }
+
+int trl_hip_pcg(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* K, const void* dinv, const void* rhs, void* X, double tol,
+                int maxit, void* work, int* iters, double* relres)
+{
+  if (iters) *iters = 0;
+  if (relres) *relres = 0.0;
+  if (!ctx || t != TRL_HIP_F64 || !valid_b(b) || check_spmat(n, K) || !(tol > 0.0) || maxit < 0 || (n && (!dinv || !rhs || !X || !work)) || rhs == X)
+    return TRL_HIP_EINVAL;
+  if (!n) return 0;
+  return launch_pcg(ctx, n, b, *K, static_cast<const double*>(dinv), static_cast<const double*>(rhs), static_cast<double*>(X), tol, maxit, work, iters,
+                    relres);
+}
 
 int trl_hip_spmm_tune(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, trl_hip_spmat* best,
                       double* best_us)

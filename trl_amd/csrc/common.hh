@@ -86,6 +86,12 @@ int noalloc_scratch(int device, void** out);
 int launch_update(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, const void* V, const double* C, void* W, const int* gate = nullptr);
 int launch_dgks(trl_hip_ctx* ctx, int b, int k, const double* C, double eta2, int* gate);
 int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes);
+// CholQR in the B-inner product with BW = B W given; BW is scaled with W
+int launch_cholqr_bprod(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, void* BW, double* R, int passes);
+// the shift-invert inner solver (FP64): workspace bytes, and the solve
+size_t pcg_workspace_bytes(size_t n, int b);
+int launch_pcg(trl_hip_ctx* ctx, size_t n, int b, const trl_hip_spmat& K, const double* dinv, const double* rhs, double* X, double tol, int maxit,
+               void* work, int* iters, double* relres);
 int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int ybc, int kc, void* Out);
 bool valid_spmm_variant(int v);
 // F, g (both or neither, axpby only): the forcing term + F_p diag(g_p), F

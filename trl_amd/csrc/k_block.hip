@@ -1183,6 +1183,36 @@ int launch_cholqr_gram(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void*
   return 0;
 }
 
+// Cholesky-QR in the inner product of a sparse SPD B, given BW = B W: per
+// pass G = W^T (BW) on MFMA, the FP64 Cholesky, and W, BW <- W R^{-1},
+// BW R^{-1} (so BW stays B times the new W without another product).
+int launch_cholqr_bprod(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, void* BW, double* R, int passes)
+{
+  double* R1 = ctx->small;
+  double* Rinv = ctx->small + 256;
+  double* G = ctx->small + 768;
+  for (int pass = 0; pass < passes; ++pass) {
+    int rc = launch_gram(ctx, t, n, b, 1, W, BW, nullptr, G);
+    if (rc) return rc;
+    const bool last = pass == passes - 1;
+    TRL_DISPATCH_T(t, {
+      TRL_DISPATCH_B(b, {
+        hipLaunchKernelGGL((k_chol_small<B>), dim3(1), dim3(64), 0, ctx->stream, G, pass == 0 ? nullptr : R1, last ? R : R1, Rinv, ctx->flags);
+        TRL_HIP_LAUNCH_CHECK();
+        if (n == 0) return 0;
+        size_t gs = (n + kThreads - 1) / kThreads;
+        const size_t gsmax = size_t(8) * ctx->num_cus;
+        if (gs > gsmax) gs = gsmax;
+        hipLaunchKernelGGL((k_scale<S, B>), dim3(unsigned(gs)), dim3(kThreads), 0, ctx->stream, static_cast<S*>(W), Rinv, n);
+        TRL_HIP_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_scale<S, B>), dim3(unsigned(gs)), dim3(kThreads), 0, ctx->stream, static_cast<S*>(BW), Rinv, n);
+        TRL_HIP_LAUNCH_CHECK();
+      });
+    });
+  }
+  return 0;
+}
+
 int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, const void* md, double* R, int passes)
 {
   // default: the Gram on MFMA (launch_cholqr_gram), measured faster than the

@@ -33,6 +33,8 @@ int code_of(const std::exception& e)
   if (w.find("Cholesky") != std::string::npos) return TRL_HIP_ECHOL;
   if (w.find("Eigendecomposition") != std::string::npos) return TRL_HIP_EEIG;
   if (w.find("allocation") != std::string::npos) return TRL_HIP_ENOMEM;
+  if (w.find(trl_hip_strerror(TRL_HIP_ENOCONV)) != std::string::npos) return TRL_HIP_ENOCONV;
+  if (w.find(trl_hip_strerror(TRL_HIP_EINDEF)) != std::string::npos) return TRL_HIP_EINDEF;
   return TRL_HIP_EINVAL;
 }
 
@@ -718,6 +720,92 @@ int dispatch_b(int b, F&& f)
   }
 }
 
+// The shift-invert solve of A x = lambda B x (examples/openmp/gevp_test.cc:
+// 62-116 with CSRGeneralizedEVP): the restated driver on
+// trl::hip::CSRGeneralizedEVP, Ritz values mu mapped back to lambda = sigma +
+// 1/mu, Ritz vectors X = V Y, true residuals |A x - lambda B x| / |A x| with
+// A uploaded for them.
+template <unsigned B>
+void solve_shift_invert_typed(const Context& c, std::size_t n, const std::vector<int32_t>& arp, const std::vector<int32_t>& acol,
+                              const std::vector<double>& aval, const std::vector<int32_t>& brp, const std::vector<int32_t>& bcol,
+                              const std::vector<double>& bval, double shift, double inner_tol, const trl_hip_solver_params& p, const double* V0,
+                              double* evals, double* mu, double* X_out, double* resid, trl_hip_solver_result& res, trl_hip_inner_stats* st)
+{
+  using EVP = trl::hip::CSRGeneralizedEVP<B>;
+  auto evp = std::make_shared<EVP>(c, n, arp, acol, aval, brp, bcol, bval, shift);
+  evp->set_order(static_cast<trl::hip::EigenvalueOrder>(p.order));
+  if (p.cholqr_passes > 0) evp->set_cholqr_passes(p.cholqr_passes);
+  if (inner_tol > 0.0) evp->set_inner_tolerance(inner_tol);
+  evp->set_dense_rayleigh_ritz(p.rr_dense != 0);
+  const unsigned nblk = (p.nev + B - 1) / B;
+  evp->set_ritz_blocks(nblk);
+  trl_amd::EigensolverParams ep{p.nev, p.ncv, p.max_restarts, p.tol};
+  auto run = [&](auto& lz) {
+    auto V0b = lz.initial_block();
+    throw_on(trl_hip_memcpy_d2d(c.get(), V0b.data_, V0, n * B * sizeof(double)), "copy V0");
+    c.sync();
+    const double t0 = now();
+    const auto r = lz.solve();
+    c.sync();
+    res.seconds = now() - t0;
+    res.converged = r.converged ? 1 : 0;
+    res.iterations = r.iterations;
+    res.n_op_apply = r.n_op_apply;
+    res.host_seconds = evp->host_seconds();
+    res.rr_steps = evp->rr_steps();
+    res.rr_structured = evp->rr_structured();
+    const auto& ev = evp->get_current_eigenvalues();
+    for (unsigned i = 0; i < p.nev && i < ev.size(); ++i) {
+      if (mu) mu[i] = ev[i];
+      evals[i] = shift + 1.0 / ev[i];
+    }
+    if (!X_out && !resid) return;
+    trl::hip::DeviceBuffer<double> own;
+    double* X = X_out;
+    if (!X) {
+      own = trl::hip::DeviceBuffer<double>(c, std::size_t(nblk) * n * B);
+      X = own.data();
+    }
+    const auto& Y = evp->get_current_eigenvectors();
+    if (lz.basis_holds_ritz_blocks())
+      throw_on(trl_hip_memcpy_d2d(c.get(), X, lz.get_basis().data(), std::size_t(nblk) * n * B * sizeof(double)), "ritz vectors");
+    else
+      throw_on(trl_hip_rotate(c.get(), TRL_HIP_F64, n, B, int(p.ncv / B), lz.get_basis().data(), Y.data(), int(Y.block_cols()), int(nblk), X),
+               "ritz vectors");
+    if (resid) {
+      trl::hip::DeviceBuffer<int32_t> drp(c, arp.size()), dcol(c, std::max<std::size_t>(acol.size(), 1));
+      trl::hip::DeviceBuffer<double> dval(c, std::max<std::size_t>(aval.size(), 1)), AX(c, n * B), BX(c, n * B);
+      throw_on(trl_hip_memcpy_h2d(c.get(), drp.data(), arp.data(), arp.size() * sizeof(int32_t)), "upload A");
+      throw_on(trl_hip_memcpy_h2d(c.get(), dcol.data(), acol.data(), acol.size() * sizeof(int32_t)), "upload A");
+      throw_on(trl_hip_memcpy_h2d(c.get(), dval.data(), aval.data(), aval.size() * sizeof(double)), "upload A");
+      SparseOperator<double> aop(c, n, drp.data(), dcol.data(), dval.data());
+      for (unsigned j = 0; j < nblk; ++j) {
+        const double* Xj = X + std::size_t(j) * n * B;
+        aop.apply(int(B), Xj, AX.data(), false);
+        evp->b_operator().apply(int(B), Xj, BX.data(), false);
+        double lam[16] = {0.0}, sums[32];
+        for (unsigned q = 0; q < B && j * B + q < p.nev; ++q) lam[q] = evals[j * B + q];
+        throw_on(trl_hip_residual_sq(c.get(), TRL_HIP_F64, n, B, BX.data(), AX.data(), nullptr, lam, sums), "residual_sq");
+        for (unsigned q = 0; q < B && j * B + q < p.nev; ++q) resid[j * B + q] = std::sqrt(sums[q]) / std::max(std::sqrt(sums[B + q]), 1e-300);
+      }
+      c.sync();
+    }
+  };
+  if (p.reorth == 0) {
+    trl_amd::BlockLanczos<EVP, trl_amd::ModifiedGS> lz(evp, ep);
+    run(lz);
+  }
+  else {
+    trl_amd::BlockLanczos<EVP, trl::hip::BlockCGS2> lz(evp, ep);
+    run(lz);
+  }
+  if (st) {
+    st->solves = evp->inner_solves();
+    st->iterations = evp->inner_iterations();
+    st->max_iterations = evp->max_inner_iterations();
+  }
+}
+
 } // namespace
 
 extern "C" {
@@ -944,6 +1032,31 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
   }
   catch (const std::exception& e) {
     if (res32) *res32 = r32;
+    return code_of(e);
+  }
+}
+
+int trl_hip_solve_csr_shift_invert(trl_hip_ctx* ctx, size_t n, const int32_t* a_rowptr, const int32_t* a_col, const double* a_val,
+                                   const int32_t* b_rowptr, const int32_t* b_col, const double* b_val, double shift, double inner_tol,
+                                   const trl_hip_solver_params* p, const double* V0, double* evals, double* mu, double* X, double* resid,
+                                   trl_hip_solver_result* res, trl_hip_inner_stats* inner)
+{
+  if (!ctx || !p || !a_rowptr || !a_col || !a_val || !b_rowptr || !b_col || !b_val || !V0 || !evals || !res || n == 0) return TRL_HIP_EINVAL;
+  if (p->order < 0 || p->order > 2 || p->reorth < 0 || p->reorth > 2 || !(inner_tol >= 0.0) || !std::isfinite(shift)) return TRL_HIP_EINVAL;
+  *res = trl_hip_solver_result{};
+  if (inner) *inner = trl_hip_inner_stats{};
+  try {
+    const Context c = Context::borrow(ctx);
+    const std::vector<int32_t> arp(a_rowptr, a_rowptr + n + 1), brp(b_rowptr, b_rowptr + n + 1);
+    if (arp[n] < 0 || brp[n] < 0) return TRL_HIP_EINVAL;
+    const std::vector<int32_t> acol(a_col, a_col + arp[n]), bcol(b_col, b_col + brp[n]);
+    const std::vector<double> aval(a_val, a_val + arp[n]), bval(b_val, b_val + brp[n]);
+    return dispatch_b(p->b, [&](auto Bc) {
+      constexpr unsigned B = decltype(Bc)::value;
+      solve_shift_invert_typed<B>(c, n, arp, acol, aval, brp, bcol, bval, shift, inner_tol, *p, V0, evals, mu, X, resid, *res, inner);
+    });
+  }
+  catch (const std::exception& e) {
     return code_of(e);
   }
 }

@@ -269,6 +269,77 @@ def stiffness_mass(nx: int = 16, ny: int = 16, nz: int = 10, dof: int = 6, seed:
     return K, mass
 
 
+def fe_laplacian_2d(nx: int, ny: int) -> tuple[CSR, CSR]:
+    """Bilinear finite elements for -Laplace u = lambda u on the unit square,
+    Dirichlet, nx x ny interior nodes (first axis fastest): stiffness
+    K = K1x (x) M1y + M1x (x) K1y and consistent mass M = M1x (x) M1y, with
+    the 1D factors K1 = tridiag(-1, 2, -1) / h and M1 = h tridiag(1, 4, 1) / 6,
+    h = 1 / (nodes + 1).  Both 9-point, SPD; M is a sparse (not diagonal) B."""
+    if nx < 3 or ny < 3:
+        raise ValueError("fe_laplacian_2d needs at least 3 nodes per side")
+
+    def one(m):
+        h = 1.0 / (m + 1)
+        k = (np.diag(np.full(m, 2.0)) - np.diag(np.ones(m - 1), 1) - np.diag(np.ones(m - 1), -1)) / h
+        mm = (np.diag(np.full(m, 4.0)) + np.diag(np.ones(m - 1), 1) + np.diag(np.ones(m - 1), -1)) * h / 6.0
+        return k, mm
+
+    kx, mx = one(nx)
+    ky, my = one(ny)
+    offs = [(dx, dy) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+
+    def kron_stencil(ax, ay):
+        # entry of offset (dx, dy) at node (i, j) = ax[i, i+dx] ay[j, j+dy]: the
+        # 1D factors are Toeplitz, so it depends on the offset only (row 1 of
+        # each factor holds all three values)
+        return [float(ax[1, 1 + dx]) * float(ay[1, 1 + dy]) for dx, dy in offs]
+
+    K = _stencil_csr((nx, ny), offs, [a + b for a, b in zip(kron_stencil(kx, my), kron_stencil(mx, ky))])
+    M = _stencil_csr((nx, ny), offs, kron_stencil(mx, my))
+    return K, M
+
+
+def fe_laplacian_2d_eigenvalues(nx: int, ny: int) -> np.ndarray:
+    """All eigenvalues of K x = lambda M x for fe_laplacian_2d(nx, ny),
+    ascending: lambda_k + lambda_l with the 1D values
+    lambda_k = (6 / h^2) (1 - cos t_k) / (2 + cos t_k), t_k = k pi / (m + 1)
+    (the sine vectors diagonalise both 1D factors)."""
+    def one(m):
+        h = 1.0 / (m + 1)
+        t = np.arange(1, m + 1) * np.pi / (m + 1)
+        return 6.0 / h**2 * (1 - np.cos(t)) / (2 + np.cos(t))
+
+    return np.sort((one(nx)[:, None] + one(ny)[None, :]).ravel())
+
+
+def stiffness_sparse_mass(nx: int = 8, ny: int = 8, nz: int = 6, dof: int = 3, seed: int = 42, coupling: float = 0.5) -> tuple[CSR, CSR]:
+    """The structural pair of stiffness_mass with a sparse SPD mass B on the
+    stiffness pattern: off-diagonal B_ij = coupling sqrt(m_i m_j) s_ij /
+    (row length), s_ij ~ U[-1, 1] (stream 8, by upper-triangle entry,
+    mirrored), diagonal m_i + sum_j |B_ij| (strictly diagonally dominant)."""
+    K, m = stiffness_mass(nx, ny, nz, dof, seed)
+    n = K.n
+    rows = np.repeat(np.arange(n), np.diff(K.rowptr))
+    cols = K.col.astype(np.int64)
+    upper = cols > rows
+    iu, ju = rows[upper], cols[upper]
+    s = 2.0 * uniform(seed, 8, iu.size) - 1.0
+    rl = np.diff(K.rowptr).astype(np.float64)
+    v = coupling * np.sqrt(m[iu] * m[ju]) * s / np.maximum(rl[iu], rl[ju])
+    Bv = np.zeros(K.nnz)
+    # positions of the upper entries and of their mirrors in K's pattern
+    key = rows * n + cols
+    pos_u = np.flatnonzero(upper)
+    pos_l = np.searchsorted(key, ju * n + iu)
+    Bv[pos_u] = v
+    Bv[pos_l] = v
+    absrow = np.zeros(n)
+    np.add.at(absrow, rows, np.abs(Bv))
+    diag = rows == cols
+    Bv[diag] = m + absrow
+    return K, CSR(n, K.rowptr.copy(), K.col.copy(), Bv)
+
+
 # ---------------------------------------------------------------------------
 # Named configurations
 

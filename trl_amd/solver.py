@@ -197,6 +197,38 @@ def solve_mixed(A: DeviceCSR, b: int, params: EigensolverParams, V0: L.DeviceArr
     return SolveOutput(evals, X, resid, res.as_dict(), res32.as_dict())
 
 
+@dataclass
+class ShiftInvertOutput:
+    evals: np.ndarray  # lambda = shift + 1/mu
+    mu: np.ndarray  # Ritz values of (A - shift B)^{-1} B
+    X: L.DeviceArray | None
+    resid: np.ndarray | None  # |A x - lambda B x| / |A x|
+    result: dict
+    inner: dict
+
+
+def solve_shift_invert(ctx: L.Context, A, B, shift: float, b: int, params: EigensolverParams, V0: L.DeviceArray, *, order="largest_magnitude",
+                       reorth="cgs2", cholqr_passes=0, inner_tol=0.0, ritz_vectors=True, rr_dense=False) -> ShiftInvertOutput:
+    """A x = lambda B x by shift-invert (trl_hip_solve_csr_shift_invert; the
+    reference's CSRGeneralizedEVP, examples/openmp/csrevp.hh:51-104): A, B
+    host CSR matrices (problems.CSR, full symmetric storage), B SPD, A -
+    shift B positive definite."""
+    assert V0.dtype == np.float64 and V0.nbytes == A.n * b * 8 and A.n == B.n
+    nblk = (params.nev + b - 1) // b
+    X = ctx.empty((nblk, A.n, b), np.float64) if ritz_vectors else None
+    evals, mu, resid = np.zeros(params.nev), np.zeros(params.nev), np.zeros(params.nev)
+    res, inner = L.SolverResult(), L.InnerStats()
+    p = _params(b, params, order, reorth, cholqr_passes, 0, False, rr_dense)
+    arp, acol, aval = (np.ascontiguousarray(A.rowptr, np.int32), np.ascontiguousarray(A.col, np.int32), np.ascontiguousarray(A.val, np.float64))
+    brp, bcol, bval = (np.ascontiguousarray(B.rowptr, np.int32), np.ascontiguousarray(B.col, np.int32), np.ascontiguousarray(B.val, np.float64))
+    rc = L.solver().trl_hip_solve_csr_shift_invert(ctx.h, A.n, arp.ctypes.data, acol.ctypes.data, aval.ctypes.data, brp.ctypes.data, bcol.ctypes.data,
+                                                   bval.ctypes.data, float(shift), float(inner_tol), C.byref(p), V0.ptr, evals.ctypes.data,
+                                                   mu.ctypes.data, X.ptr if X else None, resid.ctypes.data if X else None, C.byref(res),
+                                                   C.byref(inner))
+    L.check(rc, "trl_hip_solve_csr_shift_invert")
+    return ShiftInvertOutput(evals, mu, X, resid if X else None, res.as_dict(), inner.as_dict())
+
+
 def dense_syev(A: np.ndarray):
     """The host FP64 eigensolver of the Rayleigh-Ritz step (for tests)."""
     A = np.ascontiguousarray(A, dtype=np.float64)
