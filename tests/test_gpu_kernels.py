@@ -558,8 +558,11 @@ def test_cholqr_breakdown_flag(ctx, rng):
 
 @pytest.mark.parametrize("p", ["f64", "f32"])
 @pytest.mark.parametrize("b", BS)
-@pytest.mark.parametrize("m,kc", [(3, 1), (8, 3), (32, 5)])
+@pytest.mark.parametrize("m,kc", [(3, 1), (8, 3), (32, 5), (5, 5), (12, 4), (2, 7)])
 def test_rotate(ctx, rng, p, b, m, kc):
+    # short contractions (a tile's operand row in registers, the next tile
+    # prefetched: up to 6 super-steps) and long ones, output chunks of up to
+    # 4 x 16 columns (kc = 7 at b = 16 takes two launches), a ragged tail tile
     n = 1999
     dt = DT[p]
     V = rng.standard_normal((m, n, b)).astype(dt).astype(np.float64)

@@ -574,6 +574,64 @@ struct RotVec {
   using T = std::conditional_t<VEC == 1, S, S __attribute__((ext_vector_type(VEC)))>;
 };
 
+// Stores of one 16-row tile of the rotation: element e of lane (i16, q4) is
+// row mfma_row(q4, e), column t*16 + i16 of the output.  Transposed across
+// neighbouring lanes so that each lane writes 16 contiguous bytes of one row
+// where the block size allows.
+template <typename S, int B, int NT>
+__device__ inline void rotate_store(const typename Mfma<S>::acc_t (&acc)[NT], size_t r0, int lane, int NC, S* __restrict__ Out, size_t ldv, size_t n)
+{
+  const int i16 = lane & 15, q4 = lane >> 4;
+  using acc_t = typename Mfma<S>::acc_t;
+  if constexpr (sizeof(S) == 4 && B % 4 == 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      acc_t r = acc[t];
+      pair_transpose<kDppXor1, 1, 1>(r, 0, lane);
+      pair_transpose<kDppXor1, 1, 1>(r, 2, lane);
+      pair_transpose<kDppXor2, 2, 2>(r, 0, lane);
+      pair_transpose<kDppXor2, 2, 2>(r, 1, lane);
+      // r[j]: row 4 q4 + (i16 & 3), column t*16 + (i16 & ~3) + j
+      const int jn = t * 16 + (i16 & ~3);
+      const size_t row = r0 + 4 * q4 + (i16 & 3);
+      if (jn < NC && row < n) {
+        using V4 = S __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<V4*>(Out + size_t(jn / B) * ldv + row * B + (jn % B)) = V4{r[0], r[1], r[2], r[3]};
+      }
+    }
+  }
+  else if constexpr (sizeof(S) == 8 && B % 2 == 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      acc_t r = acc[t];
+      pair_transpose<kDppXor1, 1, 1>(r, 0, lane);
+      pair_transpose<kDppXor1, 1, 1>(r, 2, lane);
+      // (r[e0], r[e0 + 1]), e0 = 0, 2: row mfma_row(q4, e0 + (i16 & 1)), columns t*16 + (i16 & ~1) + 0, 1
+      const int jn = t * 16 + (i16 & ~1);
+      if (jn < NC) {
+        using V2 = S __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int e0 = 0; e0 < 4; e0 += 2) {
+          const size_t row = r0 + mfma_row<S>(q4, e0 + (i16 & 1));
+          if (row < n) *reinterpret_cast<V2*>(Out + size_t(jn / B) * ldv + row * B + (jn % B)) = V2{r[e0], r[e0 + 1]};
+        }
+      }
+    }
+  }
+  else {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int jn = t * 16 + i16;
+      if (jn >= NC) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const size_t row = r0 + mfma_row<S>(q4, e);
+        if (row < n) Out[size_t(jn / B) * ldv + row * B + (jn % B)] = static_cast<S>(acc[t][e]);
+      }
+    }
+  }
+}
+
 template <typename S, int B, int NT, int VEC>
 __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
                                                 S* __restrict__ Out, size_t n)
@@ -636,57 +694,81 @@ __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t 
         }
       }
     }
-    // stores: element e of lane (i16, q4) is row mfma_row(q4, e), column
-    // t*16 + i16 of the output.  Transposed across neighbouring lanes so
-    // that each lane writes 16 contiguous bytes of one row where the block
-    // size allows.
-    if constexpr (sizeof(S) == 4 && B % 4 == 0) {
+    rotate_store<S, B, NT>(acc, r0, lane, NC, Out, ldv, n);
+  }
+}
+
+// The rotation for a short contraction (K = m*B of at most NSUP super-steps:
+// the refinement sweeps, m = kc = 5): a tile's whole V row segment fits in
+// NSUP vector registers per lane, so the wave loads the next tile's segment
+// before this tile's products and stores, and a tile's loads are in flight
+// during the previous tile's work instead of after it.
+template <typename S, int B, int NT, int VEC, int NSUP>
+__global__ __launch_bounds__(512) void k_rotate_short(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
+                                                      S* __restrict__ Out, size_t n)
+{
+  using M = Mfma<S>;
+  using acc_t = typename M::acc_t;
+  constexpr int KS = 4 * VEC;
+  using VT = typename RotVec<S, VEC>::T;
+  extern __shared__ unsigned char smem_raw[];
+  S* Ys = reinterpret_cast<S*>(smem_raw);
+  const int K = m * B, NC = nb * B;
+  constexpr int ldy = NT * 16;
+  for (int idx = threadIdx.x; idx < K * ldy; idx += blockDim.x) {
+    const int kk = idx / ldy, jn = idx % ldy;
+    double v = 0.0;
+    if (jn < NC) {
+      const int i = kk / B, ci = kk % B, j = jb0 + jn / B, cj = jn % B;
+      v = Yg[((size_t(i) * ybc + j) * B + ci) * B + cj];
+    }
+    Ys[idx] = static_cast<S>(v);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const size_t ntiles = (n + 15) / 16;
+  const size_t stride = size_t(gridDim.x) * nwave;
+  // the VEC columns of super-step u of tile t held by this lane (zero past n or K)
+  auto load_tile = [&](size_t t, VT (&a)[NSUP]) {
+    const size_t ra = t * 16 + i16;
+    const bool rok = t < ntiles && ra < n;
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        acc_t r = acc[t];
-        pair_transpose<kDppXor1, 1, 1>(r, 0, lane);
-        pair_transpose<kDppXor1, 1, 1>(r, 2, lane);
-        pair_transpose<kDppXor2, 2, 2>(r, 0, lane);
-        pair_transpose<kDppXor2, 2, 2>(r, 1, lane);
-        // r[j]: row 4 q4 + (i16 & 3), column t*16 + (i16 & ~3) + j
-        const int jn = t * 16 + (i16 & ~3);
-        const size_t row = r0 + 4 * q4 + (i16 & 3);
-        if (jn < NC && row < n) {
-          using V4 = S __attribute__((ext_vector_type(4)));
-          *reinterpret_cast<V4*>(Out + size_t(jn / B) * ldv + row * B + (jn % B)) = V4{r[0], r[1], r[2], r[3]};
+    for (int u = 0; u < NSUP; ++u) {
+      const int k0 = u * KS + VEC * q4;
+      a[u] = VT{};
+      if (rok && k0 < K) a[u] = *reinterpret_cast<const VT*>(V + size_t(k0 / B) * ldv + ra * B + (k0 % B));
+    }
+  };
+  size_t rt = size_t(blockIdx.x) * nwave + wave;
+  VT cur[NSUP];
+  load_tile(rt, cur);
+  for (; rt < ntiles; rt += stride) {
+    VT nxt[NSUP];
+    load_tile(rt + stride, nxt);
+    acc_t acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < NSUP; ++u)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const int kk = u * KS + VEC * q4 + v;
+        const bool kok = kk < K;
+        S av;
+        if constexpr (VEC == 1)
+          av = cur[u];
+        else
+          av = cur[u][v];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const S bv = kok ? Ys[kk * ldy + t * 16 + i16] : S(0);
+          acc[t] = M::op(av, bv, acc[t]);
         }
       }
-    }
-    else if constexpr (sizeof(S) == 8 && B % 2 == 0) {
+    rotate_store<S, B, NT>(acc, rt * 16, lane, NC, Out, ldv, n);
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        acc_t r = acc[t];
-        pair_transpose<kDppXor1, 1, 1>(r, 0, lane);
-        pair_transpose<kDppXor1, 1, 1>(r, 2, lane);
-        // (r[e0], r[e0 + 1]), e0 = 0, 2: row mfma_row(q4, e0 + (i16 & 1)), columns t*16 + (i16 & ~1) + 0, 1
-        const int jn = t * 16 + (i16 & ~1);
-        if (jn < NC) {
-          using V2 = S __attribute__((ext_vector_type(2)));
-#pragma unroll
-          for (int e0 = 0; e0 < 4; e0 += 2) {
-            const size_t row = r0 + mfma_row<S>(q4, e0 + (i16 & 1));
-            if (row < n) *reinterpret_cast<V2*>(Out + size_t(jn / B) * ldv + row * B + (jn % B)) = V2{r[e0], r[e0 + 1]};
-          }
-        }
-      }
-    }
-    else {
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int jn = t * 16 + i16;
-        if (jn >= NC) continue;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const size_t row = r0 + mfma_row<S>(q4, e);
-          if (row < n) Out[size_t(jn / B) * ldv + row * B + (jn % B)] = static_cast<S>(acc[t][e]);
-        }
-      }
-    }
+    for (int u = 0; u < NSUP; ++u) cur[u] = nxt[u];
   }
 }
 
@@ -1260,6 +1342,60 @@ int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, c
   return 0;
 }
 
+// super-steps of the short-contraction rotation (register cost 2 NSUP vectors per lane)
+constexpr int kRotShortSteps = 6;
+
+template <typename S, int B, int VS, int NT, int NSUP>
+void launch_rotate_short_nt(trl_hip_ctx* ctx, size_t g, size_t shm, const S* v, size_t n, int m, const double* Y, int ybc, int j0, int nb, S* o)
+{
+  hipLaunchKernelGGL((k_rotate_short<S, B, NT, VS, NSUP>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+}
+
+template <typename S, int B, int VS>
+int launch_rotate_short(trl_hip_ctx* ctx, int nt, int nsup, size_t g, size_t shm, const S* v, size_t n, int m, const double* Y, int ybc, int j0, int nb,
+                        S* o)
+{
+  auto by_steps = [&](auto NTc) {
+    constexpr int NT = decltype(NTc)::value;
+    switch (nsup) {
+    case 1:
+      launch_rotate_short_nt<S, B, VS, NT, 1>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      break;
+    case 2:
+      launch_rotate_short_nt<S, B, VS, NT, 2>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      break;
+    case 3:
+      launch_rotate_short_nt<S, B, VS, NT, 3>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      break;
+    case 4:
+      launch_rotate_short_nt<S, B, VS, NT, 4>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      break;
+    case 5:
+      launch_rotate_short_nt<S, B, VS, NT, 5>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      break;
+    default:
+      launch_rotate_short_nt<S, B, VS, NT, 6>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      break;
+    }
+  };
+  switch (nt) {
+  case 1:
+    by_steps(std::integral_constant<int, 1>{});
+    break;
+  case 2:
+    by_steps(std::integral_constant<int, 2>{});
+    break;
+  case 3:
+    by_steps(std::integral_constant<int, 3>{});
+    break;
+  default:
+    by_steps(std::integral_constant<int, 4>{});
+    break;
+  }
+  TRL_HIP_LAUNCH_CHECK();
+  return 0;
+}
+
 int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int ybc, int kc, void* Out)
 {
   TRL_DISPATCH_T(t, {
@@ -1287,6 +1423,14 @@ int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, con
         const size_t shm = size_t(K) * nt * 16 * sizeof(S);
         const S* v = static_cast<const S*>(V);
         S* o = static_cast<S*>(Out) + size_t(j0) * n * B;
+        // short contractions: the whole operand row of a tile in registers, next tile prefetched
+        constexpr int VS = rot_vec<S, B>();
+        const int nsup = (K + 4 * VS - 1) / (4 * VS);
+        if (!v1 && nsup <= kRotShortSteps) {
+          const int rc = launch_rotate_short<S, B, VS>(ctx, nt, nsup, g, shm, v, n, m, Y, ybc, j0, nb, o);
+          if (rc) return rc;
+          continue;
+        }
         switch (nt) {
         case 1:
           if (v1)
