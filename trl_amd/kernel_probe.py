@@ -8,6 +8,11 @@ Kernels (--what, comma separated):
   spmm32, spmm64        Y = A X with the autotuned operator (FP32 / FP64)
   spmm_axpby32/64       Y = alpha A X + beta X + gamma Y (the Chebyshev step, FP32 / FP64)
   panels32, panels64    the same step on --panels panels per launch (the refinement's filter)
+  floor32               that FP32 step with the matrix cut to its diagonal (ELL of one slot,
+                        tuned the same way): the same X, Z and Y streams with no gathered
+                        neighbour rows, the floor of the step's memory pattern
+  stream32              one-panel launches of that diagonal step, one after another over the
+                        --panels panels: the same streams, one pass per panel
   forced32              that FP32 step plus the forcing term F diag(g) (the residual-driven
                         filter of the FP64 sweeps)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
@@ -31,6 +36,15 @@ import numpy as np
 from . import _lib as L
 from . import problems as P
 from .solver import DeviceCSR
+
+
+def diag_of(A: P.CSR) -> P.CSR:
+    """The diagonal of A as a CSR matrix (one entry per row)."""
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    d = np.zeros(A.n)
+    on = A.col == rows
+    d[rows[on]] = A.val[on]
+    return P.CSR(A.n, np.arange(A.n + 1, dtype=np.int32), np.arange(A.n, dtype=np.int32), d)
 
 
 def main(argv=None) -> int:
@@ -104,6 +118,32 @@ def main(argv=None) -> int:
                 L.check(H.trl_hip_spmm_axpby_panels(ctx.h, L.dtype_code(dt), n, b, C.byref(d), npan, X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125), "panels")
 
             timed(f"{w}[{op._name(d)}]", fn, mat + 3 * npan * n * b * s)
+        elif w in ("floor32", "stream32"):
+            # the filter step's streams without gathers: the diagonal of the matrix as ELL
+            # (one slot), the panel step on --panels panels per launch (floor32), or
+            # one panel per launch (stream32)
+            Dm = diag_of(A)
+            dD = DeviceCSR(ctx, Dm, precisions=("f32",))
+            op = dD.operator(b, "f32", tune=True)
+            npan = a.panels
+            X = ctx.to_device(rng.standard_normal((npan, n, b)).astype(np.float32))
+            Y = ctx.to_device(rng.standard_normal((npan, n, b)).astype(np.float32))
+            if w == "floor32":
+                op.tune_panels(npan)
+                d = op.desc_panels
+
+                def fn(d=d, X=X, Y=Y):
+                    L.check(H.trl_hip_spmm_axpby_panels(ctx.h, L.F32, n, b, C.byref(d), npan, X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125), "floor")
+            else:
+                d = op.desc_axpby
+
+                def fn(d=d, X=X, Y=Y):
+                    for j in range(npan):
+                        L.check(H.trl_hip_spmm_axpby(ctx.h, L.F32, n, b, C.byref(d), X.offset(j * n * b), Y.offset(j * n * b), Y.offset(j * n * b), 0.5,
+                                                     -0.25, 0.125), "stream")
+
+            mat = (d.width * n * 8) if d.format == 1 else (Dm.nnz * 8 + (n + 1) * 4)
+            timed(f"{w}[{op._name(d)}]", fn, mat * (1 if w == "floor32" else npan) + 3 * npan * n * b * 4)
         elif w == "forced32":
             # the residual-driven filter step: + F diag(g) read from a fourth panel set
             op = dA.operator(b, "f32", tune=True)
