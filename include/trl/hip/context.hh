@@ -67,6 +67,48 @@ private:
   std::shared_ptr<trl_hip_ctx> h_;
 };
 
+// A captured graph of queued work (trl_hip_graph_*), owned.
+class Graph {
+public:
+  Graph() = default;
+  explicit Graph(trl_hip_graph* g) : g_(g, [](trl_hip_graph* p) { trl_hip_graph_destroy(p); }) {}
+  explicit operator bool() const { return bool(g_); }
+  // Queues one replay; false when the graph is stale (the context replaced a
+  // scratch buffer it names), so that the caller captures it again.
+  bool launch(const Context& ctx) const
+  {
+    const int rc = trl_hip_graph_launch(ctx.get(), g_.get());
+    if (rc == TRL_HIP_ESTALE) return false;
+    throw_on(rc, "graph launch");
+    return true;
+  }
+
+private:
+  std::shared_ptr<trl_hip_graph> g_;
+};
+
+// Captures the work `fn` queues on the context stream into a graph, without
+// running it.  Returns an empty Graph when the capture failed (some call in
+// fn needed the host): the work has then not run, and the caller runs fn
+// itself.  An exception thrown by fn ends the capture and propagates.
+template <class F>
+Graph capture(const Context& ctx, F&& fn)
+{
+  if (trl_hip_graph_begin(ctx.get()) != TRL_HIP_OK) return Graph();
+  try {
+    fn();
+  }
+  catch (...) {
+    trl_hip_graph* g = nullptr;
+    (void)trl_hip_graph_end(ctx.get(), &g);
+    trl_hip_graph_destroy(g);
+    throw;
+  }
+  trl_hip_graph* g = nullptr;
+  if (trl_hip_graph_end(ctx.get(), &g) != TRL_HIP_OK) return Graph();
+  return Graph(g);
+}
+
 // RAII device buffer.
 template <class T>
 class DeviceBuffer {

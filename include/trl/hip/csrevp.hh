@@ -51,6 +51,9 @@ public:
 
   void apply(BlockView X, BlockView Y) override { op_->apply(int(block_size), X.data_, Y.data_); }
 
+  // apply only queues the operator kernel: a cycle's block steps can be captured
+  bool graph_capture_ok() const override { return true; }
+
   // Fixed kernel variant of the CSR operator (trl_hip_spmat::variant).
   void set_lanes_per_row(int l) { op_ = std::make_shared<SparseOperator<PanelScalar>>(this->ctx_, this->N, rowptr_, col_, val_, minv_, l); }
 

@@ -48,7 +48,11 @@ public:
   using BlockMatrixBlockView = typename BlockMatrix::BlockView;
   static constexpr trl_hip_dtype dtype = dtype_of<PanelScalar>();
 
-  EVPBase(const Context& ctx, std::size_t n) : ctx_(ctx), N(n) {}
+  EVPBase(const Context& ctx, std::size_t n) : ctx_(ctx), N(n)
+  {
+    int g = 0;
+    if (trl_hip_ctx_get_option(ctx_.get(), TRL_HIP_OPT_GRAPHS, &g) == TRL_HIP_OK) graphs_ = g != 0;
+  }
   virtual ~EVPBase() = default;
 
   virtual void apply(BlockView X, BlockView Y) = 0;
@@ -140,6 +144,17 @@ public:
   const PanelScalar* mass() const { return mass_; }
 
   const Context& context() const { return ctx_; }
+
+  // Whether the driver may capture a cycle's block steps into a graph and
+  // replay it (trl_hip_graph_*): true when apply only queues device work
+  // (the CSR operators); an EVP whose apply reads back from the device says
+  // false.  set_graphs turns the capture on or off (initially the context's
+  // TRL_HIP_OPT_GRAPHS, off by default).
+  virtual bool graph_capture_ok() const { return false; }
+  void set_graphs(bool on) { graphs_ = on; }
+  bool graphs() const { return graphs_; }
+  // Changes whenever this EVP replaces a scratch buffer a captured step names.
+  unsigned long long scratch_epoch() const { return coef_epoch_; }
   const std::vector<double>& last_residuals() const { return last_residuals_; }
   double host_seconds() const { return host_seconds_; }
 
@@ -178,7 +193,10 @@ public:
 protected:
   double* coef(std::size_t count)
   {
-    if (coef_.size() < count) coef_ = DeviceBuffer<double>(ctx_, std::max<std::size_t>(count, 4096));
+    if (coef_.size() < count) {
+      coef_ = DeviceBuffer<double>(ctx_, std::max<std::size_t>(count, 4096));
+      ++coef_epoch_;
+    }
     return coef_.data();
   }
 
@@ -188,6 +206,8 @@ protected:
   int cholqr_passes_ = std::is_same_v<PanelScalar, float> ? 2 : 1;
   std::size_t ritz_blocks_ = 0;
   bool dense_rr_ = false;
+  bool graphs_ = false;
+  unsigned long long coef_epoch_ = 0;
   unsigned rr_steps_ = 0, rr_structured_ = 0;
 
   std::vector<double> eigenvalues;

@@ -18,12 +18,19 @@
 //     instead of k_r*m mult_add calls (lanczos.hh:105-114);
 //   - the three-term recurrence's "mult into scratch, then -=" pairs as one
 //     pass each (subtract_product, subtract_product_transpose);
-//   - ritz_vectors(): X = V Y[:, 0:nev], which the reference never forms.
+//   - ritz_vectors(): X = V Y[:, 0:nev], which the reference never forms;
+//   - evp.graph_capture_ok(): the restart cycles' block steps captured into a
+//     graph once and replayed (extend_cycle), the same kernels in the same
+//     order.
 
+#include <concepts>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
+
+#include "trl/hip/context.hh"
 
 namespace trl_amd {
 
@@ -171,6 +178,8 @@ public:
   // Converged count of the last Rayleigh-Ritz step.  (The reference's
   // get_nconv() always returns 0: lanczos.hh:121,217.)
   unsigned int get_nconv() const { return last_nconv; }
+  // Restart cycles run from a captured graph so far.
+  unsigned int graph_cycles() const { return graph_cycles_; }
 
 private:
   // The solve loop of lanczos.hh:65-199 from block k: extend to m,
@@ -184,7 +193,7 @@ private:
 
     while (res.iterations < max_restarts) {
       ++res.iterations;
-      res.n_op_apply += extend(k, m);
+      res.n_op_apply += k == 0 ? extend(k, m) : extend_cycle(k, m);
       restarted_ = false;
 
       last_nconv = static_cast<unsigned int>(evp->solve_small_dense(T, beta, nev));
@@ -227,6 +236,51 @@ private:
       if (!quasi_step(k, res)) break;
     }
     return res;
+  }
+
+  // extend(k, m) of a restart cycle.  Every restart cycle runs the same block
+  // steps on the same buffers (V alternates between the two the driver swaps),
+  // so where the EVP allows it the steps are captured into a graph once per
+  // buffer and replayed: one launch per cycle instead of ~20 per block step.
+  // The first cycle (k = 0) runs as is and sizes every scratch buffer; a
+  // capture that fails (a step needed the host) runs the steps directly and
+  // turns capturing off; a graph whose scratch the context or the EVP has
+  // since replaced is captured again.
+  unsigned int extend_cycle(unsigned int k, unsigned int m)
+  {
+    if constexpr (requires(EVP& e) {
+                    { e.graph_capture_ok() } -> std::convertible_to<bool>;
+                    e.graphs();
+                    e.scratch_epoch();
+                    e.context();
+                  }) {
+      if (graphs_ok_ && evp->graphs() && evp->graph_capture_ok()) {
+        const void* base = static_cast<const void*>(V.data());
+        for (auto& cg : cycle_graphs_)
+          if (cg.base == base && cg.k == k && cg.m == m && cg.epoch == evp->scratch_epoch()) {
+            if (cg.graph.launch(evp->context())) {
+              ++graph_cycles_;
+              return m - k;
+            }
+            cg.base = nullptr; // stale: captured again below
+          }
+        trl::hip::Graph g;
+        try {
+          g = trl::hip::capture(evp->context(), [&] { extend(k, m); });
+        }
+        catch (...) {
+          g = trl::hip::Graph();
+        }
+        if (g && g.launch(evp->context())) {
+          std::erase_if(cycle_graphs_, [&](const CycleGraph& c) { return c.base == base || c.base == nullptr; });
+          cycle_graphs_.push_back(CycleGraph{base, k, m, evp->scratch_epoch(), std::move(g)});
+          ++graph_cycles_;
+          return m - k;
+        }
+        graphs_ok_ = false;
+      }
+    }
+    return extend(k, m);
   }
 
   // One quasi-Lanczos step from block k after a restart (lanczos.hh:150-178):
@@ -292,6 +346,16 @@ private:
   typename BMV::BlockMatrix T;
   typename BMV::BlockMatrix B;
   Reorth reorth_{};
+  // captured restart cycles (extend_cycle)
+  struct CycleGraph {
+    const void* base;
+    unsigned int k, m;
+    unsigned long long epoch;
+    trl::hip::Graph graph;
+  };
+  std::vector<CycleGraph> cycle_graphs_;
+  bool graphs_ok_ = true;
+  unsigned int graph_cycles_ = 0;
 };
 
 } // namespace trl_amd

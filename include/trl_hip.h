@@ -45,6 +45,7 @@ extern "C" {
 #define TRL_HIP_EIO (-7)      /* a file cannot be opened or parsed                      */
 #define TRL_HIP_ENOCONV (-8)  /* an inner iterative solve missed its tolerance          */
 #define TRL_HIP_EINDEF (-9)   /* an operator that must be SPD gave p^T K p <= 0         */
+#define TRL_HIP_ESTALE (-10)  /* a graph names scratch memory since replaced            */
 
 typedef enum { TRL_HIP_F32 = 0, TRL_HIP_F64 = 1 } trl_hip_dtype;
 
@@ -74,7 +75,15 @@ int trl_hip_ctx_num_cus(trl_hip_ctx* ctx);
  *   element offsets, which they otherwise take only when a panel or an ELL
  *   array exceeds 4 GiB; value 0 restores the size-based choice. */
 #define TRL_HIP_OPT_OFFSETS64 1
+/*   TRL_HIP_OPT_GRAPHS, value 1: eigenproblems created on the context let
+ *   the driver capture each restart cycle's block steps into a graph once and
+ *   replay it (the same kernels in the same order); 0 (default): the steps
+ *   are queued directly.  Measured on the 128^2 Laplacian (~15 kernels of a
+ *   few microseconds per block step) the replay is no faster: the device,
+ *   not the host, paces such steps. */
+#define TRL_HIP_OPT_GRAPHS 2
 int trl_hip_ctx_set_option(trl_hip_ctx* ctx, int option, int value);
+int trl_hip_ctx_get_option(trl_hip_ctx* ctx, int option, int* value);
 const char* trl_hip_strerror(int code);
 
 /* Zero-initialised device allocation (multivector.hh:44-48 zero-fills). */
@@ -169,6 +178,26 @@ int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
  * breakdown as trl_hip_cholqr), W <- W R^{-1} and BW <- BW R^{-1}; with
  * passes = 2 the returned R is R2 R1. */
 int trl_hip_cholqr_bprod(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, void* BW, double* R, int passes);
+
+/* ---- captured graphs -------------------------------------------------------- */
+
+/* Stream capture of queued work into a graph that the context replays with
+ * one launch (no reference counterpart: the host cost of a Lanczos cycle's
+ * ~20 launches per block step, or 2(i+1) + 8 per step with ModifiedGS,
+ * replaced by one graph launch).  trl_hip_graph_begin starts capturing the
+ * context stream: the calls that follow queue nothing until
+ * trl_hip_graph_end, which returns an executable graph, or an error and no
+ * graph when a captured call needed the host (a synchronisation, a device
+ * allocation, a read back) — the captured work then has not run.  Not in
+ * profiling mode.  trl_hip_graph_launch queues one replay on the context
+ * stream: the same kernels on the same device addresses and arguments;
+ * TRL_HIP_ESTALE if the context has since replaced a scratch buffer the
+ * graph names.  trl_hip_graph_destroy frees the graph. */
+typedef struct trl_hip_graph trl_hip_graph;
+int trl_hip_graph_begin(trl_hip_ctx* ctx);
+int trl_hip_graph_end(trl_hip_ctx* ctx, trl_hip_graph** graph);
+int trl_hip_graph_launch(trl_hip_ctx* ctx, const trl_hip_graph* graph);
+int trl_hip_graph_destroy(trl_hip_graph* graph);
 
 /* ---- operator ------------------------------------------------------------ */
 

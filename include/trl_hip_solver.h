@@ -53,6 +53,7 @@ typedef struct {
   double host_seconds;     /* of which the host Rayleigh-Ritz steps          */
   unsigned rr_steps;       /* Rayleigh-Ritz steps                            */
   unsigned rr_structured;  /* of which by the structured eigensolver         */
+  unsigned graph_cycles;   /* restart cycles replayed from a captured graph   */
 } trl_hip_solver_result;
 
 /* Standard (mass = mass_inv = NULL) or generalized problem with a diagonal

@@ -18,6 +18,7 @@ F32 = 0
 F64 = 1
 # trl_hip_ctx_set_option options
 OPT_OFFSETS64 = 1
+OPT_GRAPHS = 2
 
 c_size = C.c_size_t
 c_int = C.c_int
@@ -49,6 +50,7 @@ HIP_API = [
     ("trl_hip_ctx_stream", c_void_p, [c_void_p]),
     ("trl_hip_ctx_num_cus", c_int, [c_void_p]),
     ("trl_hip_ctx_set_option", c_int, [c_void_p, c_int, c_int]),
+    ("trl_hip_ctx_get_option", c_int, [c_void_p, c_int, c_int_p]),
     ("trl_hip_strerror", C.c_char_p, [c_int]),
     ("trl_hip_malloc", c_int, [c_void_p, c_size, C.POINTER(c_void_p)]),
     ("trl_hip_free", c_int, [c_void_p, c_void_p]),
@@ -63,6 +65,10 @@ HIP_API = [
     ("trl_hip_gram_noalloc", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_update", c_int, [c_void_p, c_int, c_size, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     ("trl_hip_cholqr", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
+    ("trl_hip_graph_begin", c_int, [c_void_p]),
+    ("trl_hip_graph_end", c_int, [c_void_p, C.POINTER(c_void_p)]),
+    ("trl_hip_graph_launch", c_int, [c_void_p, c_void_p]),
+    ("trl_hip_graph_destroy", c_int, [c_void_p]),
     ("trl_hip_cholqr_bprod", c_int, [c_void_p, c_int, c_size, c_int, c_void_p, c_void_p, c_void_p, c_int]),
     ("trl_hip_pcg_workspace", c_size, [c_size, c_int]),
     ("trl_hip_pcg", c_int, [c_void_p, c_int, c_size, c_int, c_spmat_p, c_void_p, c_void_p, c_void_p, C.c_double, c_int, c_void_p, c_int_p, c_double_p]),
@@ -300,7 +306,7 @@ class SolverParams(C.Structure):
 
 class SolverResult(C.Structure):
     _fields_ = [("converged", c_int), ("iterations", C.c_uint), ("n_op_apply", C.c_uint), ("seconds", C.c_double), ("host_seconds", C.c_double),
-                ("rr_steps", C.c_uint), ("rr_structured", C.c_uint)]
+                ("rr_steps", C.c_uint), ("rr_structured", C.c_uint), ("graph_cycles", C.c_uint)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

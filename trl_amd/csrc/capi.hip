@@ -15,6 +15,7 @@ namespace trl_hip_impl {
 int ensure_partials(trl_hip_ctx* ctx, size_t doubles)
 {
   if (doubles <= ctx->partials_cap) return 0;
+  TRL_HIP_NO_CAPTURE(ctx);
   size_t cap = ctx->partials_cap ? ctx->partials_cap : (size_t(1) << 16);
   while (cap < doubles) cap *= 2;
   // the stream may still use the old buffer
@@ -24,6 +25,7 @@ int ensure_partials(trl_hip_ctx* ctx, size_t doubles)
   ctx->partials_cap = 0;
   if (hipMalloc(&ctx->partials, cap * sizeof(double)) != hipSuccess) return TRL_HIP_ENOMEM;
   ctx->partials_cap = cap;
+  ++ctx->scratch_epoch;
   return 0;
 }
 
@@ -90,6 +92,7 @@ long long csr_nnz(trl_hip_ctx* ctx, const int32_t* rowptr, size_t n)
 {
   auto it = ctx->nnz_cache.find(rowptr);
   if (it != ctx->nnz_cache.end()) return it->second;
+  if (refuse_in_capture(ctx)) return 0;
   int32_t last = 0;
   if (hipMemcpyAsync(&last, rowptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) return 0;
   (void)hipStreamSynchronize(ctx->stream);
@@ -122,6 +125,8 @@ const char* trl_hip_strerror(int code)
     return "inner solve did not converge";
   case TRL_HIP_EINDEF:
     return "shifted operator is not positive definite";
+  case TRL_HIP_ESTALE:
+    return "graph names scratch memory the context has since replaced";
   case TRL_HIP_EIO:
     return "cannot open or parse the file";
   default:
@@ -195,6 +200,7 @@ int trl_hip_ctx_destroy(trl_hip_ctx* ctx)
 int trl_hip_ctx_sync(trl_hip_ctx* ctx)
 {
   if (!ctx) return TRL_HIP_EINVAL;
+  TRL_HIP_NO_CAPTURE(ctx);
   TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -202,6 +208,7 @@ int trl_hip_ctx_sync(trl_hip_ctx* ctx)
 int trl_hip_ctx_check(trl_hip_ctx* ctx, int* pivot)
 {
   if (!ctx) return TRL_HIP_EINVAL;
+  TRL_HIP_NO_CAPTURE(ctx);
   int flag = 0;
   TRL_HIP_TRY(hipMemcpyAsync(&flag, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -226,7 +233,25 @@ int trl_hip_ctx_set_option(trl_hip_ctx* ctx, int option, int value)
   case TRL_HIP_OPT_OFFSETS64:
     ctx->force_offsets64 = value ? 1 : 0;
     return 0;
+  case TRL_HIP_OPT_GRAPHS:
+    ctx->graphs = value ? 1 : 0;
+    return 0;
 
+  default:
+    return TRL_HIP_EINVAL;
+  }
+}
+
+int trl_hip_ctx_get_option(trl_hip_ctx* ctx, int option, int* value)
+{
+  if (!ctx || !value) return TRL_HIP_EINVAL;
+  switch (option) {
+  case TRL_HIP_OPT_OFFSETS64:
+    *value = ctx->force_offsets64;
+    return 0;
+  case TRL_HIP_OPT_GRAPHS:
+    *value = ctx->graphs;
+    return 0;
   default:
     return TRL_HIP_EINVAL;
   }
@@ -236,6 +261,7 @@ int trl_hip_malloc(trl_hip_ctx* ctx, size_t bytes, void** out)
 {
   if (!ctx || !out) return TRL_HIP_EINVAL;
   *out = nullptr;
+  TRL_HIP_NO_CAPTURE(ctx);
   if (bytes == 0) bytes = 1;
   const size_t aligned = (bytes + 255) & ~size_t(255);
   void* p = nullptr;
@@ -253,6 +279,7 @@ int trl_hip_free(trl_hip_ctx* ctx, void* p)
 {
   if (!ctx) return TRL_HIP_EINVAL;
   if (!p) return 0;
+  TRL_HIP_NO_CAPTURE(ctx);
   TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
   TRL_HIP_TRY(hipFree(p));
   return 0;
@@ -262,6 +289,7 @@ int trl_hip_memcpy_h2d(trl_hip_ctx* ctx, void* dst, const void* src, size_t byte
 {
   if (!ctx || (bytes && (!dst || !src))) return TRL_HIP_EINVAL;
   if (!bytes) return 0;
+  TRL_HIP_NO_CAPTURE(ctx);
   TRL_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return 0;
@@ -271,6 +299,7 @@ int trl_hip_memcpy_d2h(trl_hip_ctx* ctx, void* dst, const void* src, size_t byte
 {
   if (!ctx || (bytes && (!dst || !src))) return TRL_HIP_EINVAL;
   if (!bytes) return 0;
+  TRL_HIP_NO_CAPTURE(ctx);
   TRL_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return 0;
@@ -379,6 +408,7 @@ int trl_hip_reorth(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int k, co
   }
   if (ctx->prof_on) {
     // profiling mode: decide on the host so that skipped work is not counted
+    TRL_HIP_NO_CAPTURE(ctx);
     int g = 1;
     TRL_HIP_TRY(hipMemcpyAsync(&g, gate, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -401,6 +431,73 @@ int trl_hip_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, 
   if (!ctx || !valid_b(b) || !R || passes < 1 || passes > 2 || (n && !W)) return TRL_HIP_EINVAL;
   Prof pr(ctx, TRL_HIP_PROF_CHOLQR, passes * (3 * panel_bytes(t, n, b) + (mdiag ? panel_bytes(t, n, 1) : 0.0)), t);
   return launch_cholqr(ctx, t, n, b, W, mdiag, R, passes);
+}
+
+} // extern "C"
+
+// A captured, instantiated graph and the context state it depends on.
+struct trl_hip_graph {
+  hipGraphExec_t exec = nullptr;
+  const trl_hip_ctx* ctx = nullptr;
+  unsigned long long epoch = 0;
+};
+
+extern "C" {
+
+int trl_hip_graph_begin(trl_hip_ctx* ctx)
+{
+  // profiling mode records events around every call and reads them later
+  if (!ctx || ctx->capturing || ctx->prof_on) return TRL_HIP_EINVAL;
+  TRL_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = 1;
+  ctx->capture_refused = 0;
+  return 0;
+}
+
+int trl_hip_graph_end(trl_hip_ctx* ctx, trl_hip_graph** graph)
+{
+  if (!ctx || !graph || !ctx->capturing) return TRL_HIP_EINVAL;
+  *graph = nullptr;
+  ctx->capturing = 0;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  (void)hipGetLastError(); // a call refused during the capture leaves its error behind
+  if (e != hipSuccess || !g || ctx->capture_refused) {
+    // a call refused during the capture: the graph lacks its work
+    if (g) (void)hipGraphDestroy(g);
+    ctx->capture_refused = 0;
+    return e != hipSuccess ? int(e) : TRL_HIP_EINVAL;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return int(ei);
+  auto* out = new (std::nothrow) trl_hip_graph;
+  if (!out) {
+    (void)hipGraphExecDestroy(exec);
+    return TRL_HIP_ENOMEM;
+  }
+  out->exec = exec;
+  out->ctx = ctx;
+  out->epoch = ctx->scratch_epoch;
+  *graph = out;
+  return 0;
+}
+
+int trl_hip_graph_launch(trl_hip_ctx* ctx, const trl_hip_graph* graph)
+{
+  if (!ctx || !graph || !graph->exec || graph->ctx != ctx || ctx->capturing) return TRL_HIP_EINVAL;
+  if (graph->epoch != ctx->scratch_epoch) return TRL_HIP_ESTALE;
+  TRL_HIP_TRY(hipGraphLaunch(graph->exec, ctx->stream));
+  return 0;
+}
+
+int trl_hip_graph_destroy(trl_hip_graph* graph)
+{
+  if (!graph) return 0;
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  delete graph;
+  return 0;
 }
 
 int trl_hip_cholqr_bprod(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, void* BW, double* R, int passes)
@@ -539,6 +636,7 @@ int trl_hip_spmm_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int wid
 int trl_hip_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* width)
 {
   if (!ctx || !width || (n && !rowptr) || n > size_t(INT32_MAX)) return TRL_HIP_EINVAL;
+  TRL_HIP_NO_CAPTURE(ctx);
   *width = 0;
   if (!n) return 0;
   int* d = ctx->flags + 16;
@@ -553,6 +651,7 @@ int trl_hip_csr_max_row(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, int* 
 int trl_hip_csr_to_ell(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, const int32_t* rowptr, const int32_t* col, const void* val, int width, int32_t* ell_col,
                        void* ell_val)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx || width < 0 || n > size_t(INT32_MAX) || (n && (!rowptr || (width && (!ell_col || !ell_val))))) return TRL_HIP_EINVAL;
   if (!n || !width) return 0;
   return launch_csr_to_ell(ctx, t, n, rowptr, col, val, width, ell_col, ell_val);
@@ -579,6 +678,7 @@ namespace {
 int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* csr, const trl_hip_spmat* ell, bool axpby, int npanels,
                    trl_hip_spmat* best, double* best_us)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx || !valid_b(b) || !best || npanels < 1 || check_spmat(n, csr) || csr->format != 0 || (ell && (check_spmat(n, ell) || ell->format != 1)))
     return TRL_HIP_EINVAL;
   *best = *csr;
@@ -726,6 +826,7 @@ size_t trl_hip_pcg_workspace(size_t n, int b) { return valid_b(b) ? pcg_workspac
 int trl_hip_pcg(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl_hip_spmat* K, const void* dinv, const void* rhs, void* X, double tol,
                 int maxit, void* work, int* iters, double* relres)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (iters) *iters = 0;
   if (relres) *relres = 0.0;
   if (!ctx || t != TRL_HIP_F64 || !valid_b(b) || check_spmat(n, K) || !(tol > 0.0) || maxit < 0 || (n && (!dinv || !rhs || !X || !work)) || rhs == X)
@@ -784,12 +885,14 @@ int trl_hip_small_mult(trl_hip_ctx* ctx, int b, const double* A, const double* B
 
 int trl_hip_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* mdiag, const double* lambda, double* out)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx || !valid_b(b) || !out || (n && (!X || !Y))) return TRL_HIP_EINVAL;
   return launch_residual_sq(ctx, t, n, b, X, Y, mdiag, lambda, out);
 }
 
 int trl_hip_region_begin(trl_hip_ctx* ctx)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx) return TRL_HIP_EINVAL;
   for (auto& e : ctx->region)
     if (!e) TRL_HIP_TRY(hipEventCreate(&e));
@@ -799,6 +902,7 @@ int trl_hip_region_begin(trl_hip_ctx* ctx)
 
 int trl_hip_region_end(trl_hip_ctx* ctx, double* seconds)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx || !seconds || !ctx->region[0]) return TRL_HIP_EINVAL;
   TRL_HIP_TRY(hipEventRecord(ctx->region[1], ctx->stream));
   TRL_HIP_TRY(hipEventSynchronize(ctx->region[1]));
@@ -810,6 +914,7 @@ int trl_hip_region_end(trl_hip_ctx* ctx, double* seconds)
 
 int trl_hip_prof_enable(trl_hip_ctx* ctx, int on)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx) return TRL_HIP_EINVAL;
   int rc = prof_fold(ctx);
   if (rc) return rc;
@@ -833,6 +938,7 @@ int trl_hip_prof_enable(trl_hip_ctx* ctx, int on)
 
 int trl_hip_prof_read(trl_hip_ctx* ctx, int family, long long* calls, double* seconds, double* bytes)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx || family < 0 || family >= TRL_HIP_PROF_FAMILIES) return TRL_HIP_EINVAL;
   int rc = prof_fold(ctx);
   if (rc) return rc;
@@ -844,6 +950,7 @@ int trl_hip_prof_read(trl_hip_ctx* ctx, int family, long long* calls, double* se
 
 int trl_hip_prof_read_dtype(trl_hip_ctx* ctx, int family, trl_hip_dtype t, long long* calls, double* seconds, double* bytes)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx || family < 0 || family >= TRL_HIP_PROF_FAMILIES || (t != TRL_HIP_F32 && t != TRL_HIP_F64)) return TRL_HIP_EINVAL;
   int rc = prof_fold(ctx);
   if (rc) return rc;
@@ -855,6 +962,7 @@ int trl_hip_prof_read_dtype(trl_hip_ctx* ctx, int family, trl_hip_dtype t, long 
 
 int trl_hip_prof_reorth(trl_hip_ctx* ctx, long long* calls, long long* second_passes)
 {
+  if (ctx) TRL_HIP_NO_CAPTURE(ctx);
   if (!ctx) return TRL_HIP_EINVAL;
   if (calls) *calls = ctx->reorth_calls;
   if (second_passes) *second_passes = ctx->reorth_second;

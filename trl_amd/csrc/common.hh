@@ -26,6 +26,14 @@ struct trl_hip_ctx {
   // per-workgroup partial sums of the block products (FP64)
   double* partials = nullptr;
   size_t partials_cap = 0; // in doubles
+  // bumped whenever a scratch buffer is reallocated: a captured graph that
+  // names the old one is stale
+  unsigned long long scratch_epoch = 0;
+  // stream capture in progress (trl_hip_graph_begin .. _end); a call that
+  // needed the host was refused during it
+  int capturing = 0, capture_refused = 0;
+  // TRL_HIP_OPT_GRAPHS: drivers replay captured restart cycles
+  int graphs = 0;
   // flags[0]: sticky Cholesky breakdown pivot (1-based, 0 = none); [8]:
   // CholQR arrival counter; [16]: csr_max_row result; [24]: second-pass gate
   int* flags = nullptr;
@@ -114,6 +122,21 @@ int launch_combine(trl_hip_ctx* ctx, size_t n, int b, int npanels, double* X, co
 int launch_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const void* X, const void* Y, const void* md, const double* lambda, double* out);
 
 inline int hip_status(hipError_t e) { return e == hipSuccess ? 0 : static_cast<int>(e); }
+
+// A call that synchronises with the host, allocates or reads device memory
+// back cannot be part of a captured graph: while the context stream is being
+// captured it is refused before it touches the stream (so the capture stays
+// valid, and trl_hip_graph_end discards it).
+inline int refuse_in_capture(trl_hip_ctx* ctx)
+{
+  if (!ctx->capturing) return 0;
+  ctx->capture_refused = 1;
+  return TRL_HIP_EINVAL;
+}
+#define TRL_HIP_NO_CAPTURE(ctx)                                                                                                                       \
+  do {                                                                                                                                                 \
+    if (const int rc_capture_ = refuse_in_capture(ctx)) return rc_capture_;                                                                           \
+  } while (0)
 
 } // namespace trl_hip_impl
 

@@ -134,6 +134,7 @@ void solve_typed(const Context& c, std::size_t n, const int32_t* rowptr, const i
     res.host_seconds = evp->host_seconds();
     res.rr_steps = evp->rr_steps();
     res.rr_structured = evp->rr_structured();
+    res.graph_cycles = lz.graph_cycles();
     const auto& ev = evp->get_current_eigenvalues();
     for (unsigned i = 0; i < p.nev && i < ev.size(); ++i) evals[i] = ev[i];
     if (all_evals) *all_evals = ev;
