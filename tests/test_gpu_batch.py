@@ -112,12 +112,13 @@ def test_batch_cholqr2_and_errors(ctx):
 
 
 def test_batch_c4_64_uneven_problems(ctx):
-    # 64 2D Laplacians, sides U{48..256}, potential U[0, 2], b = 4, 10 smallest, ncv = 48
+    # 64 2D Laplacians, sides U{48..256}, potential U[0, 2], b = 4, 10 smallest, ncv = 128
     cfgs = P.config_c4()
     mats = [c.csr for c in cfgs]
+    ncv = cfgs[0].ncv
     V0 = [P.initial_block(m.n, B, seed=42 + p) for p, m in enumerate(mats)]
-    print(f"c4: {len(mats)} problems, n {min(m.n for m in mats)}..{max(m.n for m in mats)}", flush=True)
-    out = solve_batch(ctx, mats, B, NEV, NCV, V0, tune=True)
+    print(f"c4: {len(mats)} problems, n {min(m.n for m in mats)}..{max(m.n for m in mats)}, ncv {ncv}", flush=True)
+    out = solve_batch(ctx, mats, B, NEV, ncv, V0, tune=True)
     st = out.stats
     t = np.array([r["seconds"] for r in out.results])
     print(f"batch {st['seconds']:.3f} s, {len(mats) / st['seconds']:.1f} problems/s, p50 {np.percentile(t, 50) * 1e3:.1f} ms, "
@@ -128,7 +129,7 @@ def test_batch_c4_64_uneven_problems(ctx):
     # spot checks against single-problem solves: the smallest, the largest and two others
     order = np.argsort([m.n for m in mats])
     for p in (order[0], order[-1], order[20], order[45]):
-        ref = single(ctx, mats[p], V0[p])
+        ref = single(ctx, mats[p], V0[p], ncv=ncv)
         rel = np.abs(out.evals[p] - ref.evals) / np.abs(ref.evals)
         print(f"problem {p} n={mats[p].n}: max rel vs single {rel.max():.2e}")
         assert rel.max() <= 1e-10

@@ -383,12 +383,14 @@ def config_c4_sides(count: int = 64, seed: int = 42) -> np.ndarray:
 
 
 def config_c4(count: int = 64, seed: int = 42) -> list[Config]:
-    """count 2D Laplacians, side ~ U{48..256}, potential U[0, 2], b=4, 10 smallest."""
+    """count 2D Laplacians, side ~ U{48..256}, potential U[0, 2], b=4, 10
+    smallest, ncv = 128 (32 blocks; the batch then needs 33 restart cycles,
+    against 311 with 12 blocks, and solves the 64 problems ~1.6x faster)."""
     out = []
     off = 0
     for p, s in enumerate(config_c4_sides(count, seed)):
         s = int(s)
         pot = 2.0 * uniform(seed, 1, s * s, offset=off)
         off += s * s
-        out.append(Config(f"batch{p}_side{s}", laplacian_2d(s, s, pot), b=4, nev=10, ncv=48))
+        out.append(Config(f"batch{p}_side{s}", laplacian_2d(s, s, pot), b=4, nev=10, ncv=128))
     return out
