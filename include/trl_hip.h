@@ -73,7 +73,9 @@ int trl_hip_ctx_num_cus(trl_hip_ctx* ctx);
  * problem sizes at hand would not take):
  *   TRL_HIP_OPT_OFFSETS64, value 1: the operator kernels use their 64-bit
  *   element offsets, which they otherwise take only when a panel or an ELL
- *   array exceeds 4 GiB; value 0 restores the size-based choice. */
+ *   array exceeds 4 GiB, and the Ritz rotation its 64-bit-address form,
+ *   otherwise taken only when its buffers exceed 2 GiB; value 0 restores the
+ *   size-based choice. */
 #define TRL_HIP_OPT_OFFSETS64 1
 /*   TRL_HIP_OPT_GRAPHS, value 1: eigenproblems created on the context let
  *   the driver capture each restart cycle's block steps into a graph once and

@@ -574,6 +574,31 @@ def test_rotate(ctx, rng, p, b, m, kc):
     assert rel(Out.download(), ref) < (1e-12 if p == "f64" else 1e-5)
 
 
+@pytest.mark.parametrize("p", ["f64", "f32"])
+@pytest.mark.parametrize("b,m,kc", [(8, 5, 5), (8, 32, 5), (6, 12, 4), (16, 2, 7), (1, 40, 3)])
+def test_rotate_address_forms(ctx, rng, p, b, m, kc):
+    # the buffer-resource forms (short and long contractions) and the
+    # 64-bit-address form that buffers past 2 GiB take (forced by the
+    # OFFSETS64 option), each against the oracle on a ragged tail tile
+    n = 1999
+    dt = DT[p]
+    V = rng.standard_normal((m, n, b)).astype(dt).astype(np.float64)
+    Y = rng.standard_normal((m, kc + 1, b, b))
+    ref = oracle.rotate(V, Y.astype(dt).astype(np.float64), kc)
+    Vd, Yd = dev(ctx, V, dt), dev(ctx, Y, np.float64)
+    got = []
+    for wide in (0, 1):
+        Out = ctx.empty((kc, n, b), dt)
+        ctx.set_option(L.OPT_OFFSETS64, wide)
+        try:
+            L.check(H().trl_hip_rotate(ctx.h, L.dtype_code(dt), n, b, m, Vd.ptr, Yd.ptr, kc + 1, kc, Out.ptr), "rotate")
+        finally:
+            ctx.set_option(L.OPT_OFFSETS64, 0)
+        got.append(Out.download())
+    for g in got:
+        assert rel(g, ref) < (1e-12 if p == "f64" else 1e-5)
+
+
 def test_small_ops(ctx, rng):
     for b in (1, 3, 8, 16):
         A = rng.standard_normal((b, b))

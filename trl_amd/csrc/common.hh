@@ -18,7 +18,8 @@ struct trl_hip_ctx {
   int num_cus = 256;
   size_t max_lds = 65536; // dynamic LDS bytes a workgroup may request
   // TRL_HIP_OPT_OFFSETS64: the operator kernels take their 64-bit-offset
-  // instantiations even where 32-bit lane offsets would do
+  // instantiations even where 32-bit lane offsets would do, and the Ritz
+  // rotation its 64-bit-address form instead of its buffer-resource forms
   int force_offsets64 = 0;
   // device address of the allocation-free entry point's scratch array
   // (resolved at creation; see noalloc_scratch)

@@ -698,78 +698,349 @@ __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t 
   }
 }
 
+// Range-checked buffer accesses (raw buffer resource, no stride): an access
+// at a byte offset at or past the resource's size reads zeros and writes
+// nothing, so edge lanes need no branch around their loads and stores and
+// the compiler can count the loads and stores a wave has in flight exactly.
+constexpr int kBufFlags = 0x00020000;     // resource word 3 on gfx9-class parts
+
+template <int BYTES>
+struct BufIO;
+template <>
+struct BufIO<16> {
+  using T = unsigned __attribute__((ext_vector_type(4)));
+  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off) { return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0); }
+  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off) { __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0); }
+};
+template <>
+struct BufIO<8> {
+  using T = unsigned __attribute__((ext_vector_type(2)));
+  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off) { return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0); }
+  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off) { __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0); }
+};
+template <>
+struct BufIO<4> {
+  using T = unsigned;
+  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off) { return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0); }
+  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off) { __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0); }
+};
+
+template <typename T>
+__device__ inline T buf_load(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+  return __builtin_bit_cast(T, BufIO<sizeof(T)>::load(r, off));
+}
+template <typename T>
+__device__ inline void buf_store(const T& v, __amdgpu_buffer_rsrc_t r, unsigned off)
+{
+  BufIO<sizeof(T)>::store(__builtin_bit_cast(typename BufIO<sizeof(T)>::T, v), r, off);
+}
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, size_t bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), int(bytes), kBufFlags);
+}
+// The same for a base and size that are the same in every lane but computed
+// where the compiler cannot tell: read from the first lane, so the resource
+// sits in scalar registers (no per-lane loop around the access).
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc_uniform(const void* p, size_t bytes)
+{
+  const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(a)), hi = __builtin_amdgcn_readfirstlane(unsigned(a >> 32));
+  const int nb = __builtin_amdgcn_readfirstlane(int(bytes));
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>((static_cast<unsigned long long>(hi) << 32) | lo), short(0), nb, kBufFlags);
+}
+
+// rotate_store through a buffer resource over Out (element offsets from its
+// base; rows past n and columns past NC move nothing)
+template <typename S, int B, int NT>
+__device__ inline void rotate_store_buf(const typename Mfma<S>::acc_t (&acc)[NT], size_t r0, int lane, int NC, __amdgpu_buffer_rsrc_t out, unsigned ldv,
+                                        size_t n)
+{
+  const int i16 = lane & 15, q4 = lane >> 4;
+  using acc_t = typename Mfma<S>::acc_t;
+  if constexpr (sizeof(S) == 4 && B % 4 == 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      acc_t r = acc[t];
+      pair_transpose<kDppXor1, 1, 1>(r, 0, lane);
+      pair_transpose<kDppXor1, 1, 1>(r, 2, lane);
+      pair_transpose<kDppXor2, 2, 2>(r, 0, lane);
+      pair_transpose<kDppXor2, 2, 2>(r, 1, lane);
+      const int jn = t * 16 + (i16 & ~3);
+      const size_t row = r0 + 4 * q4 + (i16 & 3);
+      const unsigned off = ((unsigned(jn / B) * ldv + unsigned(row) * B + unsigned(jn % B)) * 4u) | (unsigned((jn >= NC) | (row >= n)) << 31);
+      buf_store(r, out, off);
+    }
+  }
+  else if constexpr (sizeof(S) == 8 && B % 2 == 0) {
+    using V2 = S __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      acc_t r = acc[t];
+      pair_transpose<kDppXor1, 1, 1>(r, 0, lane);
+      pair_transpose<kDppXor1, 1, 1>(r, 2, lane);
+      const int jn = t * 16 + (i16 & ~1);
+#pragma unroll
+      for (int e0 = 0; e0 < 4; e0 += 2) {
+        const size_t row = r0 + mfma_row<S>(q4, e0 + (i16 & 1));
+        const unsigned off = ((unsigned(jn / B) * ldv + unsigned(row) * B + unsigned(jn % B)) * 8u) | (unsigned((jn >= NC) | (row >= n)) << 31);
+        buf_store(V2{r[e0], r[e0 + 1]}, out, off);
+      }
+    }
+  }
+  else {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int jn = t * 16 + i16;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const size_t row = r0 + mfma_row<S>(q4, e);
+        const unsigned off = ((unsigned(jn / B) * ldv + unsigned(row) * B + unsigned(jn % B)) * unsigned(sizeof(S))) | (unsigned((jn >= NC) | (row >= n)) << 31);
+        buf_store(static_cast<S>(acc[t][e]), out, off);
+      }
+    }
+  }
+}
+
 // The rotation for a short contraction (K = m*B of at most NSUP super-steps:
 // the refinement sweeps, m = kc = 5): a tile's whole V row segment fits in
 // NSUP vector registers per lane, so the wave loads the next tile's segment
-// before this tile's products and stores, and a tile's loads are in flight
-// during the previous tile's work instead of after it.
-template <typename S, int B, int NT, int VEC, int NSUP>
-__global__ __launch_bounds__(512) void k_rotate_short(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
-                                                      S* __restrict__ Out, size_t n)
+// before this tile's products and stores.  Loads and stores go through
+// range-checked buffer resources (V and Out each below 2^31 bytes), so no
+// lane branches around them and a tile's stores are never waited for.  Y
+// sits in LDS lane-major: the VEC*NT coefficients a lane multiplies in one
+// super-step are contiguous (zeros past K), and with YREG all of them are
+// held in registers for the whole launch.
+template <typename S, int B, int NT, int VEC, int NSUP, bool YREG>
+__device__ inline void rotate_short_body(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
+                                         S* __restrict__ Out, size_t n)
 {
   using M = Mfma<S>;
   using acc_t = typename M::acc_t;
   constexpr int KS = 4 * VEC;
+  constexpr int YL = VEC * NT; // coefficients per lane and super-step
   using VT = typename RotVec<S, VEC>::T;
   extern __shared__ unsigned char smem_raw[];
   S* Ys = reinterpret_cast<S*>(smem_raw);
   const int K = m * B, NC = nb * B;
-  constexpr int ldy = NT * 16;
-  for (int idx = threadIdx.x; idx < K * ldy; idx += blockDim.x) {
-    const int kk = idx / ldy, jn = idx % ldy;
-    double v = 0.0;
-    if (jn < NC) {
-      const int i = kk / B, ci = kk % B, j = jb0 + jn / B, cj = jn % B;
-      v = Yg[((size_t(i) * ybc + j) * B + ci) * B + cj];
+  // Ys[((u*4 + q)*16 + i)*YL + v*NT + t] = Y(k = u*KS + VEC*q + v, column t*16 + i)
+  for (int idx = threadIdx.x; idx < NSUP * 64 * YL; idx += blockDim.x) {
+    const int g = idx / (16 * YL), rem = idx % (16 * YL);
+    const int i = rem / YL, w = rem % YL, v = w / NT, t = w % NT;
+    const int kk = (g >> 2) * KS + VEC * (g & 3) + v, jn = t * 16 + i;
+    double y = 0.0;
+    if (kk < K && jn < NC) {
+      const int bi = kk / B, ci = kk % B, j = jb0 + jn / B, cj = jn % B;
+      y = Yg[((size_t(bi) * ybc + j) * B + ci) * B + cj];
     }
-    Ys[idx] = static_cast<S>(v);
+    Ys[idx] = static_cast<S>(y);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int i16 = lane & 15, q4 = lane >> 4;
-  const size_t ntiles = (n + 15) / 16;
-  const size_t stride = size_t(gridDim.x) * nwave;
-  // the VEC columns of super-step u of tile t held by this lane (zero past n or K)
-  auto load_tile = [&](size_t t, VT (&a)[NSUP]) {
-    const size_t ra = t * 16 + i16;
-    const bool rok = t < ntiles && ra < n;
-#pragma unroll
-    for (int u = 0; u < NSUP; ++u) {
-      const int k0 = u * KS + VEC * q4;
-      a[u] = VT{};
-      if (rok && k0 < K) a[u] = *reinterpret_cast<const VT*>(V + size_t(k0 / B) * ldv + ra * B + (k0 % B));
-    }
-  };
-  size_t rt = size_t(blockIdx.x) * nwave + wave;
-  VT cur[NSUP];
-  load_tile(rt, cur);
-  for (; rt < ntiles; rt += stride) {
-    VT nxt[NSUP];
-    load_tile(rt + stride, nxt);
-    acc_t acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = acc_t{0, 0, 0, 0};
+  const S* ylane = Ys + size_t(q4 * 16 + i16) * YL; // + u * 64 * YL
+  S yr[YREG ? NSUP : 1][YL];
+  if constexpr (YREG) {
 #pragma unroll
     for (int u = 0; u < NSUP; ++u)
 #pragma unroll
+      for (int w = 0; w < YL; ++w) yr[u][w] = ylane[u * 64 * YL + w];
+  }
+  const size_t ntiles = (n + 15) / 16;
+  const size_t stride = size_t(gridDim.x) * nwave;
+  const __amdgpu_buffer_rsrc_t vr = buf_rsrc(V, size_t(m) * ldv * sizeof(S));
+  const __amdgpu_buffer_rsrc_t orc = buf_rsrc(Out, size_t(nb) * ldv * sizeof(S));
+  const unsigned ldv32 = unsigned(ldv);
+  // the VEC columns of super-step u of tile t held by this lane (zero past n or K)
+  // Offsets are formed without branches: the top bit, OR-ed in after the
+  // additions, puts an access past the resource (every valid offset is
+  // below 2^31).  Per super-step: the lane's column offset and its bit past K.
+  unsigned koff[NSUP], kbad[NSUP];
+#pragma unroll
+  for (int u = 0; u < NSUP; ++u) {
+    const int k0 = u * KS + VEC * q4;
+    koff[u] = (unsigned(k0 / B) * ldv32 + unsigned(k0 % B)) * unsigned(sizeof(S));
+    kbad[u] = unsigned(k0 >= K) << 31;
+  }
+  auto load_tile = [&](size_t t, VT (&a)[NSUP]) {
+    const unsigned roff = (unsigned(t) * 16u + unsigned(i16)) * unsigned(B * sizeof(S));
+    const unsigned rbad = unsigned(t * 16 + i16 >= n) << 31;
+#pragma unroll
+    for (int u = 0; u < NSUP; ++u) a[u] = buf_load<VT>(vr, (koff[u] + roff) | kbad[u] | rbad);
+  };
+  // one tile: products of its row segment a, then its stores
+  auto tile = [&](size_t t, const VT (&a)[NSUP]) {
+    acc_t acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < NSUP; ++u) {
+      S yu[YL];
+#pragma unroll
+      for (int w = 0; w < YL; ++w) {
+        if constexpr (YREG)
+          yu[w] = yr[u][w];
+        else
+          yu[w] = ylane[u * 64 * YL + w];
+      }
+#pragma unroll
       for (int v = 0; v < VEC; ++v) {
-        const int kk = u * KS + VEC * q4 + v;
-        const bool kok = kk < K;
         S av;
         if constexpr (VEC == 1)
-          av = cur[u];
+          av = a[u];
         else
-          av = cur[u][v];
+          av = a[u][v];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const S bv = kok ? Ys[kk * ldy + t * 16 + i16] : S(0);
-          acc[t] = M::op(av, bv, acc[t]);
-        }
+        for (int j = 0; j < NT; ++j) acc[j] = M::op(av, yu[v * NT + j], acc[j]);
       }
-    rotate_store<S, B, NT>(acc, rt * 16, lane, NC, Out, ldv, n);
-#pragma unroll
-    for (int u = 0; u < NSUP; ++u) cur[u] = nxt[u];
+    }
+    rotate_store_buf<S, B, NT>(acc, t * 16, lane, NC, orc, ldv32, n);
+  };
+  // Two register sets in turn: the loads of the next tile are issued (and
+  // kept ahead of this tile's products by a scheduling barrier) before the
+  // products and stores of the current one.  Both halves run unconditionally:
+  // a tile past the last one reads zeros and stores nothing.
+  size_t rt = size_t(blockIdx.x) * nwave + size_t(__builtin_amdgcn_readfirstlane(wave));
+  VT ra_[NSUP], rb_[NSUP];
+  load_tile(rt, ra_);
+  for (; rt < ntiles; rt += 2 * stride) {
+    load_tile(rt + stride, rb_);
+    __builtin_amdgcn_sched_barrier(0);
+    tile(rt, ra_);
+    load_tile(rt + 2 * stride, ra_);
+    __builtin_amdgcn_sched_barrier(0);
+    tile(rt + stride, rb_);
   }
+}
+template <typename S, int B, int NT, int VEC, int NSUP, bool YREG>
+__global__ __launch_bounds__(512) void k_rotate_short(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
+                                                      S* __restrict__ Out, size_t n)
+{
+  rotate_short_body<S, B, NT, VEC, NSUP, YREG>(V, ldv, m, Yg, ybc, jb0, nb, Out, n);
+}
+template <typename S, int B, int NT, int VEC, int NSUP, bool YREG>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5))) void k_rotate_short_w5(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
+                                                      S* __restrict__ Out, size_t n)
+{
+  rotate_short_body<S, B, NT, VEC, NSUP, YREG>(V, ldv, m, Yg, ybc, jb0, nb, Out, n);
+}
+
+// k-steps per prefetch group of the long-contraction rotation
+constexpr int kRotGroupK = 32;
+
+// The rotation for a long contraction (the thick restart, K = m*B in the
+// hundreds): a tile's operand row streams through two register sets in
+// groups of kRotGroupK k-steps, the loads of the next group (of this tile,
+// or the first of the next tile) issued before the products of the current
+// one.  Loads and stores are range-checked buffer accesses (zeros past n and
+// K, nothing stored past n or NC); a tile's stores are issued during the
+// next tile's first group, so no product waits for a store.  Y sits in LDS
+// lane-major over ng2 groups (an even count, zeros past K).
+template <typename S, int B, int NT, int VEC>
+__global__ __launch_bounds__(512) void k_rotate_long(const S* __restrict__ V, size_t ldv, int m, const double* __restrict__ Yg, int ybc, int jb0, int nb,
+                                                     S* __restrict__ Out, size_t n, int ng2)
+{
+  using M = Mfma<S>;
+  using acc_t = typename M::acc_t;
+  constexpr int KS = 4 * VEC;
+  constexpr int U = kRotGroupK / KS; // super-steps per group
+  constexpr int YL = VEC * NT;
+  using VT = typename RotVec<S, VEC>::T;
+  extern __shared__ unsigned char smem_raw[];
+  S* Ys = reinterpret_cast<S*>(smem_raw);
+  const int K = m * B, NC = nb * B;
+  // Ys[((u*4 + q)*16 + i)*YL + v*NT + t] = Y(k = u*KS + VEC*q + v, column t*16 + i), u over all super-steps
+  for (int idx = threadIdx.x; idx < ng2 * U * 64 * YL; idx += blockDim.x) {
+    const int g = idx / (16 * YL), rem = idx % (16 * YL);
+    const int i = rem / YL, w = rem % YL, v = w / NT, t = w % NT;
+    const int kk = (g >> 2) * KS + VEC * (g & 3) + v, jn = t * 16 + i;
+    double y = 0.0;
+    if (kk < K && jn < NC) {
+      const int bi = kk / B, ci = kk % B, j = jb0 + jn / B, cj = jn % B;
+      y = Yg[((size_t(bi) * ybc + j) * B + ci) * B + cj];
+    }
+    Ys[idx] = static_cast<S>(y);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const S* ylane = Ys + size_t(q4 * 16 + i16) * YL;
+  const size_t ntiles = (n + 15) / 16;
+  const size_t stride = size_t(gridDim.x) * nwave;
+  const __amdgpu_buffer_rsrc_t orc = buf_rsrc(Out, size_t(nb) * ldv * sizeof(S));
+  const unsigned ldv32 = unsigned(ldv);
+  // A super-step's columns lie in at most kSpan consecutive blocks of V; each
+  // super-step reads through a resource over those blocks alone (below 2^31
+  // bytes however many blocks V has).  Offsets are formed without branches:
+  // the top bit, OR-ed in after the additions, puts an access past the
+  // resource (every valid offset is below 2^31).
+  constexpr int kSpan = (KS + B - 1) / B + 1;
+  auto load_group = [&](size_t t, int g, VT (&a)[U]) {
+    const unsigned roff = (unsigned(t) * 16u + unsigned(i16)) * unsigned(B * sizeof(S));
+    const unsigned rbad = unsigned(t * 16 + i16 >= n) << 31;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s0 = (g * U + u) * KS, blk0 = s0 / B;
+      const int span = (m - blk0) < kSpan ? (m - blk0) : kSpan;
+      const __amdgpu_buffer_rsrc_t vr = buf_rsrc_uniform(V + size_t(blk0) * ldv, size_t(span > 0 ? span : 0) * ldv * sizeof(S));
+      const int k0 = s0 + VEC * q4;
+      const unsigned koff = (unsigned(k0 / B - blk0) * ldv32 + unsigned(k0 % B)) * unsigned(sizeof(S));
+      a[u] = buf_load<VT>(vr, (koff + roff) | (unsigned(k0 >= K) << 31) | rbad);
+    }
+  };
+  auto mma_group = [&](int g, const VT (&a)[U], acc_t (&acc)[NT]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      S yu[YL];
+#pragma unroll
+      for (int w = 0; w < YL; ++w) yu[w] = ylane[(g * U + u) * 64 * YL + w];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        S av;
+        if constexpr (VEC == 1)
+          av = a[u];
+        else
+          av = a[u][v];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[j] = M::op(av, yu[v * NT + j], acc[j]);
+      }
+    }
+  };
+  // the next group after (t, g): (t, g + 1), or the first of the next tile
+  auto next_t = [&](size_t t, int g) { return g + 1 >= ng2 ? t + stride : t; };
+  auto next_g = [&](int g) { return g + 1 >= ng2 ? 0 : g + 1; };
+  size_t rt = size_t(blockIdx.x) * nwave + size_t(__builtin_amdgcn_readfirstlane(wave));
+  VT A[U], Bq[U];
+  acc_t prev[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) prev[j] = acc_t{0, 0, 0, 0};
+  size_t prev_t = ntiles; // rows past n: the first tile's "previous" stores move nothing
+  load_group(rt, 0, A);
+  for (; rt < ntiles; rt += stride) {
+    acc_t acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = acc_t{0, 0, 0, 0};
+    // first pair of groups, the previous tile's stores after the prefetch
+    load_group(rt, 1, Bq);
+    rotate_store_buf<S, B, NT>(prev, prev_t * 16, lane, NC, orc, ldv32, n);
+    __builtin_amdgcn_sched_barrier(0);
+    mma_group(0, A, acc);
+    load_group(next_t(rt, 1), next_g(1), A);
+    __builtin_amdgcn_sched_barrier(0);
+    mma_group(1, Bq, acc);
+    for (int g = 2; g < ng2; g += 2) {
+      load_group(rt, g + 1, Bq);
+      __builtin_amdgcn_sched_barrier(0);
+      mma_group(g, A, acc);
+      load_group(next_t(rt, g + 1), next_g(g + 1), A);
+      __builtin_amdgcn_sched_barrier(0);
+      mma_group(g + 1, Bq, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) prev[j] = acc[j];
+    prev_t = rt;
+  }
+  rotate_store_buf<S, B, NT>(prev, prev_t * 16, lane, NC, orc, ldv32, n);
 }
 
 inline int pick_tpw(int per_wave)
@@ -1345,36 +1616,46 @@ int launch_cholqr(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, void* W, c
 // super-steps of the short-contraction rotation (register cost 2 NSUP vectors per lane)
 constexpr int kRotShortSteps = 6;
 
+// coefficient registers per lane up to which the short rotation holds Y in registers
+constexpr int kRotYRegBytes = 192;
+
 template <typename S, int B, int VS, int NT, int NSUP>
-void launch_rotate_short_nt(trl_hip_ctx* ctx, size_t g, size_t shm, const S* v, size_t n, int m, const double* Y, int ybc, int j0, int nb, S* o)
+void launch_rotate_short_nt(trl_hip_ctx* ctx, size_t g, const S* v, size_t n, int m, const double* Y, int ybc, int j0, int nb, S* o)
 {
-  hipLaunchKernelGGL((k_rotate_short<S, B, NT, VS, NSUP>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+  constexpr bool yreg = NSUP * VS * NT * int(sizeof(S)) <= kRotYRegBytes;
+  const size_t shm = size_t(NSUP) * 64 * VS * NT * sizeof(S);
+  // FP32 with Y in registers: the five-wave register budget (FP32 sweep
+  // rotation, m = kc = 5 on the 160^3 stencil: 266 us against 272 us)
+  if constexpr (sizeof(S) == 4 && yreg)
+    hipLaunchKernelGGL((k_rotate_short_w5<S, B, NT, VS, NSUP, yreg>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
+  else
+    hipLaunchKernelGGL((k_rotate_short<S, B, NT, VS, NSUP, yreg>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n);
 }
 
 template <typename S, int B, int VS>
-int launch_rotate_short(trl_hip_ctx* ctx, int nt, int nsup, size_t g, size_t shm, const S* v, size_t n, int m, const double* Y, int ybc, int j0, int nb,
+int launch_rotate_short(trl_hip_ctx* ctx, int nt, int nsup, size_t g, const S* v, size_t n, int m, const double* Y, int ybc, int j0, int nb,
                         S* o)
 {
   auto by_steps = [&](auto NTc) {
     constexpr int NT = decltype(NTc)::value;
     switch (nsup) {
     case 1:
-      launch_rotate_short_nt<S, B, VS, NT, 1>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      launch_rotate_short_nt<S, B, VS, NT, 1>(ctx, g, v, n, m, Y, ybc, j0, nb, o);
       break;
     case 2:
-      launch_rotate_short_nt<S, B, VS, NT, 2>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      launch_rotate_short_nt<S, B, VS, NT, 2>(ctx, g, v, n, m, Y, ybc, j0, nb, o);
       break;
     case 3:
-      launch_rotate_short_nt<S, B, VS, NT, 3>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      launch_rotate_short_nt<S, B, VS, NT, 3>(ctx, g, v, n, m, Y, ybc, j0, nb, o);
       break;
     case 4:
-      launch_rotate_short_nt<S, B, VS, NT, 4>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      launch_rotate_short_nt<S, B, VS, NT, 4>(ctx, g, v, n, m, Y, ybc, j0, nb, o);
       break;
     case 5:
-      launch_rotate_short_nt<S, B, VS, NT, 5>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      launch_rotate_short_nt<S, B, VS, NT, 5>(ctx, g, v, n, m, Y, ybc, j0, nb, o);
       break;
     default:
-      launch_rotate_short_nt<S, B, VS, NT, 6>(ctx, g, shm, v, n, m, Y, ybc, j0, nb, o);
+      launch_rotate_short_nt<S, B, VS, NT, 6>(ctx, g, v, n, m, Y, ybc, j0, nb, o);
       break;
     }
   };
@@ -1396,26 +1677,64 @@ int launch_rotate_short(trl_hip_ctx* ctx, int nt, int nsup, size_t g, size_t shm
   return 0;
 }
 
+template <typename S, int B, int VS>
+void launch_rotate_long(trl_hip_ctx* ctx, int nt, size_t g, size_t shm, const S* v, size_t n, int m, const double* Y, int ybc, int j0, int nb, S* o, int ng2)
+{
+  switch (nt) {
+  case 1:
+    hipLaunchKernelGGL((k_rotate_long<S, B, 1, VS>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n, ng2);
+    break;
+  case 2:
+    hipLaunchKernelGGL((k_rotate_long<S, B, 2, VS>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n, ng2);
+    break;
+  case 3:
+    hipLaunchKernelGGL((k_rotate_long<S, B, 3, VS>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n, ng2);
+    break;
+  default:
+    hipLaunchKernelGGL((k_rotate_long<S, B, 4, VS>), dim3(unsigned(g)), dim3(512), shm, ctx->stream, v, n * B, m, Y, ybc, j0, nb, o, n, ng2);
+    break;
+  }
+}
+
 int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, const void* V, const double* Y, int ybc, int kc, void* Out)
 {
   TRL_DISPATCH_T(t, {
     TRL_DISPATCH_B(b, {
       const int K = m * B;
-      // output blocks per launch: NT*16 >= nb*B, LDS = K * NT*16 * sizeof(S) <= 144 KiB
+      constexpr int VS = rot_vec<S, B>();
+      const int nsup = (K + 4 * VS - 1) / (4 * VS);
+      // a contraction that 16-byte super-steps would pad (K = 40 at B = 8 in
+      // FP32: 48) goes in 8-byte super-steps that divide it (5 of 8)
+      constexpr int VS2 = (VS == 4 && B % 2 == 0) ? 2 : VS;
+      const bool exact2 = VS2 != VS && K % (4 * VS) != 0 && K % (4 * VS2) == 0 && K / (4 * VS2) <= kRotShortSteps;
+      // buffer resources below 2^31 bytes: the Out blocks; V (short form), or
+      // the blocks one super-step spans (long form)
+      const size_t blk_bytes = n * B * sizeof(S);
+      const size_t lim = size_t(1) << 31;
+      const bool out_ok = !ctx->force_offsets64 && size_t(kc) * blk_bytes < lim;
+      const bool short_k = out_ok && size_t(m) * blk_bytes < lim && (exact2 || nsup <= kRotShortSteps);
+      // the long form's groups of kRotGroupK k-steps come in pairs: K padded to 2 kRotGroupK
+      const int ng2 = 2 * ((K + 2 * kRotGroupK - 1) / (2 * kRotGroupK));
+      const bool long_k = out_ok && size_t((4 * VS + B - 1) / B + 1) * blk_bytes < lim && !short_k;
+      const int kpad = long_k ? ng2 * kRotGroupK : short_k ? 4 * (exact2 ? VS2 * (K / (4 * VS2)) : VS * nsup) : K;
+      // output blocks per launch: NT*16 >= nb*B, LDS = kpad * NT*16 * sizeof(S) <= 144 KiB
       const size_t lds_cap = ctx->max_lds < size_t(144 * 1024) ? ctx->max_lds : size_t(144 * 1024);
-      int nt_fit = int(lds_cap / (size_t(K) * 16 * sizeof(S)));
+      int nt_fit = int(lds_cap / (size_t(kpad) * 16 * sizeof(S)));
       if (nt_fit < 1) return TRL_HIP_EINVAL;
       if (nt_fit > 4) nt_fit = 4;
       int nb_max = (nt_fit * 16) / B;
       if (nb_max < 1) return TRL_HIP_EINVAL;
       size_t g = (n + 127) / 128;
-      // four 512-lane workgroups per CU (two: 3.60 TB/s instead of 3.98 for the
-      // FP32 sweep rotation, m = kc = 5, on the 160^3 stencil)
+      // 512-lane workgroups per CU, measured on the 160^3 stencil (b = 8):
+      // the short form 2 in FP32 and 1 in FP64 (4: 287 against 272 us, 534
+      // against 516 us), the long form 2 (restart, m = 32: 2.14 ms, 1 or 4:
+      // 2.17-2.18 ms), the fallback form 4
       const size_t gmax = size_t(4) * ctx->num_cus;
-      // FP64 with many basis blocks (the Lanczos restart, m = 32) takes one
-      // element per load: 2.74 ms against 3.16 ms for two (their register
-      // cost lowers the occupancy of the long contraction loop)
+      const size_t gmax_short = size_t(sizeof(S) == 4 ? 2 : 1) * ctx->num_cus, gmax_long = size_t(2) * ctx->num_cus;
+      // fallback form, FP64 with many basis blocks: one element per load
+      // (their register cost lowers the occupancy of its contraction loop)
       const bool v1 = sizeof(S) == 8 && m >= 8;
+      const size_t g_short = g < gmax_short ? g : gmax_short, g_long = g < gmax_long ? g : gmax_long;
       if (g > gmax) g = gmax;
       for (int j0 = 0; j0 < kc; j0 += nb_max) {
         const int nb = (kc - j0) < nb_max ? (kc - j0) : nb_max;
@@ -1424,11 +1743,16 @@ int launch_rotate(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, int m, con
         const S* v = static_cast<const S*>(V);
         S* o = static_cast<S*>(Out) + size_t(j0) * n * B;
         // short contractions: the whole operand row of a tile in registers, next tile prefetched
-        constexpr int VS = rot_vec<S, B>();
-        const int nsup = (K + 4 * VS - 1) / (4 * VS);
-        if (!v1 && nsup <= kRotShortSteps) {
-          const int rc = launch_rotate_short<S, B, VS>(ctx, nt, nsup, g, shm, v, n, m, Y, ybc, j0, nb, o);
+        if (short_k) {
+          const int rc = exact2 ? launch_rotate_short<S, B, VS2>(ctx, nt, K / (4 * VS2), g_short, v, n, m, Y, ybc, j0, nb, o)
+                                : launch_rotate_short<S, B, VS>(ctx, nt, nsup, g_short, v, n, m, Y, ybc, j0, nb, o);
           if (rc) return rc;
+          continue;
+        }
+        // long contractions: groups of k-steps streamed through two register sets
+        if (long_k) {
+          launch_rotate_long<S, B, VS>(ctx, nt, g_long, size_t(ng2) * kRotGroupK * nt * 16 * sizeof(S), v, n, m, Y, ybc, j0, nb, o, ng2);
+          TRL_HIP_LAUNCH_CHECK();
           continue;
         }
         switch (nt) {

@@ -18,6 +18,7 @@ Kernels (--what, comma separated):
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
   rotate32, rotate64    the Ritz rotation of a refinement sweep (m = kc = --panels blocks)
   restart64             the FP64 Lanczos restart rotation (m = 32 blocks, kc = 5)
+  restart32             the FP32 phase-1 restart rotation (m = 24 blocks, kc = 5)
   cholqr32, cholqr64    CholQR2 of one FP32 panel, CholQR of one FP64 panel
   copy                  a plain FP64 copy kernel of 512 MiB (the counter calibration)
 Prints, per kernel, the mean time per launch (one HIP event pair on the
@@ -176,14 +177,14 @@ def main(argv=None) -> int:
             else:
                 Cs = ctx.to_device(np.zeros(k * b * b))  # W unchanged: W -= V 0
                 timed(w, lambda: L.check(H.trl_hip_update(ctx.h, L.F32, n, b, k, V.ptr, Cs.ptr, W), "update"), (k + 2) * P_)
-        elif w in ("rotate32", "rotate64", "restart64"):
+        elif w in ("rotate32", "rotate64", "restart64", "restart32"):
             # the Ritz rotation Out = V[0:m] Y: a refinement sweep's (m = kc =
-            # --panels) in FP32 / FP64, and the FP64 Lanczos restart's (m = 32
-            # blocks, kc = 5)
-            f32 = w == "rotate32"
+            # --panels) in FP32 / FP64, the FP64 Lanczos restart's (m = 32
+            # blocks, kc = 5) and the FP32 phase-1 restart's (m = 24, kc = 5)
+            f32 = w in ("rotate32", "restart32")
             dt = np.float32 if f32 else np.float64
             s_ = 4 if f32 else 8
-            m, kc = (32, 5) if w == "restart64" else (a.panels, a.panels)
+            m, kc = {"restart64": (32, 5), "restart32": (24, 5)}.get(w, (a.panels, a.panels))
             V = ctx.to_device(rng.standard_normal((m, n, b)).astype(dt))
             Yd = ctx.to_device(rng.standard_normal((m * b, kc * b)))
             Out = ctx.empty((kc, n, b), dt)
