@@ -296,4 +296,80 @@ __device__ inline void two_sum(T a, T b, T& s, T& e)
   e = (a - (s - bb)) + (b - bb);
 }
 
+// Range-checked buffer accesses (raw buffer resource, no stride): an access
+// at a byte offset at or past the resource's size reads zeros and writes
+// nothing, so edge lanes need no branch around their loads and stores and
+// the compiler can count the loads and stores a wave has in flight exactly.
+constexpr int kBufFlags = 0x00020000;     // resource word 3 on gfx9-class parts
+
+template <int BYTES>
+struct BufIO;
+template <>
+struct BufIO<16> {
+  using T = unsigned __attribute__((ext_vector_type(4)));
+  template <int AUX = 0>
+  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off)
+  {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX);
+  }
+  template <int AUX = 0>
+  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off)
+  {
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, AUX);
+  }
+};
+template <>
+struct BufIO<8> {
+  using T = unsigned __attribute__((ext_vector_type(2)));
+  template <int AUX = 0>
+  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off)
+  {
+    return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX);
+  }
+  template <int AUX = 0>
+  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off)
+  {
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, AUX);
+  }
+};
+template <>
+struct BufIO<4> {
+  using T = unsigned;
+  template <int AUX = 0>
+  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off)
+  {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX);
+  }
+  template <int AUX = 0>
+  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off)
+  {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, AUX);
+  }
+};
+
+template <typename T>
+__device__ inline T buf_load(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+  return __builtin_bit_cast(T, BufIO<sizeof(T)>::load(r, off));
+}
+template <typename T>
+__device__ inline void buf_store(const T& v, __amdgpu_buffer_rsrc_t r, unsigned off)
+{
+  BufIO<sizeof(T)>::store(__builtin_bit_cast(typename BufIO<sizeof(T)>::T, v), r, off);
+}
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, size_t bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), int(bytes), kBufFlags);
+}
+// The same for a base and size that are the same in every lane but computed
+// where the compiler cannot tell: read from the first lane, so the resource
+// sits in scalar registers (no per-lane loop around the access).
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc_uniform(const void* p, size_t bytes)
+{
+  const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(a)), hi = __builtin_amdgcn_readfirstlane(unsigned(a >> 32));
+  const int nb = __builtin_amdgcn_readfirstlane(int(bytes));
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>((static_cast<unsigned long long>(hi) << 32) | lo), short(0), nb, kBufFlags);
+}
+
 } // namespace trl_hip_dev

@@ -197,6 +197,136 @@ __global__ __launch_bounds__(kThreads) void k_gram(const S* __restrict__ V, size
   }
 }
 
+// The Gram through range-checked buffer loads, row groups in two register
+// sets: the loads of the next group are issued (kept ahead by a scheduling
+// barrier) before the products of the current one.  Rows past n, padding
+// columns and tiles past k read zeros through offsets past their resource
+// (the W and md panels, and each tile's BPT blocks of V, below 2^31 bytes:
+// checked by the launcher), so no lane branches around a load and groups
+// past the last one add exact zeros.  Same tiles, row groups, accumulation
+// order and partial-sum layout as k_gram.
+template <typename S, int B, int TPW>
+__global__ __launch_bounds__(kThreads) void k_gram_buf(const S* __restrict__ V, size_t ldv, int k, const S* __restrict__ W, const S* __restrict__ md,
+                                                       size_t n, int TG, double* __restrict__ partials, const int* __restrict__ gate)
+{
+  if (gate && *gate == 0) return;
+  using M = Mfma<S>;
+  using acc_t = typename M::acc_t;
+  constexpr int BP = pad_b(B), BPT = 16 / BP;
+  constexpr unsigned kBad = 1u << 31;
+  const int ntile = (k + BPT - 1) / BPT;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int RS = 4 / TG;
+  const int tg = wave % TG, rs = wave / TG;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const int bit = i16 / BP, col = i16 % BP;
+  const unsigned ldv32 = unsigned(ldv);
+
+  __amdgpu_buffer_rsrc_t vr[TPW];
+  unsigned aofs[TPW], abad[TPW];
+  bool aval[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    const int t = tg + TG * q;
+    const int j0 = t * BPT, j = j0 + bit;
+    aval[q] = (t < ntile) && (j < k) && (col < B);
+    const int span = t < ntile ? ((k - j0) < BPT ? (k - j0) : BPT) : 0;
+    vr[q] = buf_rsrc_uniform(V + size_t(t < ntile ? j0 : 0) * ldv, size_t(span) * ldv * sizeof(S));
+    aofs[q] = (unsigned(bit) * ldv32 + unsigned(col)) * unsigned(sizeof(S));
+    abad[q] = aval[q] ? 0u : kBad;
+  }
+  const bool wval = i16 < B;
+  const __amdgpu_buffer_rsrc_t wr = buf_rsrc(W, n * B * sizeof(S));
+  const __amdgpu_buffer_rsrc_t mr = buf_rsrc(md, md ? n * sizeof(S) : 0);
+  const unsigned wofs = unsigned(i16) * unsigned(sizeof(S)), wbad = wval ? 0u : kBad;
+
+  const size_t nquads = (n + 3) / 4;
+  const size_t ngroups = (nquads + kU - 1) / kU;
+  const size_t gstride = size_t(gridDim.x) * RS;
+
+  acc_t acc[TPW];
+  acc_t lo[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    acc[q] = acc_t{0, 0, 0, 0};
+    lo[q] = acc_t{0, 0, 0, 0};
+  }
+  // the W and V values of row group g held by this lane (zeros past n)
+  // (the md factor is applied where the products use it, so that no load
+  // waits before the next group's loads are issued)
+  auto load_group = [&](size_t g, S (&bw)[kU], S (&mv)[kU], S (&a)[kU][TPW]) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const size_t r = (g * kU + u) * 4 + q4;
+      const unsigned rbad = unsigned(r >= n) << 31;
+      const unsigned rofs = unsigned(r) * unsigned(B * sizeof(S));
+      bw[u] = buf_load<S>(wr, (rofs + wofs) | wbad | rbad);
+      mv[u] = buf_load<S>(mr, (unsigned(r) * unsigned(sizeof(S))) | rbad);
+#pragma unroll
+      for (int q = 0; q < TPW; ++q) a[u][q] = buf_load<S>(vr[q], (rofs + aofs[q]) | abad[q] | rbad);
+    }
+  };
+  auto mma_group = [&](const S (&bw0)[kU], const S (&mv)[kU], const S (&a)[kU][TPW]) {
+    S bw[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) bw[u] = md ? bw0[u] * mv[u] : bw0[u];
+    if constexpr (sizeof(S) == 8) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) acc[q] = M::op(a[u][q], bw[u], acc[q]);
+    }
+    else {
+      acc_t cur[TPW];
+#pragma unroll
+      for (int q = 0; q < TPW; ++q) cur[q] = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) cur[q] = M::op(a[u][q], bw[u], cur[q]);
+#pragma unroll
+      for (int q = 0; q < TPW; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float s, err;
+          two_sum(acc[q][e], cur[q][e], s, err);
+          acc[q][e] = s;
+          lo[q][e] += err;
+        }
+    }
+  };
+  size_t g = size_t(blockIdx.x) * RS + rs;
+  S bwA[kU], mA[kU], aA[kU][TPW], bwB[kU], mB[kU], aB[kU][TPW];
+  load_group(g, bwA, mA, aA);
+  for (; g < ngroups; g += 2 * gstride) {
+    load_group(g + gstride, bwB, mB, aB);
+    __builtin_amdgcn_sched_barrier(0);
+    mma_group(bwA, mA, aA);
+    load_group(g + 2 * gstride, bwA, mA, aA);
+    __builtin_amdgcn_sched_barrier(0);
+    mma_group(bwB, mB, aB);
+  }
+
+  const size_t NW = size_t(gridDim.x) * RS;
+  double* out = partials + size_t(blockIdx.x) * RS + rs;
+  if (!wval) return;
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    const int t = tg + TG * q;
+    if (t >= ntile) continue;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = mfma_row<S>(q4, e);
+      const int blk = t * BPT + i / BP, ci = i % BP;
+      if (blk < k && ci < B) {
+        double v = static_cast<double>(acc[q][e]);
+        if constexpr (sizeof(S) == 4) v += static_cast<double>(lo[q][e]);
+        out[((size_t(blk) * B + ci) * B + i16) * NW] = v;
+      }
+    }
+  }
+}
+
 // out[p] = sum_{w < nw} partials[p*nw + w] (entry-major), summed in a fixed
 // order (deterministic): one workgroup per entry, each lane strides over w
 // with four accumulators, then a butterfly per wave and the waves in order.
@@ -698,58 +828,6 @@ __global__ __launch_bounds__(512) void k_rotate(const S* __restrict__ V, size_t 
   }
 }
 
-// Range-checked buffer accesses (raw buffer resource, no stride): an access
-// at a byte offset at or past the resource's size reads zeros and writes
-// nothing, so edge lanes need no branch around their loads and stores and
-// the compiler can count the loads and stores a wave has in flight exactly.
-constexpr int kBufFlags = 0x00020000;     // resource word 3 on gfx9-class parts
-
-template <int BYTES>
-struct BufIO;
-template <>
-struct BufIO<16> {
-  using T = unsigned __attribute__((ext_vector_type(4)));
-  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off) { return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0); }
-  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off) { __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0); }
-};
-template <>
-struct BufIO<8> {
-  using T = unsigned __attribute__((ext_vector_type(2)));
-  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off) { return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0); }
-  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off) { __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0); }
-};
-template <>
-struct BufIO<4> {
-  using T = unsigned;
-  __device__ static T load(__amdgpu_buffer_rsrc_t r, unsigned off) { return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0); }
-  __device__ static void store(T v, __amdgpu_buffer_rsrc_t r, unsigned off) { __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0); }
-};
-
-template <typename T>
-__device__ inline T buf_load(__amdgpu_buffer_rsrc_t r, unsigned off)
-{
-  return __builtin_bit_cast(T, BufIO<sizeof(T)>::load(r, off));
-}
-template <typename T>
-__device__ inline void buf_store(const T& v, __amdgpu_buffer_rsrc_t r, unsigned off)
-{
-  BufIO<sizeof(T)>::store(__builtin_bit_cast(typename BufIO<sizeof(T)>::T, v), r, off);
-}
-__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, size_t bytes)
-{
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), int(bytes), kBufFlags);
-}
-// The same for a base and size that are the same in every lane but computed
-// where the compiler cannot tell: read from the first lane, so the resource
-// sits in scalar registers (no per-lane loop around the access).
-__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc_uniform(const void* p, size_t bytes)
-{
-  const unsigned long long a = reinterpret_cast<unsigned long long>(p);
-  const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(a)), hi = __builtin_amdgcn_readfirstlane(unsigned(a >> 32));
-  const int nb = __builtin_amdgcn_readfirstlane(int(bytes));
-  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>((static_cast<unsigned long long>(hi) << 32) | lo), short(0), nb, kBufFlags);
-}
-
 // rotate_store through a buffer resource over Out (element offsets from its
 // base; rows past n and columns past NC move nothing)
 template <typename S, int B, int NT>
@@ -1056,6 +1134,10 @@ inline int pick_tpw(int per_wave)
 // partials: where the rows go; nullptr = the context's growable buffer
 // (ctx->partials, enlarged here when needed), else a fixed buffer of cap
 // doubles, for which the workgroup count is capped so that the rows fit.
+// tiles per wave up to which the Gram takes its buffer form (two register
+// sets of kU * (TPW + 2) values per lane)
+constexpr int kGramBufTpw = 4;
+
 template <typename S, int B>
 int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* W, const S* md, int* nw, const int* gate, double* partials = nullptr,
                         size_t cap = 0)
@@ -1084,35 +1166,48 @@ int queue_gram_partials(trl_hip_ctx* ctx, size_t n, int k, const S* V, const S* 
   }
   const size_t ldv = n * B;
   const dim3 grid{unsigned(g)}, blk{unsigned(kThreads)};
+  // buffer form (FP32: 16 blocks of the 160^3 stencil, b = 8, 485 -> 440 us;
+  // FP64 over 32 blocks 1.56 ms either way, kept on k_gram): W (and md) and
+  // each tile's BPT blocks of V below 2^31 bytes
+  const size_t lim = size_t(1) << 31;
+  const bool buf = sizeof(S) == 4 && tpw <= kGramBufTpw && size_t(BPT) * ldv * sizeof(S) < lim && ldv * sizeof(S) < lim;
+#define TRL_GRAM(TPW_)                                                                                                                             \
+  do {                                                                                                                                             \
+    if (buf)                                                                                                                                       \
+      hipLaunchKernelGGL((k_gram_buf<S, B, TPW_>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);                            \
+    else                                                                                                                                           \
+      hipLaunchKernelGGL((k_gram<S, B, TPW_>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);                                \
+  } while (0)
   switch (tpw) {
   case 1:
-    hipLaunchKernelGGL((k_gram<S, B, 1>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(1);
     break;
   case 2:
-    hipLaunchKernelGGL((k_gram<S, B, 2>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(2);
     break;
   case 3:
-    hipLaunchKernelGGL((k_gram<S, B, 3>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(3);
     break;
   case 4:
-    hipLaunchKernelGGL((k_gram<S, B, 4>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(4);
     break;
   case 5:
-    hipLaunchKernelGGL((k_gram<S, B, 5>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(5);
     break;
   case 6:
-    hipLaunchKernelGGL((k_gram<S, B, 6>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(6);
     break;
   case 8:
-    hipLaunchKernelGGL((k_gram<S, B, 8>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(8);
     break;
   case 12:
-    hipLaunchKernelGGL((k_gram<S, B, 12>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(12);
     break;
   default:
-    hipLaunchKernelGGL((k_gram<S, B, 16>), grid, blk, 0, ctx->stream, V, ldv, k, W, md, n, TG, partials, gate);
+    TRL_GRAM(16);
     break;
   }
+#undef TRL_GRAM
   TRL_HIP_LAUNCH_CHECK();
   *nw = int(g * RS);
   return 0;

@@ -16,6 +16,7 @@ Kernels (--what, comma separated):
   forced32              that FP32 step plus the forcing term F diag(g) (the residual-driven
                         filter of the FP64 sweeps)
   gram32, update32      the block Gram-Schmidt products over k = 16 blocks (FP32)
+  gram64, update64      the same over k = 32 blocks in FP64 (a full basis of the plain solve)
   rotate32, rotate64    the Ritz rotation of a refinement sweep (m = kc = --panels blocks)
   restart64             the FP64 Lanczos restart rotation (m = 32 blocks, kc = 5)
   restart32             the FP32 phase-1 restart rotation (m = 24 blocks, kc = 5)
@@ -165,18 +166,22 @@ def main(argv=None) -> int:
                 L.check(H.trl_hip_spmm_axpby_forced(ctx.h, L.F32, n, b, C.byref(d), npan, X.ptr, Y.ptr, Y.ptr, 0.5, -0.25, 0.125, F.ptr, g.ptr), "forced")
 
             timed(f"{w}[{op._name(d)}]", fn, mat + 4 * npan * n * b * 4)
-        elif w in ("gram32", "update32"):
-            k = 16
-            V = ctx.to_device(rng.standard_normal((k + 1, n, b)).astype(np.float32))
+        elif w in ("gram32", "update32", "gram64", "update64"):
+            # FP32: the sweeps' k = 16; FP64: the plain Lanczos reorthogonalisation
+            # against a full basis of k = 32 blocks
+            f32 = w.endswith("32")
+            dt, code, s_ = (np.float32, L.F32, 4) if f32 else (np.float64, L.F64, 8)
+            k = 16 if f32 else 32
+            V = ctx.to_device(rng.standard_normal((k + 1, n, b)).astype(dt))
             Cd = ctx.empty((k * b * b,))
             W = V.offset(k * n * b)
-            P_ = n * b * 4
-            L.check(H.trl_hip_gram(ctx.h, L.F32, n, b, k, V.ptr, W, None, Cd.ptr), "gram")
-            if w == "gram32":
-                timed(w, lambda: L.check(H.trl_hip_gram(ctx.h, L.F32, n, b, k, V.ptr, W, None, Cd.ptr), "gram"), (k + 1) * P_)
+            P_ = n * b * s_
+            L.check(H.trl_hip_gram(ctx.h, code, n, b, k, V.ptr, W, None, Cd.ptr), "gram")
+            if w.startswith("gram"):
+                timed(w, lambda: L.check(H.trl_hip_gram(ctx.h, code, n, b, k, V.ptr, W, None, Cd.ptr), "gram"), (k + 1) * P_)
             else:
                 Cs = ctx.to_device(np.zeros(k * b * b))  # W unchanged: W -= V 0
-                timed(w, lambda: L.check(H.trl_hip_update(ctx.h, L.F32, n, b, k, V.ptr, Cs.ptr, W), "update"), (k + 2) * P_)
+                timed(w, lambda: L.check(H.trl_hip_update(ctx.h, code, n, b, k, V.ptr, Cs.ptr, W), "update"), (k + 2) * P_)
         elif w in ("rotate32", "rotate64", "restart64", "restart32"):
             # the Ritz rotation Out = V[0:m] Y: a refinement sweep's (m = kc =
             # --panels) in FP32 / FP64, the FP64 Lanczos restart's (m = 32
