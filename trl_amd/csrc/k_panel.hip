@@ -214,13 +214,18 @@ __global__ __launch_bounds__(kThreads) void k_combine(double* __restrict__ X, co
   }
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partials, int nwg, int P, double* __restrict__ out)
+// out[p] = sum over w < nwg of partials[w * P + p]: one wave per column p
+// (workgroup p), lanes over the workgroups' partials in a fixed order, then a
+// butterfly: every load of the column in flight at once instead of nwg
+// dependent steps of one lane.
+__global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ partials, int nwg, int P, double* __restrict__ out)
 {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+  const int p = blockIdx.x, lane = threadIdx.x;
   double s = 0.0;
-  for (int w = 0; w < nwg; ++w) s += partials[size_t(w) * P + p];
-  out[p] = s;
+  for (int w = lane; w < nwg; w += 64) s += partials[size_t(w) * P + p];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[p] = s;
 }
 
 } // namespace
@@ -352,7 +357,7 @@ int launch_residual_sq(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const
     });
   });
   TRL_HIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, ctx->stream, part, nwg, P, res);
+  hipLaunchKernelGGL(k_sum_partials, dim3(unsigned(P)), dim3(64), 0, ctx->stream, part, nwg, P, res);
   TRL_HIP_LAUNCH_CHECK();
   TRL_HIP_TRY(hipMemcpyAsync(out, res, P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
