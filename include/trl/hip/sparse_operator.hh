@@ -87,6 +87,19 @@ public:
     panels_n_ = 0;
   }
 
+  // Takes the blocked row order (plane, plane_block) of another operator's
+  // descriptor for the one-panel forms, untimed: an FP64 operator that adopted
+  // an FP32 operator's choice, once the FP32 panel tune has found blocks of a
+  // plane (FP64 rows, twice as wide, gain from them where FP32 rows in index
+  // order do not: 175 against 207 us for the one-panel FP64 product of the
+  // 160^3 stencil).  No change for a descriptor in index order.
+  void adopt_row_order(const trl_hip_spmat& like)
+  {
+    if (n_ == 0 || like.plane <= 0) return;
+    op_.plane = op_axpby_.plane = like.plane;
+    op_.plane_block = op_axpby_.plane_block = like.plane_block;
+  }
+
   // After tune(b): times the axpby form on npanels panels at once (the
   // matrix read once per launch) for axpby_panels.  Returns the time per
   // launch in microseconds.

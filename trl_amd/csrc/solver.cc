@@ -976,6 +976,7 @@ int trl_hip_solve_csr_mixed(trl_hip_ctx* ctx, size_t n, const int32_t* rowptr, c
       if (mp->refine == 1) {
         // ---- FP64 phase: block refinement of the FP32 Ritz blocks -----------
         if (p->spmm_tune && !rfil) tune_panels_of(*op64, B, nref);
+        if (p->spmm_tune && rfil && op32) op64->adopt_row_order(op32->panels_descriptor());
         std::unique_ptr<ResidualFilter<B>> rfilt;
         if (rfil) rfilt = std::make_unique<ResidualFilter<B>>(c, n, nref, *op32, mp->cheb_degree);
         BlockRefiner<double, B> rf(c, n, *op64, mass64, nref, asc, eta);
