@@ -68,6 +68,21 @@ void true_residuals(const Context& c, std::size_t n, const SparseOperator<S>& op
   }
 }
 
+// The same residuals from products A x already formed (FP64 panels X and
+// AX, ceil(nev/B) of them used), standard problems only.
+template <unsigned B>
+void residuals_from_products(const Context& c, std::size_t n, const double* X, const double* AX, unsigned nev, const double* evals, double* out)
+{
+  const unsigned nblk = (nev + B - 1) / B;
+  for (unsigned j = 0; j < nblk; ++j) {
+    double lam[16] = {0.0};
+    for (unsigned q = 0; q < B && j * B + q < nev; ++q) lam[q] = evals[j * B + q];
+    double sums[32];
+    throw_on(trl_hip_residual_sq(c.get(), TRL_HIP_F64, n, B, X + std::size_t(j) * n * B, AX + std::size_t(j) * n * B, nullptr, lam, sums), "residual_sq");
+    for (unsigned q = 0; q < B && j * B + q < nev; ++q) out[j * B + q] = std::sqrt(sums[q]) / std::max(std::sqrt(sums[B + q]), 1e-300);
+  }
+}
+
 // The operator of one solve: the CSR arrays with the caller's kernel
 // variant, or the fastest kernel and storage for block size B when tuning
 // is asked for (the fused axpby form too unless axpby is false).
@@ -352,8 +367,17 @@ struct ResidualFilter {
   // Filters the Ritz panels rf.X (Ritz values theta, in panel order) with
   // degree deg[j] on panel j; the degrees are non-decreasing.  Returns the
   // operator applications (FP64 and FP32).
+  // The products A x of all panels, for the true-residual check and the
+  // filter's residuals alike (standard problems: no row scaling, so the
+  // filter's operator and the check's are the same); filter(..., true) then
+  // takes them instead of forming its own.
+  void form_ax(const BlockRefiner<double, B>& rf, const SparseOperator<double>& op64)
+  {
+    for (unsigned j = 0; j < nb; ++j) op64.apply(int(B), rf.panel(rf.X, j), panel(AX.data(), n, j));
+  }
+
   std::size_t filter(BlockRefiner<double, B>& rf, const SparseOperator<double>& op64, const std::vector<unsigned>& deg, const std::vector<double>& theta,
-                     double lo, double hi, double a0)
+                     double lo, double hi, double a0, bool ax_formed = false)
   {
     const double e = 0.5 * (hi - lo), cc = 0.5 * (hi + lo);
     const double s1 = e / (a0 - cc);
@@ -395,11 +419,12 @@ struct ResidualFilter {
     throw_on(trl_hip_memcpy_h2d(c.get(), table.data(), host.data(), host.size() * sizeof(double)), "residual filter table");
     const unsigned np1 = nb - j1;
     // r = A x - theta x in FP64 for the filtered panels; R = r, E_1 = alpha_1 r in FP32
-    for (unsigned j = j1; j < nb; ++j) op64.apply(int(B), rf.panel(rf.X, j), panel(AX.data(), n, j));
+    if (!ax_formed)
+      for (unsigned j = j1; j < nb; ++j) op64.apply(int(B), rf.panel(rf.X, j), panel(AX.data(), n, j));
     throw_on(trl_hip_residual_split(c.get(), n, int(B), int(np1), panel(AX.data(), n, j1), rf.panel(rf.X, j1), table.data() + std::size_t(j1) * B, al[1],
                                     panel(R.data(), n, j1), panel(Eb.data(), n, j1)),
              "residual split");
-    std::size_t applies = np1;
+    std::size_t applies = ax_formed ? 0 : np1;
     // steps 2..dmax on the panels still being filtered (a suffix): step i
     // writes Ea for even i, Eb for odd i, and reads the other as E_{i-1};
     // the older iterate E_{i-2} is the output buffer (E_0 = 0: none at i = 2)
@@ -497,7 +522,17 @@ bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, cons
   for (;;) {
     rf.orthonormalize();
     rf.rayleigh_ritz(theta);
-    true_residuals<S, B>(c, n, op, mass, rf.X, nev, theta.data(), rs.data(), rf.KX.data());
+    bool ax_formed = false;
+    if constexpr (std::is_same_v<S, double>) {
+      if (rfilt && !mass && !op.descriptor().dscale) {
+        // one A x per panel serves the true residuals and the filter's
+        // residual split (which would otherwise form them again)
+        rfilt->form_ax(rf, op);
+        residuals_from_products<B>(c, n, rf.X, rfilt->AX.data(), nev, theta.data(), rs.data());
+        ax_formed = true;
+      }
+    }
+    if (!ax_formed) true_residuals<S, B>(c, n, op, mass, rf.X, nev, theta.data(), rs.data(), rf.KX.data());
     double worst = 0.0;
     for (double r : rs) worst = std::max(worst, r);
     if (std::getenv("TRL_HIP_VERBOSE")) {
@@ -524,7 +559,7 @@ bool refine_sweeps(BlockRefiner<S, B>& rf, const Context& c, std::size_t n, cons
     const double flo = rf.ascending ? far : lo_all - margin, fhi = rf.ascending ? hi_all + margin : far;
     if constexpr (std::is_same_v<S, double>) {
       if (rfilt) {
-        applies += rfilt->filter(rf, op, deg, theta, flo, fhi, theta.front());
+        applies += rfilt->filter(rf, op, deg, theta, flo, fhi, theta.front(), ax_formed);
         rs_prev = rs;
         deg_prev = deg;
         continue;
