@@ -84,11 +84,21 @@ int trl_hip_ctx_num_cus(trl_hip_ctx* ctx);
  *   few microseconds per block step) the replay is no faster: the device,
  *   not the host, paces such steps. */
 #define TRL_HIP_OPT_GRAPHS 2
+/*   TRL_HIP_OPT_ALLOC_CACHE, value 1 (default): trl_hip_free keeps a block
+ *   of trl_hip_malloc for reuse by a later allocation of the same size on the
+ *   context (zero-filled again; reuse is ordered on the context stream, so
+ *   the free neither synchronises nor calls hipFree); the blocks, at most 64
+ *   GiB, are released when an allocation fails, at trl_hip_ctx_destroy, or
+ *   when the option is set to 0. */
+#define TRL_HIP_OPT_ALLOC_CACHE 3
 int trl_hip_ctx_set_option(trl_hip_ctx* ctx, int option, int value);
 int trl_hip_ctx_get_option(trl_hip_ctx* ctx, int option, int* value);
 const char* trl_hip_strerror(int code);
 
-/* Zero-initialised device allocation (multivector.hh:44-48 zero-fills). */
+/* Zero-initialised device allocation (multivector.hh:44-48 zero-fills).
+ * trl_hip_free of a block from trl_hip_malloc returns it to the context's
+ * cache (TRL_HIP_OPT_ALLOC_CACHE); other pointers are freed after the
+ * context stream has finished. */
 int trl_hip_malloc(trl_hip_ctx* ctx, size_t bytes, void** out);
 int trl_hip_free(trl_hip_ctx* ctx, void* p);
 /* Synchronous with respect to the host: return after the copy. */

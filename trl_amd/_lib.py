@@ -19,6 +19,7 @@ F64 = 1
 # trl_hip_ctx_set_option options
 OPT_OFFSETS64 = 1
 OPT_GRAPHS = 2
+OPT_ALLOC_CACHE = 3
 
 c_size = C.c_size_t
 c_int = C.c_int

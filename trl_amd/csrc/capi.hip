@@ -176,11 +176,29 @@ int trl_hip_ctx_create(int device, trl_hip_ctx** out)
   return 0;
 }
 
+namespace {
+
+// bytes of freed blocks a context keeps for reuse at most
+constexpr size_t kAllocCacheCap = size_t(64) << 30;
+
+// Releases the cached blocks (after the work queued on them has finished).
+void flush_alloc_cache(trl_hip_ctx* ctx)
+{
+  if (ctx->cached.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->cached) (void)hipFree(kv.second);
+  ctx->cached.clear();
+  ctx->cached_bytes = 0;
+}
+
+} // namespace
+
 int trl_hip_ctx_destroy(trl_hip_ctx* ctx)
 {
   if (!ctx) return 0;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  flush_alloc_cache(ctx);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->flags) (void)hipFree(ctx->flags);
   if (ctx->small) (void)hipFree(ctx->small);
@@ -236,6 +254,11 @@ int trl_hip_ctx_set_option(trl_hip_ctx* ctx, int option, int value)
   case TRL_HIP_OPT_GRAPHS:
     ctx->graphs = value ? 1 : 0;
     return 0;
+  case TRL_HIP_OPT_ALLOC_CACHE:
+    TRL_HIP_NO_CAPTURE(ctx);
+    ctx->alloc_cache = value ? 1 : 0;
+    if (!value) flush_alloc_cache(ctx);
+    return 0;
 
   default:
     return TRL_HIP_EINVAL;
@@ -252,10 +275,14 @@ int trl_hip_ctx_get_option(trl_hip_ctx* ctx, int option, int* value)
   case TRL_HIP_OPT_GRAPHS:
     *value = ctx->graphs;
     return 0;
+  case TRL_HIP_OPT_ALLOC_CACHE:
+    *value = ctx->alloc_cache;
+    return 0;
   default:
     return TRL_HIP_EINVAL;
   }
 }
+
 
 int trl_hip_malloc(trl_hip_ctx* ctx, size_t bytes, void** out)
 {
@@ -265,12 +292,33 @@ int trl_hip_malloc(trl_hip_ctx* ctx, size_t bytes, void** out)
   if (bytes == 0) bytes = 1;
   const size_t aligned = (bytes + 255) & ~size_t(255);
   void* p = nullptr;
-  if (hipMalloc(&p, aligned) != hipSuccess) return TRL_HIP_ENOMEM;
+  // a freed block of the same size, reused in stream order (a hipMalloc of
+  // hundreds of MB and its hipFree cost ~0.05 and ~0.3 ms of host time, and
+  // the free a synchronisation, inside every solve that allocates scratch)
+  if (ctx->alloc_cache) {
+    auto it = ctx->cached.find(aligned);
+    if (it != ctx->cached.end()) {
+      p = it->second;
+      ctx->cached.erase(it);
+      ctx->cached_bytes -= aligned;
+    }
+  }
+  if (!p && hipMalloc(&p, aligned) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+    flush_alloc_cache(ctx); // the cached blocks may be what is missing
+    if (hipMalloc(&p, aligned) != hipSuccess) {
+      (void)hipGetLastError();
+      return TRL_HIP_ENOMEM;
+    }
+  }
   int rc = launch_fill_zero(ctx, p, aligned);
   if (rc) {
+    (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(p);
     return rc;
   }
+  ctx->live_sizes[p] = aligned;
   *out = p;
   return 0;
 }
@@ -280,6 +328,16 @@ int trl_hip_free(trl_hip_ctx* ctx, void* p)
   if (!ctx) return TRL_HIP_EINVAL;
   if (!p) return 0;
   TRL_HIP_NO_CAPTURE(ctx);
+  auto it = ctx->live_sizes.find(p);
+  if (it != ctx->live_sizes.end()) {
+    const size_t sz = it->second;
+    ctx->live_sizes.erase(it);
+    if (ctx->alloc_cache && ctx->cached_bytes + sz <= kAllocCacheCap) {
+      ctx->cached.emplace(sz, p);
+      ctx->cached_bytes += sz;
+      return 0;
+    }
+  }
   TRL_HIP_TRY(hipStreamSynchronize(ctx->stream));
   TRL_HIP_TRY(hipFree(p));
   return 0;

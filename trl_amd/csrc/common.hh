@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -35,6 +36,13 @@ struct trl_hip_ctx {
   int capturing = 0, capture_refused = 0;
   // TRL_HIP_OPT_GRAPHS: drivers replay captured restart cycles
   int graphs = 0;
+  // device allocations of trl_hip_malloc (TRL_HIP_OPT_ALLOC_CACHE): the sizes
+  // of live blocks, and freed blocks kept for reuse by size (their reuse is
+  // ordered on the context stream, so a free needs no synchronisation)
+  int alloc_cache = 1;
+  std::unordered_map<void*, size_t> live_sizes;
+  std::multimap<size_t, void*> cached;
+  size_t cached_bytes = 0;
   // flags[0]: sticky Cholesky breakdown pivot (1-based, 0 = none); [8]:
   // CholQR arrival counter; [16]: csr_max_row result; [24]: second-pass gate
   int* flags = nullptr;
