@@ -754,19 +754,42 @@ int spmm_tune_impl(trl_hip_ctx* ctx, trl_hip_dtype t, size_t n, int b, const trl
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = TRL_HIP_ENOMEM;
   double best_t = 1e300;
-  auto time_one = [&](const trl_hip_spmat& A) -> double {
-    (void)hipGetLastError();
-    if (launch_spmm(ctx, t, n, b, A, X, Y, Z, axpby, 0.5, -0.25, 0.125, npanels)) {
-      (void)hipGetLastError();
-      return -1.0; // variant not applicable to this b / dtype
-    }
-    const int reps = npanels > 1 ? 1 : 3;
+  // Several-panel candidates and small operators: an untimed launch, then 1
+  // (several panels) or 3 timed ones.  One-panel candidates on large
+  // operators (n b >= 2^24: launches of ~0.1 ms, which differ by more than
+  // 10% between candidates): one timed launch, with no untimed one before it
+  // once a first candidate has run (the tune runs inside every solve), and 3
+  // more if it took under 50 us.
+  bool warmed = false;
+  auto timed = [&](const trl_hip_spmat& A, int reps) -> double {
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return -1.0;
     for (int r = 0; r < reps; ++r) (void)launch_spmm(ctx, t, n, b, A, X, Y, Z, axpby, 0.5, -0.25, 0.125, npanels);
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
     return 1e3 * ms / reps;
+  };
+  auto time_one = [&](const trl_hip_spmat& A) -> double {
+    (void)hipGetLastError();
+    const bool direct = npanels == 1 && warmed && n * size_t(b) >= (size_t(1) << 24);
+    if (!direct) {
+      if (launch_spmm(ctx, t, n, b, A, X, Y, Z, axpby, 0.5, -0.25, 0.125, npanels)) {
+        (void)hipGetLastError();
+        return -1.0; // variant not applicable to this b / dtype
+      }
+      warmed = true;
+      return timed(A, npanels > 1 ? 1 : 3);
+    }
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) return -1.0;
+    if (launch_spmm(ctx, t, n, b, A, X, Y, Z, axpby, 0.5, -0.25, 0.125, npanels)) {
+      (void)hipGetLastError();
+      return -1.0; // variant not applicable to this b / dtype
+    }
+    if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+    const double us = 1e3 * ms;
+    return us >= 50.0 ? us : timed(A, 3);
   };
   auto consider = [&](const trl_hip_spmat& A) -> double {
     const double us = time_one(A);
