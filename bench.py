@@ -309,7 +309,7 @@ def main(argv=None) -> int:
         print(json.dumps(cpu_oracle_runs()), flush=True)
         return 0
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported (see DESIGN.md for how the path would shard)"}))
+        print(json.dumps({"error": "this backend runs on one GPU: --gpus N > 1 is not supported"}))
         return 2
 
     from trl_amd import _lib as L
