@@ -407,11 +407,9 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
     const size_t po = size_t(p) * e.pstride;
     const S* Xp = X + po;
     S zr[BC] = {}, fr[BC] = {}, xo[BC];
-    if constexpr (AXPBY) {
-      if (Z && live) RowIO<S, BC>::template load<true>(elem<O32>(Z + po, ro), zr);
-      if constexpr (FORCE)
-        if (live) RowIO<S, BC>::template load<true>(elem<O32>(F + po, ro), fr);
-    }
+    // the own row first and the step's own streams (Z, F) last: the
+    // permutes below wait for the own row only, with the gathers and those
+    // streams still in flight (0.5-0.7% faster than Z first)
     RowIO<S, BC>::load(elem<O32>(Xp, ro), xo);
     S x[kChunk][BC];
 #pragma unroll
@@ -421,6 +419,11 @@ __global__ __launch_bounds__((panel_threads<S, B / CS>())) void k_spmm_ell_panel
         if constexpr (O32) asm volatile("" : "+v"(o));
         RowIO<S, BC>::load(elem<O32>(Xp, o), x[q]);
       }
+    }
+    if constexpr (AXPBY) {
+      if (Z && live) RowIO<S, BC>::template load<true>(elem<O32>(Z + po, ro), zr);
+      if constexpr (FORCE)
+        if (live) RowIO<S, BC>::template load<true>(elem<O32>(F + po, ro), fr);
     }
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
